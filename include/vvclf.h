@@ -1,0 +1,398 @@
+/*
+ * vvclf.h — C ABI of the MI355X (gfx950) VVC in-loop filter chain:
+ * LMCS inverse luma mapping -> deblocking -> SAO -> ALF + CC-ALF.
+ *
+ * The POD types mirror the Rust types the reference filters consume, field for
+ * field, with Rust `Option<T>` turned into a `has_*` flag or a NULL pointer and
+ * `Vec<T>` / slices turned into (pointer, count).  Every entry point returns an
+ * int: VVCLF_OK (0) on success, a negative VVCLF_ERR_* code otherwise.  The
+ * reference functions are infallible; the negative codes only report
+ * descriptors the reference would panic on, or device/runtime failures.
+ *
+ * Two layers:
+ *   1. Drop-in, synchronous, host-pointer entry points: one per reference
+ *      filter entry (vvclf_apply_deblocking[_clipped], vvclf_apply_sao[_clipped],
+ *      vvclf_apply_alf[_clipped], vvclf_lmcs_inverse_luma) plus the picture-level
+ *      driver vvclf_apply_in_loop_filters.  Planes are modified in place, like
+ *      the reference's `&mut PictureBuffer`.  Work runs on the GPU.
+ *   2. Device-resident additions with no reference counterpart: contexts, jobs
+ *      (one picture with its side information uploaded to HBM), batches (many
+ *      jobs filtered by one persistent fused launch, optionally captured in a
+ *      HIP graph) and a serving loop (continuous batching over many streams).
+ *
+ * Only 4:2:0 (chroma_format_idc 1) and 4:0:0 (chroma_format_idc 0) pictures run
+ * on the GPU; other chroma formats return VVCLF_ERR_UNSUPPORTED.
+ */
+#ifndef VVCLF_H
+#define VVCLF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VVCLF_OK 0
+#define VVCLF_ERR_INVALID (-1)     /* descriptor the reference would reject or panic on */
+#define VVCLF_ERR_HIP (-2)         /* HIP runtime error (message in vvclf_last_error) */
+#define VVCLF_ERR_NOMEM (-3)       /* host or device allocation failed */
+#define VVCLF_ERR_BUDGET (-4)      /* serving: job larger than the admission budget */
+#define VVCLF_ERR_TIMEOUT (-5)     /* serving: wait timed out */
+#define VVCLF_ERR_UNSUPPORTED (-6) /* chroma format other than 4:0:0 / 4:2:0 on the GPU */
+#define VVCLF_ERR_AGAIN (-7)       /* serving: result not ready yet */
+
+/* ------------------------------------------------------------------------ */
+/* Picture buffers — PicturePlane / PictureBuffer, src/reconstruct.rs:41-52,  */
+/* :116-145.  Row-major u16 samples, stride >= width, bit_depth 8..16.        */
+/* ------------------------------------------------------------------------ */
+typedef struct vvclf_plane {
+  uint16_t* samples; /* stride * height samples */
+  uint32_t stride;
+  uint32_t width;
+  uint32_t height;
+  uint32_t bit_depth;
+} vvclf_plane;
+
+typedef struct vvclf_picture {
+  vvclf_plane luma;
+  vvclf_plane cb;
+  vvclf_plane cr;
+} vvclf_picture;
+
+/* ------------------------------------------------------------------------ */
+/* Deblocking inputs — src/deblock.rs:52-161 (DeblockCu, TbSplit, Tu64CbfMap),*/
+/* :330-364 (DeblockMvCell, DeblockMvGrid), :376-393 (DeblockParams).         */
+/* ------------------------------------------------------------------------ */
+#define VVCLF_QP_C_LEGACY INT32_MIN /* DEBLOCK_QP_C_LEGACY, src/deblock.rs:368 */
+
+typedef struct vvclf_deblock_cu {
+  uint32_t x, y, w, h;   /* luma rectangle */
+  int32_t qp_y;
+  int32_t qp_c[3];       /* [QpCb, QpCr, QpCbCr] or VVCLF_QP_C_LEGACY in qp_c[0] */
+  uint8_t intra, tu_y_coded, tu_cb_coded, tu_cr_coded;
+  uint8_t bdpcm_luma, bdpcm_chroma, plt, joint_cbcr2;
+  uint8_t ciip, ibc;
+  uint8_t has_num_sb;    /* num_sb: Option<(u32, u32)> */
+  uint8_t has_tu64;      /* tu64: Option<Tu64CbfMap> */
+  uint32_t num_sb_x, num_sb_y;
+  uint8_t tu64_y[2][2];  /* [tile_y][tile_x] */
+  uint8_t tu64_cb[2][2];
+  uint8_t tu64_cr[2][2];
+  uint8_t has_tb_split;  /* tb_split: Option<TbSplit> */
+  uint8_t tb_vertical, tb_n_bounds, tb_luma_only;
+  uint32_t tb_bounds[3];
+  uint8_t tb_y_coded[4], tb_cb_coded[4], tb_cr_coded[4];
+} vvclf_deblock_cu;
+
+typedef struct vvclf_mv_cell {
+  uint8_t pred_l0, pred_l1, pad0, pad1;
+  int32_t poc_l0, poc_l1;
+  int32_t mv_l0[2], mv_l1[2]; /* 1/16 luma sample units */
+} vvclf_mv_cell;
+
+typedef struct vvclf_mv_grid {
+  const vvclf_mv_cell* cells; /* cells_w * cells_h, one per 4x4 luma cell */
+  uint32_t cells_w, cells_h;
+} vvclf_mv_grid;
+
+typedef struct vvclf_deblock_params {
+  int32_t disabled;
+  int32_t luma_beta_offset_div2, luma_tc_offset_div2;
+  int32_t cb_beta_offset_div2, cb_tc_offset_div2;
+  int32_t cr_beta_offset_div2, cr_tc_offset_div2;
+  int32_t chroma_qp_offset_cb, chroma_qp_offset_cr;
+  uint32_t bit_depth;
+  uint32_t ctb_log2_size_y;
+} vvclf_deblock_params;
+
+/* ------------------------------------------------------------------------ */
+/* SAO inputs — src/sao.rs:53-352.                                          */
+/* ------------------------------------------------------------------------ */
+#define VVCLF_SAO_NOT_APPLIED 0
+#define VVCLF_SAO_BAND 1
+#define VVCLF_SAO_EDGE 2
+
+typedef struct vvclf_sao_ctb {
+  uint8_t sao_type_idx;  /* 0 not applied, 1 band offset, 2 edge offset */
+  uint8_t eo_class;      /* 0 horizontal, 1 vertical, 2 135 deg, 3 45 deg */
+  uint8_t band_position; /* 0..31 */
+  uint8_t pad;
+  int32_t offset_val[5]; /* SaoOffsetVal after the eq. 153 shift; [0] == 0 */
+} vvclf_sao_ctb;
+
+typedef struct vvclf_sao_ctb_params {
+  vvclf_sao_ctb luma, cb, cr;
+} vvclf_sao_ctb_params;
+
+typedef struct vvclf_sao_picture {
+  uint32_t pic_width_in_ctbs_y, pic_height_in_ctbs_y;
+  const vvclf_sao_ctb_params* ctbs; /* raster CTB order */
+} vvclf_sao_picture;
+
+typedef struct vvclf_sao_config {
+  uint8_t luma_used, chroma_used, pad0, pad1;
+  uint32_t bit_depth, ctb_log2_size_y, chroma_format_idc;
+} vvclf_sao_config;
+
+/* Tile / slice clip bounds: prefix lists in luma samples, including 0 and the
+ * picture extents (the `tile_bounds: Option<(&[u32], &[u32])>` argument of
+ * apply_sao_clipped / apply_alf_clipped).  A NULL pointer means None. */
+typedef struct vvclf_bounds {
+  const uint32_t* cols;
+  uint32_t n_cols;
+  const uint32_t* rows;
+  uint32_t n_rows;
+} vvclf_bounds;
+
+/* ------------------------------------------------------------------------ */
+/* ALF inputs — src/alf.rs:67-206 and AlfApsData, src/aps.rs:21-102.         */
+/* ------------------------------------------------------------------------ */
+#define VVCLF_ALF_NUM_FILTERS 25
+#define VVCLF_ALF_LUMA_NUM_COEFF 12
+#define VVCLF_ALF_CHROMA_NUM_COEFF 6
+#define VVCLF_ALF_CC_NUM_COEFF 7
+#define VVCLF_ALF_MAX_CHROMA_ALT 8
+#define VVCLF_ALF_MAX_CC_FILTERS 4
+
+typedef struct vvclf_alf_ctb {
+  uint8_t luma_on, cb_on, cr_on;
+  uint8_t luma_filt_set_idx; /* <16 fixed set, >=16 APS slot idx-16 */
+  uint8_t cb_alt_idx, cr_alt_idx;
+  uint8_t cc_cb_idc, cc_cr_idc; /* 0 off, k -> filter k-1 */
+} vvclf_alf_ctb;
+
+typedef struct vvclf_alf_picture {
+  uint32_t pic_width_in_ctbs_y, pic_height_in_ctbs_y;
+  const vvclf_alf_ctb* ctbs; /* raster CTB order */
+} vvclf_alf_picture;
+
+typedef struct vvclf_alf_config {
+  uint8_t alf_enabled, cb_enabled, cr_enabled, cc_cb_enabled, cc_cr_enabled;
+  uint8_t pad0, pad1, pad2;
+  uint32_t bit_depth, ctb_log2_size_y, chroma_format_idc;
+} vvclf_alf_config;
+
+/* AlfApsData with its Vec fields as fixed arrays plus their lengths. */
+typedef struct vvclf_alf_aps {
+  uint8_t alf_luma_filter_signal_flag, alf_chroma_filter_signal_flag;
+  uint8_t alf_cc_cb_filter_signal_flag, alf_cc_cr_filter_signal_flag;
+  uint8_t alf_luma_clip_flag, alf_chroma_clip_flag, pad0, pad1;
+  uint32_t alf_chroma_num_alt_filters_minus1;
+  uint32_t n_luma_coeff;   /* luma_coeff.len(): 25 when the luma filter is signalled */
+  uint32_t n_chroma_coeff; /* chroma_coeff.len(), <= 8 */
+  uint32_t n_cc_cb_coeff;  /* cc_cb_coeff.len(), <= 4 */
+  uint32_t n_cc_cr_coeff;  /* cc_cr_coeff.len(), <= 4 */
+  int32_t luma_coeff[VVCLF_ALF_NUM_FILTERS][VVCLF_ALF_LUMA_NUM_COEFF];
+  uint8_t luma_clip_idx[VVCLF_ALF_NUM_FILTERS][VVCLF_ALF_LUMA_NUM_COEFF];
+  int32_t chroma_coeff[VVCLF_ALF_MAX_CHROMA_ALT][VVCLF_ALF_CHROMA_NUM_COEFF];
+  uint8_t chroma_clip_idx[VVCLF_ALF_MAX_CHROMA_ALT][VVCLF_ALF_CHROMA_NUM_COEFF];
+  int32_t cc_cb_coeff[VVCLF_ALF_MAX_CC_FILTERS][VVCLF_ALF_CC_NUM_COEFF];
+  int32_t cc_cr_coeff[VVCLF_ALF_MAX_CC_FILTERS][VVCLF_ALF_CC_NUM_COEFF];
+} vvclf_alf_aps;
+
+typedef struct vvclf_alf_aps_binding {
+  const vvclf_alf_aps* const* luma_apses; /* entries may be NULL (None slots) */
+  uint32_t n_luma_apses;
+  uint32_t pad;
+  const vvclf_alf_aps* chroma_aps; /* NULL = None */
+  const vvclf_alf_aps* cc_cb_aps;
+  const vvclf_alf_aps* cc_cr_aps;
+} vvclf_alf_aps_binding;
+
+/* ------------------------------------------------------------------------ */
+/* LMCS — LmcsData / LmcsDerived, src/lmcs.rs:51-153; LmcsBinding,           */
+/* src/ctu.rs:712-725.                                                       */
+/* ------------------------------------------------------------------------ */
+#define VVCLF_LMCS_NUM_BINS 16
+
+typedef struct vvclf_lmcs_data {
+  uint8_t lmcs_min_bin_idx, lmcs_delta_max_bin_idx, lmcs_delta_cw_prec_minus1, lmcs_delta_abs_crs;
+  uint8_t lmcs_delta_sign_crs_flag, pad0, pad1, pad2;
+  uint32_t lmcs_delta_abs_cw[VVCLF_LMCS_NUM_BINS];
+  uint8_t lmcs_delta_sign_cw_flag[VVCLF_LMCS_NUM_BINS];
+} vvclf_lmcs_data;
+
+typedef struct vvclf_lmcs_derived {
+  uint32_t bit_depth, org_cw;
+  uint32_t lmcs_cw[VVCLF_LMCS_NUM_BINS];
+  uint32_t input_pivot[VVCLF_LMCS_NUM_BINS];
+  uint32_t lmcs_pivot[VVCLF_LMCS_NUM_BINS + 1];
+  uint32_t scale_coeff[VVCLF_LMCS_NUM_BINS];
+  uint32_t inv_scale_coeff[VVCLF_LMCS_NUM_BINS];
+  int32_t lmcs_delta_crs;
+  uint32_t chroma_scale_coeff[VVCLF_LMCS_NUM_BINS];
+} vvclf_lmcs_derived;
+
+typedef struct vvclf_lmcs_binding {
+  const vvclf_lmcs_derived* derived; /* NULL = LMCS not active */
+  uint8_t min_bin_idx, max_bin_idx, pad0, pad1;
+} vvclf_lmcs_binding;
+
+/* ------------------------------------------------------------------------ */
+/* Picture-level descriptor for the driver, src/ctu.rs:11822-12066.         */
+/* ------------------------------------------------------------------------ */
+typedef struct vvclf_frame {
+  vvclf_picture pic; /* in/out */
+  uint32_t chroma_format_idc;
+  uint32_t pad0;
+  vvclf_lmcs_binding lmcs; /* lmcs_active() when derived != NULL */
+  /* deblocking (apply_deblocking_clipped arguments) */
+  const vvclf_deblock_cu* cus;
+  uint32_t n_cus;
+  uint32_t has_chroma_cus; /* chroma_cus: Option<&[DeblockCu]> */
+  const vvclf_deblock_cu* chroma_cus;
+  uint32_t n_chroma_cus;
+  uint32_t pad1;
+  const vvclf_mv_grid* mv_grid; /* NULL = None */
+  vvclf_deblock_params deblock;
+  const uint32_t* no_filter_cols;
+  uint32_t n_no_filter_cols;
+  uint32_t n_no_filter_rows;
+  const uint32_t* no_filter_rows;
+  /* SAO */
+  vvclf_sao_picture sao;
+  vvclf_sao_config sao_cfg;
+  /* ALF */
+  vvclf_alf_picture alf;
+  vvclf_alf_config alf_cfg;
+  vvclf_alf_aps_binding alf_binding;
+  /* SAO / ALF tile-slice clip bounds (NULL = None) */
+  const vvclf_bounds* clip_bounds;
+} vvclf_frame;
+
+/* ======================================================================== */
+/* Layer 1: drop-in synchronous entry points (host pointers, in place).      */
+/* ======================================================================== */
+
+/* Replaces `apply_deblocking(out, cus, params, chroma_format_idc)`,
+ * src/deblock.rs:480-487. */
+int vvclf_apply_deblocking(vvclf_picture* out, const vvclf_deblock_cu* cus, uint32_t n_cus,
+                           const vvclf_deblock_params* params, uint32_t chroma_format_idc);
+
+/* Replaces `apply_deblocking_clipped(out, cus, chroma_cus, mv_grid, params,
+ * chroma_format_idc, no_filter_cols, no_filter_rows)`, src/deblock.rs:497-617.
+ * chroma_cus == NULL means None; mv_grid == NULL means None. */
+int vvclf_apply_deblocking_clipped(vvclf_picture* out, const vvclf_deblock_cu* cus, uint32_t n_cus,
+                                   const vvclf_deblock_cu* chroma_cus, uint32_t n_chroma_cus,
+                                   const vvclf_mv_grid* mv_grid, const vvclf_deblock_params* params,
+                                   uint32_t chroma_format_idc, const uint32_t* no_filter_cols,
+                                   uint32_t n_no_filter_cols, const uint32_t* no_filter_rows,
+                                   uint32_t n_no_filter_rows);
+
+/* Replaces `apply_sao(out, sao_pic, cfg)`, src/sao.rs:379-381. */
+int vvclf_apply_sao(vvclf_picture* out, const vvclf_sao_picture* sao_pic, const vvclf_sao_config* cfg);
+
+/* Replaces `apply_sao_clipped(out, sao_pic, cfg, tile_bounds)`, src/sao.rs:389-497. */
+int vvclf_apply_sao_clipped(vvclf_picture* out, const vvclf_sao_picture* sao_pic,
+                            const vvclf_sao_config* cfg, const vvclf_bounds* tile_bounds);
+
+/* Replaces `apply_alf(out, alf_pic, cfg, binding)`, src/alf.rs:220-227. */
+int vvclf_apply_alf(vvclf_picture* out, const vvclf_alf_picture* alf_pic, const vvclf_alf_config* cfg,
+                    const vvclf_alf_aps_binding* binding);
+
+/* Replaces `apply_alf_clipped(out, alf_pic, cfg, binding, tile_bounds)`,
+ * src/alf.rs:235-434. */
+int vvclf_apply_alf_clipped(vvclf_picture* out, const vvclf_alf_picture* alf_pic,
+                            const vvclf_alf_config* cfg, const vvclf_alf_aps_binding* binding,
+                            const vvclf_bounds* tile_bounds);
+
+/* Replaces the per-picture LMCS inverse-mapping loop of the driver,
+ * src/ctu.rs:11837-11849, built on LmcsDerived::idx_y_inv /
+ * inverse_map_luma_sample, src/lmcs.rs:209-238.  No-op when binding->derived
+ * is NULL (lmcs_active() false, src/ctu.rs:1675-1677). */
+int vvclf_lmcs_inverse_luma(vvclf_plane* luma, const vvclf_lmcs_binding* binding);
+
+/* Replaces `CtuWalker::apply_in_loop_filters_with_alf(out, alf_binding)`,
+ * src/ctu.rs:11822-12066, with the walker state (SPS/PPS/slice header fields,
+ * CU records, motion grid, SAO/ALF pictures, gating lists) flattened into
+ * the frame descriptor.  Order: LMCS inverse, deblock, SAO, ALF + CC-ALF. */
+int vvclf_apply_in_loop_filters(vvclf_frame* frame);
+
+/* Replaces `derive_lmcs(data, bit_depth)`, src/lmcs.rs:302-404 (host-side). */
+int vvclf_derive_lmcs(const vvclf_lmcs_data* data, uint32_t bit_depth, vvclf_lmcs_derived* out);
+
+/* ======================================================================== */
+/* Layer 2: device-resident jobs, batches and serving (no reference          */
+/* counterpart; the reference is single-threaded CPU code).                  */
+/* ======================================================================== */
+typedef struct vvclf_ctx vvclf_ctx;
+typedef struct vvclf_job vvclf_job;
+typedef struct vvclf_batch vvclf_batch;
+typedef struct vvclf_server vvclf_server;
+
+/* Last error message of the calling thread ("" when none). */
+const char* vvclf_last_error(void);
+/* Size of a public struct by name ("vvclf_deblock_cu", ...), 0 if unknown;
+ * lets bindings verify their layouts. */
+size_t vvclf_abi_sizeof(const char* type_name);
+
+/* Host-only dry run: validates the frame and derives every table the GPU
+ * path would upload, without touching a device.  stats (may be NULL) receives
+ * [0..2] luma deblock segments visited with bS 0 / 1 / 2, [3] chroma segments
+ * filtered, [4] SAO CTBs with any component on, [5] ALF CTBs with luma on,
+ * [6] device bytes the job would hold, [7] reserved. */
+int vvclf_frame_check(const vvclf_frame* frame, uint64_t stats[8]);
+
+/* Context: one per device. */
+int vvclf_ctx_create(int device, vvclf_ctx** out);
+int vvclf_ctx_destroy(vvclf_ctx* ctx);
+
+/* Job: one picture plus its side information, resident in HBM.  The host
+ * derives the compact per-edge and per-CTB tables once here; input samples
+ * are copied from frame->pic.  The input planes stay untouched by runs, so a
+ * job can be run repeatedly with identical results. */
+int vvclf_job_create(vvclf_ctx* ctx, const vvclf_frame* frame, vvclf_job** out);
+int vvclf_job_destroy(vvclf_job* job);
+/* Re-upload input samples (same geometry as at creation). */
+int vvclf_job_set_input(vvclf_job* job, const vvclf_picture* pic);
+/* Plain per-picture form: separate stage kernels on `stream` (hipStream_t,
+ * NULL = default stream).  Asynchronous. */
+int vvclf_job_run(vvclf_job* job, void* stream);
+/* Stage timing for the plain form: a pool of HIP events for up to
+ * `max_runs` runs.  vvclf_job_run_prof records events between the stage
+ * kernels on the run's stream (no synchronisation); vvclf_prof_stage_ms waits
+ * for the recorded runs and returns, per stage, the summed milliseconds:
+ * [prep, deblock luma V, deblock luma H, deblock chroma V, deblock chroma H,
+ *  SAO, ALF luma, ALF chroma + CC-ALF]. */
+typedef struct vvclf_prof vvclf_prof;
+int vvclf_prof_create(vvclf_ctx* ctx, uint32_t max_runs, vvclf_prof** out);
+int vvclf_prof_destroy(vvclf_prof* prof);
+int vvclf_prof_reset(vvclf_prof* prof);
+int vvclf_job_run_prof(vvclf_job* job, void* stream, vvclf_prof* prof);
+int vvclf_prof_stage_ms(vvclf_prof* prof, float stage_ms[8], uint32_t* runs);
+/* Copy the filtered picture to host planes (synchronous). */
+int vvclf_job_download(vvclf_job* job, vvclf_picture* out);
+/* Device pointer of the filtered output planes (luma, cb, cr) and their
+ * pitch in samples. */
+int vvclf_job_output(vvclf_job* job, uint16_t** luma, uint16_t** cb, uint16_t** cr,
+                     uint32_t* pitch_luma, uint32_t* pitch_chroma);
+/* Device bytes a job holds (side information + sample planes). */
+uint64_t vvclf_job_device_bytes(const vvclf_job* job);
+/* Luma pixels of the job's picture. */
+uint64_t vvclf_job_luma_pixels(const vvclf_job* job);
+
+/* Tile shape for the fused kernel (chosen by vvclf_autotune when tile_w == 0). */
+typedef struct vvclf_tune {
+  uint32_t tile_w, tile_h;   /* output tile in luma samples */
+  uint32_t threads;          /* workgroup size */
+  uint32_t wgs_per_cu;       /* persistent workgroups per CU */
+} vvclf_tune;
+
+/* Batch: many jobs (uneven sizes) filtered by one persistent fused
+ * deblock->SAO->ALF launch over a global tile queue, largest first. */
+int vvclf_batch_create(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs,
+                       const vvclf_tune* tune, vvclf_batch** out);
+int vvclf_batch_destroy(vvclf_batch* batch);
+int vvclf_batch_run(vvclf_batch* batch, void* stream);
+/* Capture the batch launch in a hipGraph once, then replay it. */
+int vvclf_batch_graph_launch(vvclf_batch* batch, void* stream);
+int vvclf_batch_get_tune(const vvclf_batch* batch, vvclf_tune* out);
+/* Pick a tile shape for the given jobs by timing candidates on the device. */
+int vvclf_autotune(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, vvclf_tune* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VVCLF_H */

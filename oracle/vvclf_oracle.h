@@ -1,0 +1,86 @@
+/*
+ * vvclf_oracle.h — CPU restatement of the reference VVC in-loop filters.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity oracle: tests, the smoke check
+ * and the CPU-baseline leg of the benchmark call it as the checker.  The
+ * product library (libvvclf.so) never links, loads or calls it.
+ *
+ * Every function follows the cited reference lines step by step, on the same
+ * POD types as the product's C ABI (include/vvclf.h).
+ */
+#ifndef VVCLF_ORACLE_H
+#define VVCLF_ORACLE_H
+
+#include "../include/vvclf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/lmcs.rs:302-404 */
+int oracle_derive_lmcs(const vvclf_lmcs_data* data, uint32_t bit_depth, vvclf_lmcs_derived* out);
+/* src/lmcs.rs:209-220 */
+uint32_t oracle_lmcs_idx_y_inv(const vvclf_lmcs_derived* d, uint32_t s, uint32_t min_bin, uint32_t max_bin);
+/* src/lmcs.rs:230-238 */
+uint32_t oracle_lmcs_inverse_map(const vvclf_lmcs_derived* d, uint32_t s, uint32_t idx);
+/* src/lmcs.rs:186-195 */
+int64_t oracle_lmcs_forward_map(const vvclf_lmcs_derived* d, uint32_t pred_sample);
+/* src/ctu.rs:11837-11849 */
+void oracle_lmcs_inverse_luma(vvclf_plane* luma, const vvclf_lmcs_binding* b);
+
+/* src/deblock.rs:424-452 */
+int32_t oracle_beta_prime(int32_t q);
+int32_t oracle_tc_prime(int32_t q);
+int32_t oracle_scale_tc(int32_t tc_prime, uint32_t bit_depth);
+int32_t oracle_scale_beta(int32_t beta_prime, uint32_t bit_depth);
+/* src/deblock.rs:168-291 (tu_coded_at, luma_tb_len, chroma_tb_len, is_*_tb_boundary) */
+int oracle_tu_coded_at(const vvclf_deblock_cu* cu, uint32_t c_idx, int32_t lx, int32_t ly);
+uint32_t oracle_luma_tb_len(const vvclf_deblock_cu* cu, int32_t lx, int32_t ly, int vertical);
+uint32_t oracle_chroma_tb_len(const vvclf_deblock_cu* cu, int32_t lx, int32_t ly, int vertical, uint32_t sub);
+int oracle_is_luma_tb_boundary(const vvclf_deblock_cu* cu, uint32_t off, int vertical);
+int oracle_is_chroma_tb_boundary(const vvclf_deblock_cu* cu, uint32_t off, int vertical);
+/* src/deblock.rs:497-617 */
+void oracle_apply_deblocking_clipped(vvclf_picture* out, const vvclf_deblock_cu* cus, uint32_t n_cus,
+                                     const vvclf_deblock_cu* chroma_cus, uint32_t n_chroma_cus,
+                                     const vvclf_mv_grid* mv_grid, const vvclf_deblock_params* params,
+                                     uint32_t chroma_format_idc, const uint32_t* no_filter_cols,
+                                     uint32_t n_no_filter_cols, const uint32_t* no_filter_rows,
+                                     uint32_t n_no_filter_rows);
+
+/* src/sao.rs:389-497 */
+void oracle_apply_sao_clipped(vvclf_picture* out, const vvclf_sao_picture* sao_pic,
+                              const vvclf_sao_config* cfg, const vvclf_bounds* tile_bounds);
+/* src/sao.rs:356-365 */
+void oracle_sao_derive_offset_val(const uint32_t offset_abs[4], const uint32_t sign[4], uint32_t bit_depth,
+                                  int32_t out[5]);
+
+/* src/alf.rs:627-644 */
+int32_t oracle_alf_clip_value(uint32_t bit_depth, uint8_t clip_idx);
+/* src/alf.rs:1117-1143, :528-539 */
+void oracle_alf_table_45(int32_t y, int32_t ctb, int lbb, int32_t out[4]);
+void oracle_alf_table_46(int32_t y, int32_t ctb_h_c, int lbb, int32_t out[3]);
+void oracle_alf_table_47(int32_t y_line, int32_t ctb, int lbb, int32_t out[2]);
+/* src/alf.rs:731-742 */
+void oracle_alf_transpose_idx(uint8_t t, uint8_t out[12]);
+/* src/alf_fixed.rs:173-185: returns 0 and fills row, or -1 for None */
+int oracle_alf_fixed_coeff_row(uint32_t set_idx, uint32_t class_idx, int32_t row[12]);
+/* src/alf.rs:822-1010: cls[(sy*(ctb/4)+sx)] = filt_idx | transpose_idx << 8 */
+void oracle_alf_classify_ctb(const uint16_t* pre, uint32_t stride, int32_t pw, int32_t ph, const int32_t* clip,
+                             uint32_t rx, uint32_t ry, uint32_t ctb_size_y, uint32_t bit_depth, uint16_t* cls);
+/* src/alf.rs:235-434; returns VVCLF_ERR_INVALID on an APS whose Vec lengths the
+ * reference would index out of range, VVCLF_OK otherwise. */
+int oracle_apply_alf_clipped(vvclf_picture* out, const vvclf_alf_picture* alf_pic, const vvclf_alf_config* cfg,
+                             const vvclf_alf_aps_binding* binding, const vvclf_bounds* tile_bounds);
+
+/* src/ctu.rs:11822-12066 */
+int oracle_apply_in_loop_filters(vvclf_frame* frame);
+
+/* Runs oracle_apply_in_loop_filters on n frames with up to n_threads host
+ * threads (one frame per thread at a time); used for the CPU baseline. */
+int oracle_apply_in_loop_filters_mt(vvclf_frame* const* frames, uint32_t n, uint32_t n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,579 @@
+// vvclf_api.cpp — job lifetime and the synchronous drop-in entry points of
+// include/vvclf.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vvclf.h"
+#include "vvclf_host_internal.h"
+#include "vvclf_internal.h"
+
+using namespace vvclf;
+
+namespace {
+
+inline uint32_t round_up(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
+
+int check_plane(const vvclf_plane& p, const char* name) {
+  if (!p.samples) return fail(VVCLF_ERR_INVALID, "%s: NULL samples", name);
+  if (p.width == 0 || p.height == 0) return fail(VVCLF_ERR_INVALID, "%s: empty plane", name);
+  if (p.stride < p.width) return fail(VVCLF_ERR_INVALID, "%s: stride %u < width %u", name, p.stride, p.width);
+  if (p.width > 65535 || p.height > 65535) return fail(VVCLF_ERR_UNSUPPORTED, "%s: plane larger than 65535", name);
+  if (p.bit_depth < 8 || p.bit_depth > 16) return fail(VVCLF_ERR_INVALID, "%s: bit_depth %u", name, p.bit_depth);
+  return VVCLF_OK;
+}
+
+struct side_builder {
+  std::vector<uint8_t> blob;
+  size_t add(const void* p, size_t n) {
+    size_t off = (blob.size() + 255) / 256 * 256;
+    blob.resize(off + std::max<size_t>(n, 4));
+    if (n) memcpy(blob.data() + off, p, n);
+    return off;
+  }
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    return add(v.data(), v.size() * sizeof(T));
+  }
+};
+
+vvclf_ctx* default_ctx() {
+  static std::mutex mu;
+  static vvclf_ctx* ctx = nullptr;
+  std::lock_guard<std::mutex> g(mu);
+  if (!ctx) {
+    if (vvclf_ctx_create(0, &ctx) != VVCLF_OK) ctx = nullptr;
+  }
+  return ctx;
+}
+
+}  // namespace
+
+namespace vvclf {
+
+int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvclf_job** out,
+                      uint64_t* dry_stats) {
+  clear_error();
+  if ((!ctx && !dry_stats) || !f || (!out && !dry_stats)) return fail(VVCLF_ERR_INVALID, "job_create: NULL argument");
+  if (out) *out = nullptr;
+  int rc;
+  if ((rc = check_plane(f->pic.luma, "luma"))) return rc;
+  const uint32_t cfi = f->chroma_format_idc;
+  const uint32_t W = f->pic.luma.width, H = f->pic.luma.height;
+  uint32_t Wc = 0, Hc = 0;
+  // Chroma is processed when any active stage reads it.
+  bool want_chroma = false;
+  if ((stages & STAGE_DBK) && !f->deblock.disabled && cfi != 0) want_chroma = true;
+  if ((stages & STAGE_SAO) && f->sao_cfg.chroma_used && f->sao_cfg.chroma_format_idc != 0) want_chroma = true;
+  if ((stages & STAGE_ALF) && f->alf_cfg.alf_enabled && f->alf_cfg.chroma_format_idc != 0) want_chroma = true;
+  if (want_chroma) {
+    if (cfi != 1 && (stages & STAGE_DBK) && !f->deblock.disabled)
+      return fail(VVCLF_ERR_UNSUPPORTED, "chroma_format_idc %u: only 4:0:0 and 4:2:0 run on the GPU", cfi);
+    if ((rc = check_plane(f->pic.cb, "cb")) || (rc = check_plane(f->pic.cr, "cr"))) return rc;
+    if (f->pic.cb.width != f->pic.cr.width || f->pic.cb.height != f->pic.cr.height)
+      return fail(VVCLF_ERR_INVALID, "cb and cr planes differ in size");
+    if (f->pic.cb.bit_depth != f->pic.luma.bit_depth || f->pic.cr.bit_depth != f->pic.luma.bit_depth)
+      return fail(VVCLF_ERR_UNSUPPORTED, "planes with different bit depths");
+    Wc = f->pic.cb.width;
+    Hc = f->pic.cb.height;
+  }
+  host_tables t;
+  bool do_lmcs = (stages & STAGE_LMCS) && f->lmcs.derived;
+  if (do_lmcs) {
+    if (f->lmcs.derived->bit_depth < 8 || f->lmcs.derived->bit_depth > 16 || f->lmcs.max_bin_idx > 15)
+      return fail(VVCLF_ERR_INVALID, "lmcs: invalid derived tables");
+    build_lmcs_lut(f->lmcs, t.lmcs_lut);
+  }
+  bool do_dbk = (stages & STAGE_DBK) && !f->deblock.disabled;
+  if (do_dbk) {
+    if (f->deblock.bit_depth < 8 || f->deblock.bit_depth > 16)
+      return fail(VVCLF_ERR_INVALID, "deblock: bit_depth %u", f->deblock.bit_depth);
+    if (f->n_cus && !f->cus) return fail(VVCLF_ERR_INVALID, "deblock: NULL cus");
+    if ((rc = derive_deblock_maps(*f, W, H, Wc, Hc, t))) return rc;
+  }
+  if ((rc = derive_sao_alf(*f, W, H, t, stages))) return rc;
+  if (dry_stats) {
+    for (int i = 0; i < 3; i++) dry_stats[i] = t.bs_hist[i];
+    dry_stats[3] = t.chroma_filtered;
+    uint64_t sao_on = 0, alf_on = 0;
+    for (size_t i = 0; i + 2 < t.sao.size(); i += 3)
+      if ((t.sao[i] | t.sao[i + 1] | t.sao[i + 2]) & 3) sao_on++;
+    for (uint32_t w : t.alf_ctb)
+      if ((w & 0xff) != ALF_OFF8) alf_on++;
+    dry_stats[4] = sao_on;
+    dry_stats[5] = alf_on;
+    uint64_t py = round_up(W, 64), pc = Wc ? round_up(Wc, 64) : 0;
+    uint64_t side = t.lmcs_lut.size() * 2 + (t.e_lv.size() + t.e_lh.size()) * 4 +
+                    (t.e_cv[0].size() + t.e_cv[1].size() + t.e_ch[0].size() + t.e_ch[1].size()) * 4 +
+                    (t.sao.size() + t.sao_rect.size() + t.alf_rect.size()) * 8 +
+                    (t.alf_ctb.size() + t.alf_bank.size() + t.alf_cbank.size() + t.alf_ccbank.size()) * 4;
+    dry_stats[6] = 4 * (py * H + 2 * pc * Hc) * 2 + side;
+    dry_stats[7] = 0;
+    return VVCLF_OK;
+  }
+
+  vvclf_job* j = new vvclf_job();
+  memset(&j->d, 0, sizeof(j->d));
+  j->ctx = ctx;
+  j->w = W;
+  j->h = H;
+  j->wc = Wc;
+  j->hc = Hc;
+  j->stages = stages;
+  dev_job& d = j->d;
+  d.w = (int32_t)W;
+  d.h = (int32_t)H;
+  d.wc = (int32_t)Wc;
+  d.hc = (int32_t)Hc;
+  d.py = (int32_t)round_up(W, 64);
+  d.pc = Wc ? (int32_t)round_up(Wc, 64) : 0;
+  d.bd = (int32_t)f->pic.luma.bit_depth;
+  d.bd_dbk = (int32_t)f->deblock.bit_depth;
+  d.do_lmcs = do_lmcs;
+  d.do_dbk = do_dbk;
+  d.do_dbk_chroma = do_dbk && t.has_chroma_dbk;
+  d.do_sao = t.do_sao;
+  d.sao_luma = t.sao_luma;
+  d.sao_chroma = t.sao_chroma && Wc > 0;
+  d.sao_ctb_log2 = (int32_t)t.sao_ctb_log2;
+  d.swctb = (int32_t)t.swctb;
+  d.shctb = (int32_t)t.shctb;
+  d.sao_bd = (int32_t)t.sao_bd;
+  d.do_alf = t.do_alf;
+  d.ctb_log2 = (int32_t)t.alf_ctb_log2;
+  d.wctb = (int32_t)t.wctb;
+  d.hctb = (int32_t)t.hctb;
+  d.alf_bd = (int32_t)t.alf_bd;
+
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) {
+    delete j;
+    return hip_fail(e, "hipSetDevice");
+  }
+  // sample planes: in, a, b, out
+  size_t luma_elems = (size_t)d.py * H, chroma_elems = (size_t)d.pc * Hc;
+  size_t set_bytes = (luma_elems + 2 * chroma_elems) * 2;
+  set_bytes = (set_bytes + 255) / 256 * 256;
+  j->bytes_planes = set_bytes * 4;
+  e = hipMalloc(&j->dev_planes, j->bytes_planes);
+  if (e != hipSuccess) {
+    delete j;
+    return fail(VVCLF_ERR_NOMEM, "hipMalloc planes (%zu bytes): %s", (size_t)j->bytes_planes, hipGetErrorString(e));
+  }
+  dev_planes* sets[4] = {&d.in, &d.a, &d.b, &d.out};
+  for (int s = 0; s < 4; s++) {
+    uint8_t* base = (uint8_t*)j->dev_planes + s * set_bytes;
+    sets[s]->y = (uint16_t*)base;
+    sets[s]->cb = Wc ? (uint16_t*)(base + luma_elems * 2) : nullptr;
+    sets[s]->cr = Wc ? (uint16_t*)(base + (luma_elems + chroma_elems) * 2) : nullptr;
+  }
+  if (!t.do_sao) d.b = d.a;
+  // side tables
+  side_builder sb;
+  size_t o_lut = sb.add(t.lmcs_lut), o_lv = sb.add(t.e_lv), o_lh = sb.add(t.e_lh);
+  size_t o_cv0 = sb.add(t.e_cv[0]), o_cv1 = sb.add(t.e_cv[1]), o_ch0 = sb.add(t.e_ch[0]), o_ch1 = sb.add(t.e_ch[1]);
+  size_t o_sao = sb.add(t.sao), o_saor = sb.add(t.sao_rect);
+  size_t o_actb = sb.add(t.alf_ctb), o_arect = sb.add(t.alf_rect), o_bank = sb.add(t.alf_bank);
+  size_t o_cbank = sb.add(t.alf_cbank), o_ccbank = sb.add(t.alf_ccbank);
+  j->bytes_side = sb.blob.size();
+  e = hipMalloc(&j->dev_side, j->bytes_side);
+  if (e != hipSuccess) {
+    hipFree(j->dev_planes);
+    delete j;
+    return fail(VVCLF_ERR_NOMEM, "hipMalloc side tables: %s", hipGetErrorString(e));
+  }
+  e = hipMemcpy(j->dev_side, sb.blob.data(), j->bytes_side, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    vvclf_job_destroy(j);
+    return hip_fail(e, "upload side tables");
+  }
+  uint8_t* sd = (uint8_t*)j->dev_side;
+  d.lmcs_lut = do_lmcs ? (const uint16_t*)(sd + o_lut) : nullptr;
+  d.gw = (int32_t)t.gw;
+  d.gh = (int32_t)t.gh;
+  d.e_lv = (const uint32_t*)(sd + o_lv);
+  d.e_lh = (const uint32_t*)(sd + o_lh);
+  d.cvw = (int32_t)t.cvw;
+  d.cvh = (int32_t)t.cvh;
+  d.chw = (int32_t)t.chw;
+  d.chh = (int32_t)t.chh;
+  d.e_cv[0] = (const uint32_t*)(sd + o_cv0);
+  d.e_cv[1] = (const uint32_t*)(sd + o_cv1);
+  d.e_ch[0] = (const uint32_t*)(sd + o_ch0);
+  d.e_ch[1] = (const uint32_t*)(sd + o_ch1);
+  d.sao = (const uint64_t*)(sd + o_sao);
+  d.sao_rect = t.sao_rect.empty() ? nullptr : (const uint64_t*)(sd + o_saor);
+  d.alf_ctb = (const uint32_t*)(sd + o_actb);
+  d.alf_rect = (const uint64_t*)(sd + o_arect);
+  d.alf_bank = (const uint32_t*)(sd + o_bank);
+  d.alf_cbank = (const uint32_t*)(sd + o_cbank);
+  d.alf_ccbank = (const int32_t*)(sd + o_ccbank);
+  if ((rc = vvclf_job_set_input(j, &f->pic))) {
+    vvclf_job_destroy(j);
+    return rc;
+  }
+  *out = j;
+  return VVCLF_OK;
+}
+
+const uint16_t* job_final(const vvclf_job* j, int comp) {
+  const dev_planes& p = j->d.do_alf ? j->d.out : (j->d.do_sao ? j->d.b : j->d.a);
+  return comp == 0 ? p.y : (comp == 1 ? p.cb : p.cr);
+}
+
+}  // namespace vvclf
+
+extern "C" {
+
+const char* vvclf_last_error(void) { return vvclf::last_error(); }
+
+size_t vvclf_abi_sizeof(const char* n) {
+  if (!n) return 0;
+#define S(T) \
+  if (!strcmp(n, #T)) return sizeof(T);
+  S(vvclf_plane) S(vvclf_picture) S(vvclf_deblock_cu) S(vvclf_mv_cell) S(vvclf_mv_grid) S(vvclf_deblock_params)
+  S(vvclf_sao_ctb) S(vvclf_sao_ctb_params) S(vvclf_sao_picture) S(vvclf_sao_config) S(vvclf_bounds)
+  S(vvclf_alf_ctb) S(vvclf_alf_picture) S(vvclf_alf_config) S(vvclf_alf_aps) S(vvclf_alf_aps_binding)
+  S(vvclf_lmcs_data) S(vvclf_lmcs_derived) S(vvclf_lmcs_binding) S(vvclf_frame) S(vvclf_tune)
+#undef S
+  return 0;
+}
+
+int vvclf_ctx_create(int device, vvclf_ctx** out) {
+  if (!out) return fail(VVCLF_ERR_INVALID, "ctx_create: NULL out");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(VVCLF_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return fail(VVCLF_ERR_INVALID, "device %d of %d", device, n);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  vvclf_ctx* c = new vvclf_ctx();
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = c;
+  return VVCLF_OK;
+}
+
+int vvclf_ctx_destroy(vvclf_ctx* ctx) {
+  if (!ctx) return VVCLF_OK;
+  hipSetDevice(ctx->device);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return VVCLF_OK;
+}
+
+int vvclf_job_create(vvclf_ctx* ctx, const vvclf_frame* frame, vvclf_job** out) {
+  return job_create_stages(ctx, frame, STAGE_ALL, out, nullptr);
+}
+
+int vvclf_frame_check(const vvclf_frame* frame, uint64_t stats[8]) {
+  uint64_t local[8];
+  return job_create_stages(nullptr, frame, STAGE_ALL, nullptr, stats ? stats : local);
+}
+
+int vvclf_job_destroy(vvclf_job* j) {
+  if (!j) return VVCLF_OK;
+  hipSetDevice(j->ctx->device);
+  if (j->dev_planes) hipFree(j->dev_planes);
+  if (j->dev_side) hipFree(j->dev_side);
+  delete j;
+  return VVCLF_OK;
+}
+
+int vvclf_job_set_input(vvclf_job* j, const vvclf_picture* pic) {
+  if (!j || !pic) return fail(VVCLF_ERR_INVALID, "job_set_input: NULL argument");
+  if (pic->luma.width != j->w || pic->luma.height != j->h || !pic->luma.samples)
+    return fail(VVCLF_ERR_INVALID, "job_set_input: luma geometry differs");
+  hipSetDevice(j->ctx->device);
+  hipError_t e = hipMemcpy2D(j->d.in.y, (size_t)j->d.py * 2, pic->luma.samples, (size_t)pic->luma.stride * 2,
+                             (size_t)j->w * 2, j->h, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "upload luma");
+  if (j->wc) {
+    if (!pic->cb.samples || !pic->cr.samples || pic->cb.width != j->wc || pic->cb.height != j->hc)
+      return fail(VVCLF_ERR_INVALID, "job_set_input: chroma geometry differs");
+    e = hipMemcpy2D(j->d.in.cb, (size_t)j->d.pc * 2, pic->cb.samples, (size_t)pic->cb.stride * 2, (size_t)j->wc * 2,
+                    j->hc, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy2D(j->d.in.cr, (size_t)j->d.pc * 2, pic->cr.samples, (size_t)pic->cr.stride * 2,
+                      (size_t)j->wc * 2, j->hc, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "upload chroma");
+  }
+  return VVCLF_OK;
+}
+
+int vvclf_job_run(vvclf_job* j, void* stream) {
+  if (!j) return fail(VVCLF_ERR_INVALID, "job_run: NULL job");
+  hipSetDevice(j->ctx->device);
+  hipError_t e = launch_plain(j->d, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "launch");
+  return VVCLF_OK;
+}
+
+int vvclf_job_download(vvclf_job* j, vvclf_picture* out) {
+  if (!j || !out) return fail(VVCLF_ERR_INVALID, "job_download: NULL argument");
+  hipSetDevice(j->ctx->device);
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "synchronize");
+  if (out->luma.width != j->w || out->luma.height != j->h || !out->luma.samples)
+    return fail(VVCLF_ERR_INVALID, "job_download: luma geometry differs");
+  e = hipMemcpy2D(out->luma.samples, (size_t)out->luma.stride * 2, job_final(j, 0), (size_t)j->d.py * 2,
+                  (size_t)j->w * 2, j->h, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "download luma");
+  if (j->wc) {
+    e = hipMemcpy2D(out->cb.samples, (size_t)out->cb.stride * 2, job_final(j, 1), (size_t)j->d.pc * 2,
+                    (size_t)j->wc * 2, j->hc, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy2D(out->cr.samples, (size_t)out->cr.stride * 2, job_final(j, 2), (size_t)j->d.pc * 2,
+                      (size_t)j->wc * 2, j->hc, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "download chroma");
+  }
+  return VVCLF_OK;
+}
+
+int vvclf_job_output(vvclf_job* j, uint16_t** y, uint16_t** cb, uint16_t** cr, uint32_t* py, uint32_t* pc) {
+  if (!j) return fail(VVCLF_ERR_INVALID, "job_output: NULL job");
+  if (y) *y = (uint16_t*)job_final(j, 0);
+  if (cb) *cb = (uint16_t*)job_final(j, 1);
+  if (cr) *cr = (uint16_t*)job_final(j, 2);
+  if (py) *py = (uint32_t)j->d.py;
+  if (pc) *pc = (uint32_t)j->d.pc;
+  return VVCLF_OK;
+}
+
+int vvclf_prof_create(vvclf_ctx* ctx, uint32_t max_runs, vvclf_prof** out) {
+  if (!ctx || !out || max_runs == 0) return fail(VVCLF_ERR_INVALID, "prof_create: bad argument");
+  hipSetDevice(ctx->device);
+  vvclf_prof* p = new vvclf_prof();
+  p->device = ctx->device;
+  p->cap = max_runs;
+  p->used = 0;
+  p->ev.resize((size_t)max_runs * (PLAIN_STAGES + 1));
+  for (auto& e : p->ev) {
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) {
+      delete p;
+      return hip_fail(r, "hipEventCreate");
+    }
+  }
+  *out = p;
+  return VVCLF_OK;
+}
+
+int vvclf_prof_destroy(vvclf_prof* p) {
+  if (!p) return VVCLF_OK;
+  hipSetDevice(p->device);
+  for (auto& e : p->ev) hipEventDestroy(e);
+  delete p;
+  return VVCLF_OK;
+}
+
+int vvclf_prof_reset(vvclf_prof* p) {
+  if (!p) return fail(VVCLF_ERR_INVALID, "prof_reset: NULL");
+  p->used = 0;
+  return VVCLF_OK;
+}
+
+int vvclf_job_run_prof(vvclf_job* j, void* stream, vvclf_prof* p) {
+  if (!j || !p) return fail(VVCLF_ERR_INVALID, "job_run_prof: NULL argument");
+  if (p->used >= p->cap) return fail(VVCLF_ERR_INVALID, "job_run_prof: event pool full");
+  hipSetDevice(j->ctx->device);
+  hipError_t e = launch_plain(j->d, (hipStream_t)stream, &p->ev[(size_t)p->used * (PLAIN_STAGES + 1)]);
+  if (e != hipSuccess) return hip_fail(e, "launch");
+  p->used++;
+  return VVCLF_OK;
+}
+
+int vvclf_prof_stage_ms(vvclf_prof* p, float ms[8], uint32_t* runs) {
+  if (!p || !ms) return fail(VVCLF_ERR_INVALID, "prof_stage_ms: NULL argument");
+  hipSetDevice(p->device);
+  for (int s = 0; s < PLAIN_STAGES; s++) ms[s] = 0.f;
+  for (uint32_t r = 0; r < p->used; r++) {
+    hipEvent_t* ev = &p->ev[(size_t)r * (PLAIN_STAGES + 1)];
+    hipError_t e = hipEventSynchronize(ev[PLAIN_STAGES]);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    for (int s = 0; s < PLAIN_STAGES; s++) {
+      float t = 0.f;
+      e = hipEventElapsedTime(&t, ev[s], ev[s + 1]);
+      if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+      ms[s] += t;
+    }
+  }
+  if (runs) *runs = p->used;
+  return VVCLF_OK;
+}
+
+uint64_t vvclf_job_device_bytes(const vvclf_job* j) { return j ? j->bytes_planes + j->bytes_side : 0; }
+uint64_t vvclf_job_luma_pixels(const vvclf_job* j) { return j ? (uint64_t)j->w * j->h : 0; }
+
+// ---------------------------------------------------------------------------
+// Drop-in synchronous entry points: upload, run the requested stages, copy the
+// result back into the caller's planes (in place, like `&mut PictureBuffer`).
+// ---------------------------------------------------------------------------
+static int run_sync(vvclf_frame* f, uint32_t stages) {
+  vvclf_ctx* ctx = default_ctx();
+  if (!ctx) return VVCLF_ERR_HIP;
+  vvclf_job* j = nullptr;
+  int rc = job_create_stages(ctx, f, stages, &j, nullptr);
+  if (rc) return rc;
+  // Nothing to do: leave the planes untouched.
+  if (!j->d.do_lmcs && !j->d.do_dbk && !j->d.do_sao && !j->d.do_alf) {
+    vvclf_job_destroy(j);
+    return VVCLF_OK;
+  }
+  rc = vvclf_job_run(j, ctx->stream);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "stream synchronize");
+  }
+  if (!rc) rc = vvclf_job_download(j, &f->pic);
+  vvclf_job_destroy(j);
+  return rc;
+}
+
+static void frame_zero(vvclf_frame& f, vvclf_picture* pic, uint32_t cfi) {
+  memset(&f, 0, sizeof(f));
+  f.pic = *pic;
+  f.chroma_format_idc = cfi;
+  f.deblock.disabled = 1;
+}
+
+int vvclf_apply_deblocking(vvclf_picture* out, const vvclf_deblock_cu* cus, uint32_t n_cus,
+                           const vvclf_deblock_params* params, uint32_t cfi) {
+  return vvclf_apply_deblocking_clipped(out, cus, n_cus, nullptr, 0, nullptr, params, cfi, nullptr, 0, nullptr, 0);
+}
+
+int vvclf_apply_deblocking_clipped(vvclf_picture* out, const vvclf_deblock_cu* cus, uint32_t n_cus,
+                                   const vvclf_deblock_cu* chroma_cus, uint32_t n_chroma_cus,
+                                   const vvclf_mv_grid* mv_grid, const vvclf_deblock_params* params, uint32_t cfi,
+                                   const uint32_t* nfc, uint32_t n_nfc, const uint32_t* nfr, uint32_t n_nfr) {
+  clear_error();
+  if (!out || !params) return fail(VVCLF_ERR_INVALID, "apply_deblocking: NULL argument");
+  if (params->disabled) return VVCLF_OK;
+  vvclf_frame f;
+  frame_zero(f, out, cfi);
+  f.cus = cus;
+  f.n_cus = n_cus;
+  f.has_chroma_cus = chroma_cus != nullptr;
+  f.chroma_cus = chroma_cus;
+  f.n_chroma_cus = n_chroma_cus;
+  f.mv_grid = mv_grid;
+  f.deblock = *params;
+  f.no_filter_cols = nfc;
+  f.n_no_filter_cols = n_nfc;
+  f.no_filter_rows = nfr;
+  f.n_no_filter_rows = n_nfr;
+  int rc = run_sync(&f, STAGE_DBK);
+  return rc;
+}
+
+int vvclf_apply_sao(vvclf_picture* out, const vvclf_sao_picture* sp, const vvclf_sao_config* cfg) {
+  return vvclf_apply_sao_clipped(out, sp, cfg, nullptr);
+}
+
+int vvclf_apply_sao_clipped(vvclf_picture* out, const vvclf_sao_picture* sp, const vvclf_sao_config* cfg,
+                            const vvclf_bounds* tb) {
+  clear_error();
+  if (!out || !sp || !cfg) return fail(VVCLF_ERR_INVALID, "apply_sao: NULL argument");
+  vvclf_frame f;
+  frame_zero(f, out, cfg->chroma_format_idc);
+  f.sao = *sp;
+  f.sao_cfg = *cfg;
+  f.clip_bounds = tb;
+  return run_sync(&f, STAGE_SAO);
+}
+
+int vvclf_apply_alf(vvclf_picture* out, const vvclf_alf_picture* ap, const vvclf_alf_config* cfg,
+                    const vvclf_alf_aps_binding* b) {
+  return vvclf_apply_alf_clipped(out, ap, cfg, b, nullptr);
+}
+
+int vvclf_apply_alf_clipped(vvclf_picture* out, const vvclf_alf_picture* ap, const vvclf_alf_config* cfg,
+                            const vvclf_alf_aps_binding* b, const vvclf_bounds* tb) {
+  clear_error();
+  if (!out || !ap || !cfg) return fail(VVCLF_ERR_INVALID, "apply_alf: NULL argument");
+  vvclf_frame f;
+  frame_zero(f, out, cfg->chroma_format_idc);
+  f.alf = *ap;
+  f.alf_cfg = *cfg;
+  if (b) f.alf_binding = *b;
+  f.clip_bounds = tb;
+  return run_sync(&f, STAGE_ALF);
+}
+
+int vvclf_lmcs_inverse_luma(vvclf_plane* luma, const vvclf_lmcs_binding* b) {
+  clear_error();
+  if (!luma || !b) return fail(VVCLF_ERR_INVALID, "lmcs_inverse_luma: NULL argument");
+  if (!b->derived) return VVCLF_OK;
+  vvclf_picture pic;
+  memset(&pic, 0, sizeof(pic));
+  pic.luma = *luma;
+  vvclf_frame f;
+  frame_zero(f, &pic, 0);
+  f.lmcs = *b;
+  return run_sync(&f, STAGE_LMCS);
+}
+
+int vvclf_apply_in_loop_filters(vvclf_frame* frame) {
+  clear_error();
+  if (!frame) return fail(VVCLF_ERR_INVALID, "apply_in_loop_filters: NULL frame");
+  return run_sync(frame, STAGE_ALL);
+}
+
+// derive_lmcs, src/lmcs.rs:302-404 (host-side table derivation).
+int vvclf_derive_lmcs(const vvclf_lmcs_data* d, uint32_t bd, vvclf_lmcs_derived* o) {
+  clear_error();
+  if (!d || !o) return fail(VVCLF_ERR_INVALID, "derive_lmcs: NULL argument");
+  if (bd < 8 || bd > 16) return fail(VVCLF_ERR_INVALID, "LMCS: BitDepth out of range (8..=16, got %u)", bd);
+  if (d->lmcs_delta_max_bin_idx > 15 || d->lmcs_min_bin_idx > 15)
+    return fail(VVCLF_ERR_INVALID, "LMCS: bin index out of range");
+  vvclf_lmcs_derived r;
+  memset(&r, 0, sizeof(r));
+  const uint32_t org = (1u << bd) / 16, lg = bd - 4;
+  const int64_t lo = org >> 3, hi = (int64_t)(org << 3) - 1;
+  const uint32_t mn = d->lmcs_min_bin_idx, mx = 15u - d->lmcs_delta_max_bin_idx;
+  for (uint32_t i = mn; i <= mx; i++) {
+    int64_t cw = (int64_t)org + (d->lmcs_delta_sign_cw_flag[i] ? -1 : 1) * (int64_t)d->lmcs_delta_abs_cw[i];
+    if (cw < lo || cw > hi) return fail(VVCLF_ERR_INVALID, "LMCS: lmcsCW[%u] = %lld outside band", i, (long long)cw);
+    r.lmcs_cw[i] = (uint32_t)cw;
+  }
+  uint64_t sum = 0;
+  for (uint32_t v : r.lmcs_cw) sum += v;
+  if (sum > (1u << bd) - 1) return fail(VVCLF_ERR_INVALID, "LMCS: sum of lmcsCW (%llu) exceeds budget", (unsigned long long)sum);
+  for (uint32_t i = 0; i < 16; i++) {
+    r.input_pivot[i] = i * org;
+    r.lmcs_pivot[i + 1] = r.lmcs_pivot[i] + r.lmcs_cw[i];
+    r.scale_coeff[i] = (r.lmcs_cw[i] * (1u << 11) + (1u << (lg - 1))) >> lg;
+    r.inv_scale_coeff[i] = r.lmcs_cw[i] ? org * (1u << 11) / r.lmcs_cw[i] : 0;
+  }
+  const uint32_t sh = bd - 5;
+  for (uint32_t i = mn; i <= mx; i++)
+    if (r.lmcs_pivot[i] % (1u << sh) != 0 && (r.lmcs_pivot[i] >> sh) == (r.lmcs_pivot[i + 1] >> sh))
+      return fail(VVCLF_ERR_INVALID, "LMCS: LmcsPivot[%u] bin-crossing clause violated", i);
+  const int32_t crs = (d->lmcs_delta_sign_crs_flag ? -1 : 1) * (int32_t)d->lmcs_delta_abs_crs;
+  for (uint32_t i = 0; i < 16; i++) {
+    if (r.lmcs_cw[i] == 0) {
+      r.chroma_scale_coeff[i] = 1u << 11;
+    } else {
+      int64_t joint = (int64_t)r.lmcs_cw[i] + crs;
+      if (joint < lo || joint > hi) return fail(VVCLF_ERR_INVALID, "LMCS: lmcsCW[%u] + lmcsDeltaCrs outside band", i);
+      r.chroma_scale_coeff[i] = org * (1u << 11) / (uint32_t)joint;
+    }
+  }
+  r.bit_depth = bd;
+  r.org_cw = org;
+  r.lmcs_delta_crs = crs;
+  *o = r;
+  return VVCLF_OK;
+}
+
+// Batches and autotuning are implemented in vvclf_batch.cpp.
+
+}  // extern "C"

@@ -1,0 +1,377 @@
+// vvclf_filters.h — per-line / per-block filter arithmetic shared by the
+// plain stage kernels and the fused tile kernel.  All integer, bit-exact
+// restatements of the reference sample filters; the callers decide where the
+// samples come from (HBM or LDS) and where they go.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vvclf_internal.h"
+
+namespace vvclf {
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Table 43 (spec data): beta' and tC' indexed by the clamped Q.
+__constant__ static const int16_t kBeta[64] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,
+                                              6,  7,  8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24,
+                                              26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56,
+                                              58, 60, 62, 64, 66, 68, 70, 72, 74, 76, 78, 80, 82, 84, 86, 88};
+__constant__ static const int16_t kTc[66] = {0,  0,  0,  0,  0,  0,   0,   0,   0,   0,   0,   0,   0,   0,
+                                            0,  0,  0,  0,  3,  4,   4,   4,   4,   5,   5,   5,   5,   7,
+                                            7,  8,  9,  10, 10, 11,  13,  14,  15,  17,  19,  21,  24,  25,
+                                            29, 33, 36, 41, 45, 51,  57,  64,  71,  80,  89,  100, 112, 125,
+                                            141, 157, 177, 198, 222, 250, 280, 314, 352, 395};
+
+// beta and tC of a packed edge descriptor (eqs. 1276-1279, src/deblock.rs:439-452).
+__device__ __forceinline__ void edge_thresholds(uint32_t e, int bd, int& beta, int& tc) {
+  int qb = (e >> E_QB_SHIFT) & 63, qt = (e >> E_QT_SHIFT) & 127;
+  beta = (int)kBeta[qb] * (1 << (bd - 8));
+  int tcp = kTc[qt];
+  if (bd < 10) {
+    tc = (tcp + (1 << (9 - bd))) >> (10 - bd);
+  } else {
+    tc = tcp * (1 << (bd - 10));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Luma deblocking, one sample line across the edge: p[0..7] (p0 nearest the
+// edge) and q[0..7].  The decision (src/deblock.rs:1224-1409) combines the
+// per-line terms of lines 0 and 3 of the 4-line segment.
+// ---------------------------------------------------------------------------
+struct luma_terms {
+  int dp, dq, sp, sq, spq, dpl, dql, sp7, sq7, spm, sqm;
+};
+
+__device__ __forceinline__ luma_terms luma_line_terms(const int* p, const int* q, int mflp, int mflq) {
+  luma_terms t;
+  t.dp = abs(p[2] - 2 * p[1] + p[0]);
+  t.dq = abs(q[2] - 2 * q[1] + q[0]);
+  t.sp = abs(p[3] - p[0]);
+  t.sq = abs(q[0] - q[3]);
+  t.spq = abs(p[0] - q[0]);
+  t.dpl = abs(p[5] - 2 * p[4] + p[3]);
+  t.dql = abs(q[5] - 2 * q[4] + q[3]);
+  t.sp7 = abs(p[7] - p[6] - p[5] + p[4]);
+  t.sq7 = abs(q[4] - q[5] - q[6] + q[7]);
+  // |p3 - p_mflP| of eq. 1369; only consumed when the side is large (mfl 5 or 7).
+  t.spm = abs(p[3] - (mflp >= 7 ? p[7] : p[5]));
+  t.sqm = abs(q[3] - (mflq >= 7 ? q[7] : q[5]));
+  return t;
+}
+
+struct luma_decision {
+  int dE, dEp, dEq, mflp, mflq;
+};
+
+// luma_edge_decision, src/deblock.rs:1224-1409, over the terms of lines 0 and 3.
+__device__ __forceinline__ luma_decision luma_decide(const luma_terms& a, const luma_terms& b, int beta, int tc,
+                                                     int mflp, int mflq, bool ctb_row) {
+  bool sp_large = mflp > 3, sq_large = mflq > 3;
+  if (ctb_row) sp_large = false;
+  bool dsam0 = false, dsam3 = false;
+  if (sp_large || sq_large) {
+    int dp0l, dp3l, dq0l, dq3l;
+    if (sp_large) {
+      dp0l = (a.dp + a.dpl + 1) >> 1;
+      dp3l = (b.dp + b.dpl + 1) >> 1;
+    } else {
+      mflp = 3;
+      dp0l = a.dp;
+      dp3l = b.dp;
+    }
+    if (sq_large) {
+      dq0l = (a.dq + a.dql + 1) >> 1;
+      dq3l = (b.dq + b.dql + 1) >> 1;
+    } else {
+      dq0l = a.dq;
+      dq3l = b.dq;
+    }
+    int sp0l = a.sp, sp3l = b.sp, sq0l = a.sq, sq3l = b.sq;
+    if (mflp == 7) {
+      sp0l += a.sp7;
+      sp3l += b.sp7;
+    }
+    if (mflq == 7) {
+      sq0l += a.sq7;
+      sq3l += b.sq7;
+    }
+    int dl = (dp0l + dq0l) + (dp3l + dq3l);
+    if (dl < beta) {
+      int thr1 = (3 * beta) >> 5, thr2 = beta >> 4, tcl = (5 * tc + 1) >> 1;
+      int spin0 = sp_large ? (sp0l + a.spm + 1) >> 1 : sp0l;
+      int sqin0 = sq_large ? (sq0l + a.sqm + 1) >> 1 : sq0l;
+      int spin3 = sp_large ? (sp3l + b.spm + 1) >> 1 : sp3l;
+      int sqin3 = sq_large ? (sq3l + b.sqm + 1) >> 1 : sq3l;
+      dsam0 = 2 * (dp0l + dq0l) < thr2 && spin0 + sqin0 < thr1 && a.spq < tcl;
+      dsam3 = 2 * (dp3l + dq3l) < thr2 && spin3 + sqin3 < thr1 && b.spq < tcl;
+    }
+  }
+  luma_decision d;
+  d.dE = 0;
+  d.dEp = 0;
+  d.dEq = 0;
+  if (dsam0 && dsam3) {
+    d.dE = 3;
+    d.dEp = 1;
+    d.dEq = 1;
+    d.mflp = mflp;
+    d.mflq = mflq;
+    return d;
+  }
+  int dpq0 = a.dp + a.dq, dpq3 = b.dp + b.dq, dp = a.dp + b.dp, dq = a.dq + b.dq, dd = dpq0 + dpq3;
+  bool ssam0 = false, ssam3 = false;
+  if (dd < beta && mflp > 2 && mflq > 2) {
+    int t1 = beta >> 3, t2 = beta >> 2, tcl = (5 * tc + 1) >> 1;
+    ssam0 = 2 * dpq0 < t2 && a.sp + a.sq < t1 && a.spq < tcl;
+    ssam3 = 2 * dpq3 < t2 && b.sp + b.sq < t1 && b.spq < tcl;
+  }
+  if (dd < beta) {
+    d.dE = 1;
+    if (ssam0 && ssam3) {
+      d.dE = 2;
+      mflp = 3;
+      mflq = 3;
+    }
+    if (mflp > 1 && mflq > 1) {
+      int thr = (beta + (beta >> 1)) >> 3;
+      if (dp < thr) d.dEp = 1;
+      if (dq < thr) d.dEq = 1;
+    }
+    if (d.dE == 1) {
+      mflp = 1 + d.dEp;
+      mflq = 1 + d.dEq;
+    }
+  }
+  d.mflp = mflp;
+  d.mflq = mflq;
+  return d;
+}
+
+// Filters one line in registers: np[]/nq[] receive the new values, `nwp` /
+// `nwq` how many samples per side changed (short: src/deblock.rs:1680-1776,
+// long: :1519-1673).  Palette sides are handled by the caller (write skip).
+__device__ __forceinline__ void luma_filter_line(const int* p, const int* q, const luma_decision& d, int tc,
+                                                 int* np, int* nq, int& nwp, int& nwq) {
+  nwp = 0;
+  nwq = 0;
+  if (d.dE == 3) {
+    int mp = d.mflp, mq = d.mflq;
+    int rm;
+    if (mp == 5 && mq == 5) {
+      rm = (p[4] + p[3] + 2 * (p[2] + p[1] + p[0] + q[0] + q[1] + q[2]) + q[3] + q[4] + 8) >> 4;
+    } else if (mp == mq) {
+      // mp == mq here is 7 or 3; eq. 1390 reads up to p6/q6, which the
+      // reference leaves at 0 beyond the fetched depth (mfl + 1 samples).
+      int p4 = mp >= 4 ? p[4] : 0, p5 = mp >= 5 ? p[5] : 0, p6 = mp >= 6 ? p[6] : 0;
+      int q4 = mq >= 4 ? q[4] : 0, q5 = mq >= 5 ? q[5] : 0, q6 = mq >= 6 ? q[6] : 0;
+      rm = (p6 + p5 + p4 + p[3] + p[2] + p[1] + 2 * (p[0] + q[0]) + q[1] + q[2] + q[3] + q4 + q5 + q6 + 8) >> 4;
+    } else if ((mp == 7 && mq == 5) || (mp == 5 && mq == 7)) {
+      rm = (p[5] + p[4] + p[3] + p[2] + 2 * (p[1] + p[0] + q[0] + q[1]) + q[2] + q[3] + q[4] + q[5] + 8) >> 4;
+    } else if ((mp == 5 && mq == 3) || (mp == 3 && mq == 5)) {
+      rm = (p[3] + p[2] + p[1] + p[0] + q[0] + q[1] + q[2] + q[3] + 4) >> 3;
+    } else if (mp == 3 && mq == 7) {
+      rm = (2 * (p[2] + p[1] + p[0] + q[0]) + p[0] + p[1] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + 8) >> 4;
+    } else {
+      // eq. 1394 arm; p6..p1 beyond the fetched depth read as 0.
+      int pp[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) pp[i] = i <= mp ? p[i] : 0;
+      int qq[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) qq[j] = j <= mq ? q[j] : 0;
+      rm = (pp[6] + pp[5] + pp[4] + pp[3] + pp[2] + pp[1] + 2 * (qq[2] + qq[1] + qq[0] + pp[0]) + qq[0] + qq[1] + 8) >>
+           4;
+    }
+    int refp = (p[mp] + p[mp - 1] + 1) >> 1;
+    int refq = (q[mq] + q[mq - 1] + 1) >> 1;
+    const int F7[7] = {59, 50, 41, 32, 23, 14, 5}, F5[5] = {58, 45, 32, 19, 6}, F3[3] = {53, 32, 11};
+    const int T7[7] = {6, 5, 4, 3, 2, 1, 1}, T5[5] = {6, 5, 4, 3, 2}, T3[3] = {6, 4, 2};
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      if (i < mp) {
+        int f = mp == 7 ? F7[i] : (mp == 5 ? F5[i < 5 ? i : 0] : F3[i < 3 ? i : 0]);
+        int t = mp == 7 ? T7[i] : (mp == 5 ? T5[i < 5 ? i : 0] : T3[i < 3 ? i : 0]);
+        int lo = p[i] - ((tc * t) >> 1), hi = p[i] + ((tc * t) >> 1);
+        np[i] = clampi((rm * f + refp * (64 - f) + 32) >> 6, lo, hi);
+      }
+      if (i < mq) {
+        int g = mq == 7 ? F7[i] : (mq == 5 ? F5[i < 5 ? i : 0] : F3[i < 3 ? i : 0]);
+        int t = mq == 7 ? T7[i] : (mq == 5 ? T5[i < 5 ? i : 0] : T3[i < 3 ? i : 0]);
+        int lo = q[i] - ((tc * t) >> 1), hi = q[i] + ((tc * t) >> 1);
+        nq[i] = clampi((rm * g + refq * (64 - g) + 32) >> 6, lo, hi);
+      }
+    }
+    nwp = mp;
+    nwq = mq;
+  } else if (d.dE == 2) {
+    int p3 = p[3], p2 = p[2], p1 = p[1], p0 = p[0], q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    np[0] = clampi((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, p0 - 3 * tc, p0 + 3 * tc);
+    np[1] = clampi((p2 + p1 + p0 + q0 + 2) >> 2, p1 - 2 * tc, p1 + 2 * tc);
+    np[2] = clampi((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, p2 - tc, p2 + tc);
+    nq[0] = clampi((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, q0 - 3 * tc, q0 + 3 * tc);
+    nq[1] = clampi((p0 + q0 + q1 + q2 + 2) >> 2, q1 - 2 * tc, q1 + 2 * tc);
+    nq[2] = clampi((p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3, q2 - tc, q2 + tc);
+    nwp = 3;
+    nwq = 3;
+  } else if (d.dE == 1) {
+    int p2 = p[2], p1 = p[1], p0 = p[0], q0 = q[0], q1 = q[1], q2 = q[2];
+    int delta_raw = (9 * (q0 - p0) - 3 * (q1 - p1) + 8) >> 4;
+    if (abs(delta_raw) < tc * 10) {
+      int delta = clampi(delta_raw, -tc, tc);
+      np[0] = p0 + delta;
+      nq[0] = q0 - delta;
+      nwp = 1;
+      nwq = 1;
+      if (d.dEp == 1) {
+        np[1] = p1 + clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -(tc >> 1), tc >> 1);
+        nwp = 2;
+      }
+      if (d.dEq == 1) {
+        nq[1] = q1 + clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -(tc >> 1), tc >> 1);
+        nwq = 2;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Chroma deblocking, one line: c[0..7] = p3 p2 p1 p0 q0 q1 q2 q3.
+// ---------------------------------------------------------------------------
+// chroma_strong_decision_v/h, src/deblock.rs:1932-1982
+__device__ __forceinline__ bool chroma_strong_row(const int* c, int beta, int tc, bool short_p) {
+  int p1 = c[2];
+  int p3 = short_p ? p1 : c[0];
+  int p2 = short_p ? p1 : c[1];
+  int p0 = c[3], q0 = c[4], q1 = c[5], q2 = c[6], q3 = c[7];
+  int dpq = 2 * (abs(p2 - 2 * p1 + p0) + abs(q2 - 2 * q1 + q0));
+  int edge = abs(p3 - p0) + abs(q0 - q3);
+  return dpq < (beta >> 2) && edge < (beta >> 3) && abs(p0 - q0) < ((5 * tc + 1) >> 1);
+}
+
+// Strong / weak chroma line filter (src/deblock.rs:1803-1926, :1986-2074).
+// n[] receives new values at the same indices as c[]; lo/hi give the index
+// range written.
+__device__ __forceinline__ void chroma_filter_line(const int* c, bool strong, bool short_p, int tc, int* n, int& lo,
+                                                   int& hi) {
+  int p3 = c[0], p2 = c[1], p1 = c[2], p0 = c[3], q0 = c[4], q1 = c[5], q2 = c[6], q3 = c[7];
+  if (strong) {
+    if (short_p) {
+      n[3] = clampi((3 * p1 + 2 * p0 + q0 + q1 + q2 + 4) >> 3, p0 - tc, p0 + tc);
+      n[4] = clampi((2 * p1 + p0 + 2 * q0 + q1 + q2 + q3 + 4) >> 3, q0 - tc, q0 + tc);
+      n[5] = clampi((p1 + p0 + q0 + 2 * q1 + q2 + 2 * q3 + 4) >> 3, q1 - tc, q1 + tc);
+      n[6] = clampi((p0 + q0 + q1 + 2 * q2 + 3 * q3 + 4) >> 3, q2 - tc, q2 + tc);
+      lo = 3;
+      hi = 6;
+      return;
+    }
+    n[3] = clampi((p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2 + 4) >> 3, p0 - tc, p0 + tc);
+    n[2] = clampi((2 * p3 + p2 + 2 * p1 + p0 + q0 + q1 + 4) >> 3, p1 - tc, p1 + tc);
+    n[1] = clampi((3 * p3 + 2 * p2 + p1 + p0 + q0 + 4) >> 3, p2 - tc, p2 + tc);
+    n[4] = clampi((p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3 + 4) >> 3, q0 - tc, q0 + tc);
+    n[5] = clampi((p1 + p0 + q0 + 2 * q1 + q2 + 2 * q3 + 4) >> 3, q1 - tc, q1 + tc);
+    n[6] = clampi((p0 + q0 + q1 + 2 * q2 + 3 * q3 + 4) >> 3, q2 - tc, q2 + tc);
+    lo = 1;
+    hi = 6;
+    return;
+  }
+  int delta = clampi((((q0 - p0) * 4) + p1 - q1 + 4) >> 3, -tc, tc);
+  n[3] = p0 + delta;
+  n[4] = q0 - delta;
+  lo = 3;
+  hi = 4;
+}
+
+// ---------------------------------------------------------------------------
+// SAO (src/sao.rs:504-626) for one sample, given the pre-SAO sample and its
+// two EO neighbours (ignored for band offset).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int sao_offset(int off_sel, uint64_t prm) {
+  // offsets are 14-bit two's complement fields
+  int raw = (int)((prm >> (8 + 14 * (off_sel - 1))) & 0x3fff);
+  return (raw ^ 0x2000) - 0x2000;
+}
+
+__device__ __forceinline__ int sao_edge_sample(int p, int n0, int n1, uint64_t prm, int maxv) {
+  int e = 2 + ((p > n0) - (p < n0)) + ((p > n1) - (p < n1));
+  if (e <= 2) e = e == 2 ? 0 : e + 1;
+  int off = e >= 1 ? sao_offset(e, prm) : 0;
+  return clampi(p + off, 0, maxv);
+}
+
+__device__ __forceinline__ int sao_band_sample(int p, uint64_t prm, int bd, int maxv) {
+  int pos = (int)((prm >> 2) & 31);
+  int band = bd >= 5 ? ((p >> (bd - 5)) & 31) : (p & 31);
+  int k = (band - pos) & 31;
+  int off = k < 4 ? sao_offset(k + 1, prm) : 0;
+  return clampi(p + off, 0, maxv);
+}
+
+// Table 44 neighbour offsets per EO class.
+__device__ __forceinline__ void eo_offsets(int cls, int& h0, int& v0, int& h1, int& v1) {
+  switch (cls & 3) {
+    case 0: h0 = -1; v0 = 0; h1 = 1; v1 = 0; break;
+    case 1: h0 = 0; v0 = -1; h1 = 0; v1 = 1; break;
+    case 2: h0 = -1; v0 = -1; h1 = 1; v1 = 1; break;
+    default: h0 = 1; v0 = -1; h1 = -1; v1 = 1; break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ALF helpers
+// ---------------------------------------------------------------------------
+// table_45, src/alf.rs:1117-1130 (applyAlfLineBufBoundary = 1): returns
+// shift and the three vertical offsets.
+__device__ __forceinline__ void alf_t45(int j, int ctb, int& sh, int& y1, int& y2, int& y3) {
+  sh = 7; y1 = 1; y2 = 2; y3 = 3;
+  if (j == ctb - 5 || j == ctb - 4) { sh = 10; y1 = 0; y2 = 0; y3 = 0; }
+  else if (j == ctb - 6 || j == ctb - 3) { y2 = 1; y3 = 1; }
+  else if (j == ctb - 7 || j == ctb - 2) { y3 = 2; }
+}
+
+// table_46, src/alf.rs:1133-1143
+__device__ __forceinline__ void alf_t46(int j, int h, int& sh, int& y1, int& y2) {
+  sh = 7; y1 = 1; y2 = 2;
+  if (j == h - 2 || j == h - 3) { sh = 10; y1 = 0; y2 = 0; }
+  else if (j == h - 1 || j == h - 4) { y2 = 1; }
+}
+
+// cc_alf_table_47, src/alf.rs:528-539
+__device__ __forceinline__ void alf_t47(int yl, int ctb, bool lbb, int& y1, int& y2) {
+  y1 = 1; y2 = 2;
+  if (lbb) {
+    if (yl == ctb - 5 || yl == ctb - 4) { y1 = 0; y2 = 0; }
+    else if (yl == ctb - 6 || yl == ctb - 3) { y1 = 1; y2 = 1; }
+  }
+}
+
+// One clipped tap pair: f * (clamp(a - c, -k, k) + clamp(b - c, -k, k)).
+__device__ __forceinline__ int alf_pair(uint32_t w, int a, int b, int c) {
+  int f = (int)(int16_t)(w & 0xffff);
+  int k = (int)(w >> 16);
+  return f * (clampi(a - c, -k, k) + clampi(b - c, -k, k));
+}
+
+// 4x4-block classification (src/alf.rs:822-1010) from the Laplacian sums.
+__device__ __forceinline__ int alf_class_from_sums(int sh, int sv, int sd0, int sd1, int ac, int bd) {
+  const uint8_t VAR_TAB[16] = {0, 1, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3, 4};
+  const uint8_t TRANSPOSE[8] = {0, 1, 0, 2, 2, 3, 1, 3};
+  int hv1, hv0, dir_hv, d1, d0, dir_d;
+  if (sv > sh) { hv1 = sv; hv0 = sh; dir_hv = 1; } else { hv1 = sh; hv0 = sv; dir_hv = 3; }
+  if (sd0 > sd1) { d1 = sd0; d0 = sd1; dir_d = 0; } else { d1 = sd1; d0 = sd0; dir_d = 2; }
+  bool prefer_d = (int64_t)d1 * hv0 > (int64_t)hv1 * d0;
+  int hvd1 = prefer_d ? d1 : hv1, hvd0 = prefer_d ? d0 : hv0;
+  int dir1 = prefer_d ? dir_d : dir_hv, dir2 = prefer_d ? dir_hv : dir_d;
+  int dir_s = hvd1 * 2 > 9 * hvd0 ? 2 : (hvd1 > 2 * hvd0 ? 1 : 0);
+  int64_t prod = (int64_t)(sh + sv) * ac;
+  int64_t shifted = prod >> (bd - 1);
+  shifted = shifted < 0 ? 0 : (shifted > 15 ? 15 : shifted);
+  int filt = VAR_TAB[shifted];
+  int tr = TRANSPOSE[dir1 * 2 + (dir2 >> 1)];
+  if (dir_s != 0) filt += (((dir1 & 1) << 1) + dir_s) * 5;
+  return filt | (tr << 5);
+}
+
+}  // namespace vvclf

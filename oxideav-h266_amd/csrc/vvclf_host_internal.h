@@ -1,0 +1,73 @@
+// vvclf_host_internal.h — host-only declarations shared by the C ABI sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+
+#include <vector>
+
+#include "../../include/vvclf.h"
+#include "vvclf_internal.h"
+
+namespace vvclf {
+
+enum : uint32_t { STAGE_LMCS = 1, STAGE_DBK = 2, STAGE_SAO = 4, STAGE_ALF = 8, STAGE_ALL = 15 };
+
+// Host copies of the derived tables, before upload.
+struct host_tables {
+  // deblock
+  uint32_t gw = 0, gh = 0, cvw = 0, cvh = 0, chw = 0, chh = 0;
+  std::vector<uint32_t> e_lv, e_lh, e_cv[2], e_ch[2];
+  bool has_chroma_dbk = false;
+  uint64_t bs_hist[3] = {0, 0, 0};
+  uint64_t chroma_filtered = 0;
+  // SAO
+  bool do_sao = false, sao_luma = false, sao_chroma = false;
+  uint32_t swctb = 0, shctb = 0, sao_ctb_log2 = 7, sao_bd = 10;
+  std::vector<uint64_t> sao, sao_rect;
+  // ALF
+  bool do_alf = false;
+  uint32_t wctb = 0, hctb = 0, alf_ctb_log2 = 7, alf_bd = 10;
+  std::vector<uint32_t> alf_ctb, alf_bank, alf_cbank;
+  std::vector<int32_t> alf_ccbank;
+  std::vector<uint64_t> alf_rect;
+  // LMCS
+  std::vector<uint16_t> lmcs_lut;
+};
+
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+const char* last_error();
+void clear_error();
+
+int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc, uint32_t Hc, host_tables& t);
+int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t, uint32_t stages);
+void build_lmcs_lut(const vvclf_lmcs_binding& b, std::vector<uint16_t>& lut);
+
+static constexpr int PLAIN_STAGES = 8;
+hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev = nullptr);
+
+}  // namespace vvclf
+
+// Opaque handle bodies (global namespace, matching the C typedefs).
+struct vvclf_ctx {
+  int device;
+  hipStream_t stream;
+};
+
+// Per-stage timing of plain runs: PLAIN_STAGES + 1 events per recorded run.
+struct vvclf_prof {
+  int device;
+  uint32_t cap, used;
+  std::vector<hipEvent_t> ev;
+};
+
+struct vvclf_job {
+  vvclf_ctx* ctx;
+  vvclf::dev_job d;
+  void* dev_planes;   // one allocation: in, a, b, out
+  void* dev_side;     // one allocation: all side tables
+  uint64_t bytes_planes, bytes_side;
+  uint32_t w, h, wc, hc;
+  uint32_t stages;
+};

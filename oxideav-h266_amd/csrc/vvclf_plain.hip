@@ -1,0 +1,461 @@
+// vvclf_plain.hip — the plain per-picture form: one kernel per stage on one
+// stream, picture resident in HBM.
+//   prep      in -> A        LMCS inverse LUT (src/ctu.rs:11837-11849) + copy
+//   dbk_lv    A in place     luma vertical edges   (all V before any H,
+//   dbk_lh    A in place     luma horizontal edges  src/deblock.rs:528-555)
+//   dbk_c     A in place     Cb/Cr vertical, then horizontal
+//   sao       A -> B         src/sao.rs:389-626
+//   alf_luma  B -> out       classification + 7x7 diamond per 4x4 block
+//   alf_chroma B -> out      5x5 chroma ALF + CC-ALF (src/alf.rs:447-523, :1014-1065)
+// Deblocking writes in place: edges of one direction never touch samples
+// another edge of that direction reads (filter extents <= edge spacing), so
+// all segments of a pass run concurrently with the sequential result.
+#include <hip/hip_runtime.h>
+
+#include "vvclf_filters.h"
+#include "vvclf_internal.h"
+
+namespace vvclf {
+
+struct plane_view {
+  uint16_t* p;
+  int pitch, w, h;
+};
+
+__device__ __forceinline__ plane_view pick(const dev_planes& d, const dev_job& j, int comp) {
+  plane_view v;
+  if (comp == 0) {
+    v.p = d.y; v.pitch = j.py; v.w = j.w; v.h = j.h;
+  } else {
+    v.p = comp == 1 ? d.cb : d.cr; v.pitch = j.pc; v.w = j.wc; v.h = j.hc;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int rdc(const plane_view& v, int x, int y) {
+  x = clampi(x, 0, v.w - 1);
+  y = clampi(y, 0, v.h - 1);
+  return v.p[(size_t)y * v.pitch + x];
+}
+
+// ---------------------------------------------------------------------------
+// prep: 8 samples per thread, rows of every plane (blockIdx.z = component).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_prep(dev_job j) {
+  int comp = blockIdx.z;
+  plane_view src = pick(j.in, j, comp), dst = pick(j.a, j, comp);
+  int y = blockIdx.y;
+  int x = (blockIdx.x * 256 + threadIdx.x) * 8;
+  if (y >= src.h || x >= src.w) return;
+  const uint4* s = reinterpret_cast<const uint4*>(src.p + (size_t)y * src.pitch + x);
+  uint4 v = *s;
+  if (comp == 0 && j.do_lmcs) {
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      uint32_t lo = j.lmcs_lut[w[i] & 0xffff], hi = j.lmcs_lut[w[i] >> 16];
+      w[i] = lo | (hi << 16);
+    }
+    v = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  *reinterpret_cast<uint4*>(dst.p + (size_t)y * dst.pitch + x) = v;
+}
+
+// plain copy between buffers (used when a stage is inactive)
+__global__ void __launch_bounds__(256) k_copy(dev_job j, dev_planes from, dev_planes to) {
+  int comp = blockIdx.z;
+  plane_view src = pick(from, j, comp), dst = pick(to, j, comp);
+  int y = blockIdx.y;
+  int x = (blockIdx.x * 256 + threadIdx.x) * 8;
+  if (y >= src.h || x >= src.w) return;
+  *reinterpret_cast<uint4*>(dst.p + (size_t)y * dst.pitch + x) =
+      *reinterpret_cast<const uint4*>(src.p + (size_t)y * src.pitch + x);
+}
+
+// ---------------------------------------------------------------------------
+// Luma deblocking.  One lane per sample line; the four lines of a segment sit
+// in four consecutive lanes, which exchange the decision terms of lines 0 and
+// 3.  Block = 4 waves; each wave covers 16 segments.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ luma_terms shfl_terms(const luma_terms& t, int src) {
+  luma_terms r;
+  r.dp = __shfl(t.dp, src); r.dq = __shfl(t.dq, src); r.sp = __shfl(t.sp, src);
+  r.sq = __shfl(t.sq, src); r.spq = __shfl(t.spq, src); r.dpl = __shfl(t.dpl, src);
+  r.dql = __shfl(t.dql, src); r.sp7 = __shfl(t.sp7, src); r.sq7 = __shfl(t.sq7, src);
+  r.spm = __shfl(t.spm, src); r.sqm = __shfl(t.sqm, src);
+  return r;
+}
+
+template <bool VERT>
+__global__ void __launch_bounds__(256) k_dbk_luma(dev_job j) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int seg_local = lane >> 2, line = lane & 3;
+  int gx, gy;
+  if (VERT) {
+    gx = blockIdx.x * 16 + seg_local;  // edge column (x = 4*gx)
+    gy = blockIdx.y * 4 + wave;        // segment row
+  } else {
+    gx = blockIdx.x * 16 + seg_local;  // segment column
+    gy = blockIdx.y * 4 + wave;        // edge row (y = 4*gy)
+  }
+  if (gx >= j.gw || gy >= j.gh) return;  // whole quad leaves together
+  uint32_t e = (VERT ? j.e_lv : j.e_lh)[(size_t)gy * j.gw + gx];
+  if (e == 0) return;
+  plane_view v = pick(j.a, j, 0);
+  const int x0 = VERT ? 4 * gx : 4 * gx + line;  // q0 position of this line
+  const int y0 = VERT ? 4 * gy + line : 4 * gy;
+  int p[8], q[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    p[i] = VERT ? rdc(v, x0 - 1 - i, y0) : rdc(v, x0, y0 - 1 - i);
+    q[i] = VERT ? rdc(v, x0 + i, y0) : rdc(v, x0, y0 + i);
+  }
+  int beta, tc;
+  edge_thresholds(e, j.bd_dbk, beta, tc);
+  int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
+  luma_terms mine = luma_line_terms(p, q, mflp, mflq);
+  int base = lane & ~3;
+  luma_terms t0 = shfl_terms(mine, base), t3 = shfl_terms(mine, base + 3);
+  luma_decision d = luma_decide(t0, t3, beta, tc, mflp, mflq, (e & E_CTBROW_BIT) != 0);
+  int np[8], nq[8], nwp, nwq;
+  luma_filter_line(p, q, d, tc, np, nq, nwp, nwq);
+  if (e & E_PLTP_BIT) nwp = 0;
+  if (e & E_PLTQ_BIT) nwq = 0;
+  const int maxv = (1 << j.bd) - 1;
+  if (VERT) {
+    if (y0 >= v.h) return;
+    uint16_t* row = v.p + (size_t)y0 * v.pitch;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      if (i < nwp && x0 - 1 - i >= 0) row[x0 - 1 - i] = (uint16_t)clampi(np[i], 0, maxv);
+      if (i < nwq && x0 + i < v.w) row[x0 + i] = (uint16_t)clampi(nq[i], 0, maxv);
+    }
+  } else {
+    if (x0 >= v.w) return;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      if (i < nwp && y0 - 1 - i >= 0) v.p[(size_t)(y0 - 1 - i) * v.pitch + x0] = (uint16_t)clampi(np[i], 0, maxv);
+      if (i < nwq && y0 + i < v.h) v.p[(size_t)(y0 + i) * v.pitch + x0] = (uint16_t)clampi(nq[i], 0, maxv);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Chroma deblocking: one lane per chroma line, the two lines of a segment in
+// lanes 2k, 2k+1.  blockIdx.z = Cb / Cr.
+// ---------------------------------------------------------------------------
+template <bool VERT>
+__global__ void __launch_bounds__(256) k_dbk_chroma(dev_job j) {
+  const int comp = blockIdx.z + 1;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int line = t & 1;
+  int mw = VERT ? j.cvw : j.chw, mh = VERT ? j.cvh : j.chh;
+  int seg = t >> 1;
+  int sx = seg % mw, sy = seg / mw;
+  if (sy >= mh) return;
+  uint32_t e = (VERT ? j.e_cv[comp - 1] : j.e_ch[comp - 1])[(size_t)sy * mw + sx];
+  if (e == 0) return;
+  plane_view v = pick(j.a, j, comp);
+  int x0, y0;  // q0 of this line
+  if (VERT) { x0 = 8 * sx; y0 = 2 * sy + line; } else { x0 = 2 * sx + line; y0 = 8 * sy; }
+  int c[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) c[i] = VERT ? rdc(v, x0 - 4 + i, y0) : rdc(v, x0, y0 - 4 + i);
+  int beta, tc;
+  edge_thresholds(e, j.bd_dbk, beta, tc);
+  int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
+  bool eligible, short_p;
+  if (VERT) { eligible = mflp == 3 && mflq == 3; short_p = false; }
+  else { eligible = mflq == 3 && (mflp == 3 || mflp == 1); short_p = mflp == 1; }
+  bool mine = eligible && chroma_strong_row(c, beta, tc, short_p);
+  bool other = __shfl_xor((int)mine, 1) != 0;
+  bool strong = eligible && mine && other;
+  int n[8], lo, hi;
+  chroma_filter_line(c, strong, short_p, tc, n, lo, hi);
+  const int maxv = (1 << j.bd) - 1;
+  bool wp = !(e & E_PLTP_BIT), wq = !(e & E_PLTQ_BIT);
+  for (int i = lo; i <= hi; i++) {
+    if ((i < 4 && !wp) || (i >= 4 && !wq)) continue;
+    int x = VERT ? x0 - 4 + i : x0, y = VERT ? y0 : y0 - 4 + i;
+    if (x < 0 || y < 0 || x >= v.w || y >= v.h) continue;
+    v.p[(size_t)y * v.pitch + x] = (uint16_t)clampi(n[i], 0, maxv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SAO: A -> B, 4 samples per thread.  blockIdx.z = component.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_sao(dev_job j) {
+  const int comp = blockIdx.z;
+  plane_view src = pick(j.a, j, comp), dst = pick(j.b, j, comp);
+  int y = blockIdx.y;
+  int x0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (y >= src.h || x0 >= src.w) return;
+  const int sub = comp ? 1 : 0;
+  const int log2c = j.sao_ctb_log2 - sub;
+  bool active = comp == 0 ? j.sao_luma : j.sao_chroma;
+  const int maxv = (1 << j.sao_bd) - 1;
+  const uint16_t* srow = src.p + (size_t)y * src.pitch;
+  uint16_t* drow = dst.p + (size_t)y * dst.pitch;
+  for (int i = 0; i < 4; i++) {
+    int x = x0 + i;
+    if (x >= src.w) break;
+    int pv = srow[x];
+    int out = pv;
+    int rx = x >> log2c, ry = y >> log2c;
+    if (active && rx < j.swctb && ry < j.shctb) {
+      size_t ci = (size_t)ry * j.swctb + rx;
+      uint64_t prm = j.sao[ci * 3 + comp];
+      int type = (int)(prm & 3);
+      if (type == 2) {
+        int h0, v0, h1, v1;
+        eo_offsets((int)((prm >> 2) & 31), h0, v0, h1, v1);
+        int nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
+        bool outside = nx0 < 0 || ny0 < 0 || nx1 < 0 || ny1 < 0 || nx0 >= src.w || ny0 >= src.h || nx1 >= src.w ||
+                       ny1 >= src.h;
+        if (j.sao_rect) {
+          uint64_t r = j.sao_rect[ci];
+          int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff),
+              cy1 = (int)(r >> 48);
+          if (sub) {
+            cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1;
+          }
+          outside = outside || nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
+                    nx1 >= cx1 || ny1 >= cy1;
+        }
+        if (outside) {
+          out = clampi(pv, 0, maxv);
+        } else {
+          int n0 = src.p[(size_t)ny0 * src.pitch + nx0], n1 = src.p[(size_t)ny1 * src.pitch + nx1];
+          out = sao_edge_sample(pv, n0, n1, prm, maxv);
+        }
+      } else if (type == 1) {
+        out = sao_band_sample(pv, prm, j.sao_bd, maxv);
+      }
+    }
+    drow[x] = (uint16_t)out;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ALF luma: one thread per 4x4 block.  The block's 10x10 neighbourhood
+// (-3..+6) feeds both the classification window and the 7x7 diamond.
+// A workgroup covers one CTB-aligned tile and stages that CTB's permuted
+// filter bank in LDS.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
+  __shared__ uint32_t bank[ALF_BANK_WORDS];
+  const int tile = 1 << tile_log2, tb = tile >> 2;
+  const int bx = threadIdx.x % tb, by = threadIdx.x / tb;
+  const int tx0 = blockIdx.x * tile, ty0 = blockIdx.y * tile;
+  const int ctb = 1 << j.ctb_log2;
+  const int rx = tx0 >> j.ctb_log2, ry = ty0 >> j.ctb_log2;
+  plane_view src = pick(j.b, j, 0), dst = pick(j.out, j, 0);
+  // CTBs outside the ALF picture's grid are left unfiltered (copied).
+  const bool in_grid = rx < j.wctb && ry < j.hctb;
+  const size_t ci = in_grid ? (size_t)ry * j.wctb + rx : 0;
+  uint32_t cw = (j.do_alf && in_grid) ? j.alf_ctb[ci] : 0xffffffffu;
+  int bank_idx = (int)(cw & 0xff);
+  const int x4 = tx0 + bx * 4, y4 = ty0 + by * 4;
+  if (bank_idx == (int)ALF_OFF8) {
+    if (by >= tb) return;
+    for (int r = 0; r < 4; r++) {
+      int y = y4 + r;
+      if (y >= src.h) break;
+      for (int c = 0; c < 4; c++) {
+        int x = x4 + c;
+        if (x < src.w) dst.p[(size_t)y * dst.pitch + x] = src.p[(size_t)y * src.pitch + x];
+      }
+    }
+    return;
+  }
+  for (int i = threadIdx.x; i < ALF_BANK_WORDS; i += blockDim.x)
+    bank[i] = j.alf_bank[(size_t)bank_idx * ALF_BANK_WORDS + i];
+  __syncthreads();
+  if (by >= tb) return;
+  if (x4 >= src.w || y4 >= src.h) return;
+  uint64_t r = j.alf_rect[ci];
+  const int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff) - 1,
+            cy1 = (int)(r >> 48) - 1;
+  // 10x10 window, rect-clamped: W[r][c] = sample(x4-3+c, y4-3+r)
+  int W[10][10];
+#pragma unroll
+  for (int rr = 0; rr < 10; rr++) {
+    int yy = clampi(y4 - 3 + rr, cy0, cy1);
+    const uint16_t* row = src.p + (size_t)yy * src.pitch;
+#pragma unroll
+    for (int cc = 0; cc < 10; cc++) W[rr][cc] = row[clampi(x4 - 3 + cc, cx0, cx1)];
+  }
+  // classification (src/alf.rs:822-1010)
+  const int y_ctb = ry * ctb;
+  const int yb = y4 - y_ctb;  // y4 within the CTB
+  const int ph = j.h;
+  const bool carve = !(y_ctb + ctb >= ph) || (ph - y_ctb) > ctb - 4;
+  int min_y = -2, max_y = 5, ac = 2;
+  if (yb == ctb - 8 && carve) { max_y = 3; ac = 3; }
+  else if (yb == ctb - 4 && carve) { min_y = 0; ac = 3; }
+  const int vb = y_ctb + ctb - 4;
+  const bool has_top = yb >= ctb - 4;
+  const bool has_bot = (ctb - 4 - yb) > 0 && (ctb - 4 - yb) < 5;
+  // padded row -> window row index
+  auto prow = [&](int yy) -> int {
+    if (has_top && yy < vb) yy = vb;
+    if (has_bot && yy > vb - 1) yy = vb - 1;
+    return yy - (y4 - 3);
+  };
+  int sh = 0, sv = 0, sd0 = 0, sd1 = 0;
+  for (int jj = min_y; jj <= max_y; jj++) {
+    int rc = prow(y4 + jj), ru = prow(y4 + jj - 1), rd = prow(y4 + jj + 1);
+#pragma unroll
+    for (int ii = -2; ii <= 5; ii++) {
+      if (((ii & 1) != (jj & 1))) continue;
+      int cc = ii + 3;
+      int c2 = W[rc][cc] * 2;
+      sh += abs(c2 - W[rc][cc - 1] - W[rc][cc + 1]);
+      sv += abs(c2 - W[ru][cc] - W[rd][cc]);
+      sd0 += abs(c2 - W[ru][cc - 1] - W[rd][cc + 1]);
+      sd1 += abs(c2 - W[ru][cc + 1] - W[rd][cc - 1]);
+    }
+  }
+  int cls = alf_class_from_sums(sh, sv, sd0, sd1, ac, j.alf_bd);
+  const uint32_t* f = bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
+  uint32_t fw[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) fw[k] = f[k];
+  const int maxv = (1 << j.alf_bd) - 1;
+  for (int rr = 0; rr < 4; rr++) {
+    int y = y4 + rr;
+    if (y >= src.h) break;
+    int sft, y1, y2, y3;
+    alf_t45(y - y_ctb, ctb, sft, y1, y2, y3);
+    int R = rr + 3;
+    for (int c = 0; c < 4; c++) {
+      int x = x4 + c;
+      if (x >= src.w) break;
+      int C = c + 3;
+      int cur = W[R][C];
+      int sum = 0;
+      sum += alf_pair(fw[0], W[R + y3][C], W[R - y3][C], cur);
+      sum += alf_pair(fw[1], W[R + y2][C + 1], W[R - y2][C - 1], cur);
+      sum += alf_pair(fw[2], W[R + y2][C], W[R - y2][C], cur);
+      sum += alf_pair(fw[3], W[R + y2][C - 1], W[R - y2][C + 1], cur);
+      sum += alf_pair(fw[4], W[R + y1][C + 2], W[R - y1][C - 2], cur);
+      sum += alf_pair(fw[5], W[R + y1][C + 1], W[R - y1][C - 1], cur);
+      sum += alf_pair(fw[6], W[R + y1][C], W[R - y1][C], cur);
+      sum += alf_pair(fw[7], W[R + y1][C - 1], W[R - y1][C + 1], cur);
+      sum += alf_pair(fw[8], W[R + y1][C - 2], W[R - y1][C + 2], cur);
+      sum += alf_pair(fw[9], W[R][C + 3], W[R][C - 3], cur);
+      sum += alf_pair(fw[10], W[R][C + 2], W[R][C - 2], cur);
+      sum += alf_pair(fw[11], W[R][C + 1], W[R][C - 1], cur);
+      int val = cur + ((sum + (1 << (sft - 1))) >> sft);
+      dst.p[(size_t)y * dst.pitch + x] = (uint16_t)clampi(val, 0, maxv);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ALF chroma + CC-ALF: one thread per chroma sample, B -> out.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
+  const int comp = blockIdx.z + 1;
+  plane_view src = pick(j.b, j, comp), dst = pick(j.out, j, comp);
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= src.w || y >= src.h) return;
+  int cur = src.p[(size_t)y * src.pitch + x];
+  int out = cur;
+  const int ctb = 1 << j.ctb_log2, ctbc = ctb >> 1;
+  const int rx = (2 * x) >> j.ctb_log2, ry = (2 * y) >> j.ctb_log2;
+  if (j.do_alf && rx < j.wctb && ry < j.hctb) {
+    const size_t ci = (size_t)ry * j.wctb + rx;
+    uint32_t cw = j.alf_ctb[ci];
+    uint64_t r = j.alf_rect[ci];
+    const int lx0 = (int)(r & 0xffff), ly0 = (int)((r >> 16) & 0xffff), lx1 = (int)((r >> 32) & 0xffff),
+              ly1 = (int)(r >> 48);
+    const int maxv = (1 << j.alf_bd) - 1;
+    int alt = (int)((cw >> (comp == 1 ? 8 : 16)) & 0xff);
+    if (alt != (int)ALF_OFF8) {
+      const int cx0 = lx0 >> 1, cy0 = ly0 >> 1, cx1 = ((lx1 + 1) >> 1) - 1, cy1 = ((ly1 + 1) >> 1) - 1;
+      const uint32_t* f = j.alf_cbank + alt * 6;
+      int sft, y1, y2;
+      alf_t46(y - ry * ctbc, ctbc, sft, y1, y2);
+      auto S = [&](int dx, int dy) -> int {
+        return src.p[(size_t)clampi(y + dy, cy0, cy1) * src.pitch + clampi(x + dx, cx0, cx1)];
+      };
+      int c0 = S(0, 0);
+      int sum = 0;
+      sum += alf_pair(f[0], S(0, y2), S(0, -y2), c0);
+      sum += alf_pair(f[1], S(1, y1), S(-1, -y1), c0);
+      sum += alf_pair(f[2], S(0, y1), S(0, -y1), c0);
+      sum += alf_pair(f[3], S(-1, y1), S(1, -y1), c0);
+      sum += alf_pair(f[4], S(2, 0), S(-2, 0), c0);
+      sum += alf_pair(f[5], S(1, 0), S(-1, 0), c0);
+      out = clampi(c0 + ((sum + (1 << (sft - 1))) >> sft), 0, maxv);
+    }
+    int ccf = (int)((cw >> (comp == 1 ? 24 : 28)) & 0xf);
+    if (ccf != (int)ALF_OFF4) {
+      const int32_t* cf = j.alf_ccbank + (comp - 1) * ALF_CCBANK_WORDS + ccf * 7;
+      plane_view lum = pick(j.b, j, 0);
+      const int xl = 2 * x, yl = 2 * y;
+      const int y_ctb_c = ry * ctbc;
+      const int ph = j.h;
+      const bool lbb = !(yl + ctb >= ph && (ph - y_ctb_c * 2) <= ctb - 4);
+      int yp1, yp2;
+      alf_t47((y - y_ctb_c) * 2, ctb, lbb, yp1, yp2);
+      auto L = [&](int dx, int dy) -> int {
+        return lum.p[(size_t)clampi(yl + dy, ly0, ly1 - 1) * lum.pitch + clampi(xl + dx, lx0, lx1 - 1)];
+      };
+      int cen = L(0, 0);
+      int sum = cf[0] * (L(0, -yp1) - cen) + cf[1] * (L(-1, 0) - cen) + cf[2] * (L(1, 0) - cen) +
+                cf[3] * (L(-1, yp1) - cen) + cf[4] * (L(0, yp1) - cen) + cf[5] * (L(1, yp1) - cen) +
+                cf[6] * (L(0, yp2) - cen);
+      const int half = 1 << (j.alf_bd - 1);
+      int sc = clampi((sum + 64) >> 7, -half, half - 1);
+      out = clampi(out + sc, 0, maxv);
+    }
+  }
+  dst.p[(size_t)y * dst.pitch + x] = (uint16_t)out;
+}
+
+// ---------------------------------------------------------------------------
+// Host launcher for the plain form.
+// ---------------------------------------------------------------------------
+static inline unsigned cdiv(unsigned a, unsigned b) { return (a + b - 1) / b; }
+
+// Stage order of the plain form; `ev` (optional, PLAIN_STAGES + 1 events)
+// brackets every stage so callers can time each kernel on this stream.
+hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
+  const int ncomp = j.wc > 0 ? 3 : 1;
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], s);
+  };
+  mark(0);
+  // prep: in -> A (rows of 2048 samples per block)
+  hipLaunchKernelGGL(k_prep, dim3(cdiv(j.w, 2048), j.h, ncomp), dim3(256), 0, s, j);
+  mark(1);
+  if (j.do_dbk) hipLaunchKernelGGL(k_dbk_luma<true>, dim3(cdiv(j.gw, 16), cdiv(j.gh, 4)), dim3(256), 0, s, j);
+  mark(2);
+  if (j.do_dbk) hipLaunchKernelGGL(k_dbk_luma<false>, dim3(cdiv(j.gw, 16), cdiv(j.gh, 4)), dim3(256), 0, s, j);
+  mark(3);
+  bool dc = j.do_dbk && j.do_dbk_chroma && ncomp == 3;
+  if (dc) hipLaunchKernelGGL(k_dbk_chroma<true>, dim3(cdiv(2u * j.cvw * j.cvh, 256), 1, 2), dim3(256), 0, s, j);
+  mark(4);
+  if (dc) hipLaunchKernelGGL(k_dbk_chroma<false>, dim3(cdiv(2u * j.chw * j.chh, 256), 1, 2), dim3(256), 0, s, j);
+  mark(5);
+  if (j.do_sao) hipLaunchKernelGGL(k_sao, dim3(cdiv(j.w, 1024), j.h, ncomp), dim3(256), 0, s, j);
+  mark(6);
+  if (j.do_alf) {
+    int tl = j.ctb_log2 < 6 ? j.ctb_log2 : 6;
+    int tile = 1 << tl;
+    unsigned threads = (unsigned)((tile / 4) * (tile / 4));
+    hipLaunchKernelGGL(k_alf_luma, dim3(cdiv(j.w, tile), cdiv(j.h, tile)), dim3(threads < 64 ? 64 : threads), 0, s, j,
+                       tl);
+  }
+  mark(7);
+  if (j.do_alf && ncomp == 3)
+    hipLaunchKernelGGL(k_alf_chroma, dim3(cdiv(j.wc, 64), cdiv(j.hc, 4), 2), dim3(256), 0, s, j);
+  mark(8);
+  return hipGetLastError();
+}
+
+}  // namespace vvclf

@@ -1,0 +1,109 @@
+"""CPU checks of the C ABI library: it loads, exports every entry point the
+header declares, agrees with the Python struct mirrors, and its host-side
+derivation validates descriptors the way the reference's preconditions say.
+No compute runs here (no device)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "vvclf.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vvclf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported(vv):
+    names = header_functions()
+    assert len(names) >= 28
+    missing = [n for n in names if not hasattr(vv.lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match(vv):
+    for name, st in vv.STRUCTS.items():
+        assert vv.lib.vvclf_abi_sizeof(name.encode()) == C.sizeof(st), name
+    assert vv.lib.vvclf_abi_sizeof(b"nope") == 0
+
+
+def test_frame_check_accepts_synthetic_workloads(vv):
+    for cfg, (w, h), kw in [(0, (416, 240), {}), (1, (832, 480), {"rich": 1, "tiles": 1}),
+                            (3, (416, 240), {}), (0, (256, 128), {"rich": 1})]:
+        f = vv.SynthFrame(cfg, w, h, seed=5, **kw)
+        st = (vv.u64 * 8)()
+        assert vv.lib.vvclf_frame_check(f.desc, st) == 0, vv.lib.vvclf_last_error()
+        assert st[0] + st[1] + st[2] > 0
+        assert st[6] > 4 * w * h * 2
+
+
+def test_bs_histogram_of_the_1080p_workload(vv):
+    # the 1920x1080 workload targets bS (0, 1, 2) ~ (0.5, 0.3, 0.2)
+    f = vv.SynthFrame(0, 1920, 1080, seed=0)
+    st = (vv.u64 * 8)()
+    assert vv.lib.vvclf_frame_check(f.desc, st) == 0
+    tot = st[0] + st[1] + st[2]
+    frac = [st[i] / tot for i in range(3)]
+    assert abs(frac[0] - 0.5) < 0.08 and abs(frac[1] - 0.3) < 0.06 and abs(frac[2] - 0.2) < 0.05
+
+
+def test_overlapping_cus_are_rejected(vv):
+    from helpers import cu, cu_array, dparams
+    f = vv.SynthFrame(0, 64, 64, seed=1, sao=0, alf=0)
+    d = f.desc.contents
+    # two records claim the same vertical edge segments at x = 16
+    arr = cu_array(vv, [cu(vv, 0, 0, 16, 64, 30), cu(vv, 16, 0, 48, 64, 30), cu(vv, 16, 0, 16, 64, 30)])
+    d.cus = C.cast(arr, C.POINTER(vv.DeblockCu))
+    d.n_cus = 3
+    d.mv_grid = C.POINTER(vv.MvGrid)()
+    assert vv.lib.vvclf_frame_check(f.desc, None) == vv.ERR_INVALID
+    assert b"overlap" in vv.lib.vvclf_last_error()
+
+
+def test_unsupported_chroma_format(vv):
+    f = vv.SynthFrame(0, 64, 64, seed=1)
+    f.desc.contents.chroma_format_idc = 3
+    assert vv.lib.vvclf_frame_check(f.desc, None) == vv.ERR_UNSUPPORTED
+
+
+def test_short_luma_aps_is_invalid(vv):
+    f = vv.SynthFrame(0, 128, 128, seed=1)
+    aps = f.desc.contents.alf_binding.luma_apses[0].contents
+    aps.n_luma_coeff = 3
+    assert vv.lib.vvclf_frame_check(f.desc, None) == vv.ERR_INVALID
+
+
+def test_derive_lmcs_matches_oracle(vv, orc):
+    rng = np.random.default_rng(0)
+    for trial in range(300):
+        d = vv.LmcsData()
+        d.lmcs_min_bin_idx = int(rng.integers(0, 4))
+        d.lmcs_delta_max_bin_idx = int(rng.integers(0, 4))
+        for i in range(16):
+            d.lmcs_delta_abs_cw[i] = int(rng.integers(0, 40))
+            d.lmcs_delta_sign_cw_flag[i] = int(rng.integers(0, 2))
+        d.lmcs_delta_abs_crs = int(rng.integers(0, 8))
+        d.lmcs_delta_sign_crs_flag = int(rng.integers(0, 2))
+        bd = int(rng.integers(8, 13))
+        a, b = vv.LmcsDerived(), vv.LmcsDerived()
+        ra = vv.lib.vvclf_derive_lmcs(C.byref(d), bd, C.byref(a))
+        rb = orc.oracle_derive_lmcs(C.byref(d), bd, C.byref(b))
+        assert (ra == 0) == (rb == 0)
+        if ra == 0:
+            assert bytes(a) == bytes(b)
+
+
+def test_package_fails_loudly_without_library(tmp_path):
+    import importlib.util
+    import shutil
+    pkg = tmp_path / "pkg"
+    pkg.mkdir()
+    shutil.copy(os.path.join(ROOT, "oxideav-h266_amd", "__init__.py"), pkg / "__init__.py")
+    spec = importlib.util.spec_from_file_location("vv_missing", pkg / "__init__.py")
+    mod = importlib.util.module_from_spec(spec)
+    with pytest.raises(ImportError):
+        spec.loader.exec_module(mod)
