@@ -337,6 +337,9 @@ int vvclf_frame_check(const vvclf_frame* frame, uint64_t stats[8]);
 /* Context: one per device. */
 int vvclf_ctx_create(int device, vvclf_ctx** out);
 int vvclf_ctx_destroy(vvclf_ctx* ctx);
+/* The context's HIP stream (hipStream_t) and a wait for all its work. */
+void* vvclf_ctx_stream(vvclf_ctx* ctx);
+int vvclf_ctx_synchronize(vvclf_ctx* ctx);
 
 /* Job: one picture plus its side information, resident in HBM.  The host
  * derives the compact per-edge and per-CTB tables once here; input samples
@@ -390,6 +393,52 @@ int vvclf_batch_graph_launch(vvclf_batch* batch, void* stream);
 int vvclf_batch_get_tune(const vvclf_batch* batch, vvclf_tune* out);
 /* Pick a tile shape for the given jobs by timing candidates on the device. */
 int vvclf_autotune(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, vvclf_tune* out);
+
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching over many streams.                           */
+/* ------------------------------------------------------------------------ */
+typedef struct vvclf_server_config {
+  uint64_t hbm_budget_bytes; /* admission cap on device bytes of admitted jobs */
+  uint32_t max_inflight;     /* admitted jobs not yet retired (slots) */
+  uint32_t max_batch;        /* jobs per fused launch */
+  uint32_t prep_threads;     /* host threads deriving tables and uploading */
+  uint32_t pad;
+  vvclf_tune tune;           /* fused tile shape; tile_w == 0 selects 64x64 */
+} vvclf_server_config;
+
+typedef struct vvclf_server_stats {
+  uint64_t submitted, completed, failed, rejected;
+  uint64_t luma_pixels;        /* of completed jobs */
+  uint64_t launches;           /* fused launches issued */
+  uint64_t bytes_admitted_peak;
+  double p50_ms, p99_ms, max_ms, mean_ms; /* submit -> result in the caller's planes */
+  double wall_s;               /* first submit -> last completion */
+} vvclf_server_stats;
+
+/* A server owns three HIP streams (upload, compute, download), a pool of
+ * prep threads, a launch thread and a retire thread.  Jobs submitted while
+ * others run are admitted as slots and HBM budget free up and join the next
+ * fused launch; uploads, launches and downloads of different jobs overlap. */
+int vvclf_server_create(vvclf_ctx* ctx, const vvclf_server_config* cfg, vvclf_server** out);
+int vvclf_server_destroy(vvclf_server* srv);
+/* A stream: a sequence of pictures that share resident LMCS / ALF tables
+ * (the APS state of src/stream.rs:90-92); tables are re-uploaded only when
+ * the content of a submitted frame's LMCS binding or APS binding changes. */
+int vvclf_server_open_stream(vvclf_server* srv, uint32_t* stream_id);
+int vvclf_server_close_stream(vvclf_server* srv, uint32_t stream_id);
+/* Queue one picture.  `frame` and everything it points to must stay valid
+ * until the ticket completes; the result is written into frame->pic.
+ * VVCLF_ERR_BUDGET if the job alone exceeds the budget. */
+int vvclf_server_submit(vvclf_server* srv, uint32_t stream_id, vvclf_frame* frame, uint64_t* ticket);
+/* VVCLF_OK when done (or the job's own error), VVCLF_ERR_AGAIN if pending. */
+int vvclf_server_poll(vvclf_server* srv, uint64_t ticket);
+/* Blocks until done or timeout: VVCLF_OK / the job's error / VVCLF_ERR_TIMEOUT. */
+int vvclf_server_wait(vvclf_server* srv, uint64_t ticket, uint32_t timeout_ms);
+int vvclf_server_drain(vvclf_server* srv, uint32_t timeout_ms);
+int vvclf_server_get_stats(vvclf_server* srv, vvclf_server_stats* out);
+/* Test hook: make the next `n` jobs fail at admission with VVCLF_ERR_HIP, to
+ * check that one failed job does not disturb the others. */
+int vvclf_server_inject_failures(vvclf_server* srv, uint32_t n);
 
 #ifdef __cplusplus
 }

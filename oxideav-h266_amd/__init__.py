@@ -166,6 +166,17 @@ class Tune(C.Structure):
     _fields_ = [("tile_w", u32), ("tile_h", u32), ("threads", u32), ("wgs_per_cu", u32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", u64), ("max_inflight", u32), ("max_batch", u32), ("prep_threads", u32),
+                ("pad", u32), ("tune", Tune)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("failed", u64), ("rejected", u64), ("luma_pixels", u64),
+                ("launches", u64), ("bytes_admitted_peak", u64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("max_ms", C.c_double), ("mean_ms", C.c_double), ("wall_s", C.c_double)]
+
+
 class SynthSpec(C.Structure):
     _fields_ = [("width", u32), ("height", u32), ("bit_depth", u32), ("ctb_log2", u32), ("seed", u64),
                 ("min_cu_w", u32), ("min_cu_h", u32), ("split_scale", f32), ("p_intra", f32), ("p_mv_change", f32),
@@ -180,7 +191,8 @@ STRUCTS = {"vvclf_plane": Plane, "vvclf_picture": Picture, "vvclf_deblock_cu": D
            "vvclf_sao_config": SaoConfig, "vvclf_bounds": Bounds, "vvclf_alf_ctb": AlfCtb,
            "vvclf_alf_picture": AlfPicture, "vvclf_alf_config": AlfConfig, "vvclf_alf_aps": AlfAps,
            "vvclf_alf_aps_binding": AlfApsBinding, "vvclf_lmcs_data": LmcsData, "vvclf_lmcs_derived": LmcsDerived,
-           "vvclf_lmcs_binding": LmcsBinding, "vvclf_frame": Frame, "vvclf_tune": Tune}
+           "vvclf_lmcs_binding": LmcsBinding, "vvclf_frame": Frame, "vvclf_tune": Tune,
+           "vvclf_server_config": ServerConfig, "vvclf_server_stats": ServerStats}
 
 P = C.POINTER
 VP = C.c_void_p
@@ -201,6 +213,8 @@ SIGNATURES = {
     "vvclf_abi_sizeof": (C.c_size_t, [C.c_char_p]),
     "vvclf_ctx_create": (C.c_int, [C.c_int, P(VP)]),
     "vvclf_ctx_destroy": (C.c_int, [VP]),
+    "vvclf_ctx_stream": (VP, [VP]),
+    "vvclf_ctx_synchronize": (C.c_int, [VP]),
     "vvclf_job_create": (C.c_int, [VP, P(Frame), P(VP)]),
     "vvclf_job_destroy": (C.c_int, [VP]),
     "vvclf_job_set_input": (C.c_int, [VP, P(Picture)]),
@@ -220,6 +234,16 @@ SIGNATURES = {
     "vvclf_batch_graph_launch": (C.c_int, [VP, VP]),
     "vvclf_batch_get_tune": (C.c_int, [VP, P(Tune)]),
     "vvclf_autotune": (C.c_int, [VP, P(VP), u32, P(Tune)]),
+    "vvclf_server_create": (C.c_int, [VP, P(ServerConfig), P(VP)]),
+    "vvclf_server_destroy": (C.c_int, [VP]),
+    "vvclf_server_open_stream": (C.c_int, [VP, P(u32)]),
+    "vvclf_server_close_stream": (C.c_int, [VP, u32]),
+    "vvclf_server_submit": (C.c_int, [VP, u32, P(Frame), P(u64)]),
+    "vvclf_server_poll": (C.c_int, [VP, u64]),
+    "vvclf_server_wait": (C.c_int, [VP, u64, u32]),
+    "vvclf_server_drain": (C.c_int, [VP, u32]),
+    "vvclf_server_get_stats": (C.c_int, [VP, P(ServerStats)]),
+    "vvclf_server_inject_failures": (C.c_int, [VP, u32]),
 }
 
 
@@ -346,6 +370,13 @@ class Context:
     def __init__(self, device=0):
         self.h = VP()
         check(lib.vvclf_ctx_create(device, C.byref(self.h)))
+
+    @property
+    def stream(self):
+        return C.c_void_p(lib.vvclf_ctx_stream(self.h))
+
+    def synchronize(self):
+        check(lib.vvclf_ctx_synchronize(self.h))
 
     def close(self):
         if self.h:

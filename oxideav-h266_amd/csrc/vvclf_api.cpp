@@ -56,7 +56,9 @@ vvclf_ctx* default_ctx() {
 namespace vvclf {
 
 int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvclf_job** out,
-                      uint64_t* dry_stats) {
+                      uint64_t* dry_stats, const job_opts* opts, host_tables* keep) {
+  const resident_tables* res = opts ? opts->res : nullptr;
+  hipStream_t ps = opts ? opts->stream : nullptr;
   clear_error();
   if ((!ctx && !dry_stats) || !f || (!out && !dry_stats)) return fail(VVCLF_ERR_INVALID, "job_create: NULL argument");
   if (out) *out = nullptr;
@@ -159,7 +161,8 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   size_t set_bytes = (luma_elems + 2 * chroma_elems) * 2;
   set_bytes = (set_bytes + 255) / 256 * 256;
   j->bytes_planes = set_bytes * 4;
-  e = hipMalloc(&j->dev_planes, j->bytes_planes);
+  j->pool_stream = ps;
+  e = ps ? hipMallocAsync(&j->dev_planes, j->bytes_planes, ps) : hipMalloc(&j->dev_planes, j->bytes_planes);
   if (e != hipSuccess) {
     delete j;
     return fail(VVCLF_ERR_NOMEM, "hipMalloc planes (%zu bytes): %s", (size_t)j->bytes_planes, hipGetErrorString(e));
@@ -174,25 +177,33 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   if (!t.do_sao) d.b = d.a;
   // side tables
   side_builder sb;
+  if (res) {  // resident per-stream tables replace the per-job copies
+    t.lmcs_lut.clear();
+    t.alf_bank.clear();
+    t.alf_cbank.clear();
+    t.alf_ccbank.clear();
+  }
   size_t o_lut = sb.add(t.lmcs_lut), o_lv = sb.add(t.e_lv), o_lh = sb.add(t.e_lh);
   size_t o_cv0 = sb.add(t.e_cv[0]), o_cv1 = sb.add(t.e_cv[1]), o_ch0 = sb.add(t.e_ch[0]), o_ch1 = sb.add(t.e_ch[1]);
   size_t o_sao = sb.add(t.sao), o_saor = sb.add(t.sao_rect);
   size_t o_actb = sb.add(t.alf_ctb), o_arect = sb.add(t.alf_rect), o_bank = sb.add(t.alf_bank);
   size_t o_cbank = sb.add(t.alf_cbank), o_ccbank = sb.add(t.alf_ccbank);
   j->bytes_side = sb.blob.size();
-  e = hipMalloc(&j->dev_side, j->bytes_side);
+  e = ps ? hipMallocAsync(&j->dev_side, j->bytes_side, ps) : hipMalloc(&j->dev_side, j->bytes_side);
   if (e != hipSuccess) {
-    hipFree(j->dev_planes);
-    delete j;
+    j->dev_side = nullptr;
+    vvclf_job_destroy(j);
     return fail(VVCLF_ERR_NOMEM, "hipMalloc side tables: %s", hipGetErrorString(e));
   }
-  e = hipMemcpy(j->dev_side, sb.blob.data(), j->bytes_side, hipMemcpyHostToDevice);
+  e = ps ? hipMemcpyAsync(j->dev_side, sb.blob.data(), j->bytes_side, hipMemcpyHostToDevice, ps)
+         : hipMemcpy(j->dev_side, sb.blob.data(), j->bytes_side, hipMemcpyHostToDevice);
+  if (e == hipSuccess && ps) e = hipStreamSynchronize(ps);
   if (e != hipSuccess) {
     vvclf_job_destroy(j);
     return hip_fail(e, "upload side tables");
   }
   uint8_t* sd = (uint8_t*)j->dev_side;
-  d.lmcs_lut = do_lmcs ? (const uint16_t*)(sd + o_lut) : nullptr;
+  d.lmcs_lut = do_lmcs ? (res ? res->lut : (const uint16_t*)(sd + o_lut)) : nullptr;
   d.gw = (int32_t)t.gw;
   d.gh = (int32_t)t.gh;
   d.e_lv = (const uint32_t*)(sd + o_lv);
@@ -209,9 +220,10 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   d.sao_rect = t.sao_rect.empty() ? nullptr : (const uint64_t*)(sd + o_saor);
   d.alf_ctb = (const uint32_t*)(sd + o_actb);
   d.alf_rect = (const uint64_t*)(sd + o_arect);
-  d.alf_bank = (const uint32_t*)(sd + o_bank);
-  d.alf_cbank = (const uint32_t*)(sd + o_cbank);
-  d.alf_ccbank = (const int32_t*)(sd + o_ccbank);
+  d.alf_bank = res ? res->bank : (const uint32_t*)(sd + o_bank);
+  d.alf_cbank = res ? res->cbank : (const uint32_t*)(sd + o_cbank);
+  d.alf_ccbank = res ? res->ccbank : (const int32_t*)(sd + o_ccbank);
+  if (keep) *keep = std::move(t);
   if ((rc = vvclf_job_set_input(j, &f->pic))) {
     vvclf_job_destroy(j);
     return rc;
@@ -221,7 +233,7 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
 }
 
 const uint16_t* job_final(const vvclf_job* j, int comp) {
-  const dev_planes& p = j->d.do_alf ? j->d.out : (j->d.do_sao ? j->d.b : j->d.a);
+  const dev_planes& p = (j->fused_out || j->d.do_alf) ? j->d.out : (j->d.do_sao ? j->d.b : j->d.a);
   return comp == 0 ? p.y : (comp == 1 ? p.cb : p.cr);
 }
 
@@ -239,6 +251,7 @@ size_t vvclf_abi_sizeof(const char* n) {
   S(vvclf_sao_ctb) S(vvclf_sao_ctb_params) S(vvclf_sao_picture) S(vvclf_sao_config) S(vvclf_bounds)
   S(vvclf_alf_ctb) S(vvclf_alf_picture) S(vvclf_alf_config) S(vvclf_alf_aps) S(vvclf_alf_aps_binding)
   S(vvclf_lmcs_data) S(vvclf_lmcs_derived) S(vvclf_lmcs_binding) S(vvclf_frame) S(vvclf_tune)
+  S(vvclf_server_config) S(vvclf_server_stats)
 #undef S
   return 0;
 }
@@ -270,6 +283,15 @@ int vvclf_ctx_destroy(vvclf_ctx* ctx) {
   return VVCLF_OK;
 }
 
+void* vvclf_ctx_stream(vvclf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int vvclf_ctx_synchronize(vvclf_ctx* ctx) {
+  if (!ctx) return fail(VVCLF_ERR_INVALID, "ctx_synchronize: NULL");
+  hipSetDevice(ctx->device);
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? VVCLF_OK : hip_fail(e, "hipDeviceSynchronize");
+}
+
 int vvclf_job_create(vvclf_ctx* ctx, const vvclf_frame* frame, vvclf_job** out) {
   return job_create_stages(ctx, frame, STAGE_ALL, out, nullptr);
 }
@@ -282,6 +304,12 @@ int vvclf_frame_check(const vvclf_frame* frame, uint64_t stats[8]) {
 int vvclf_job_destroy(vvclf_job* j) {
   if (!j) return VVCLF_OK;
   hipSetDevice(j->ctx->device);
+  if (j->pool_stream) {  // back to the stream-ordered pool, no device-wide sync
+    if (j->dev_planes) hipFreeAsync(j->dev_planes, j->pool_stream);
+    if (j->dev_side) hipFreeAsync(j->dev_side, j->pool_stream);
+    delete j;
+    return VVCLF_OK;
+  }
   if (j->dev_planes) hipFree(j->dev_planes);
   if (j->dev_side) hipFree(j->dev_side);
   delete j;
@@ -293,19 +321,23 @@ int vvclf_job_set_input(vvclf_job* j, const vvclf_picture* pic) {
   if (pic->luma.width != j->w || pic->luma.height != j->h || !pic->luma.samples)
     return fail(VVCLF_ERR_INVALID, "job_set_input: luma geometry differs");
   hipSetDevice(j->ctx->device);
-  hipError_t e = hipMemcpy2D(j->d.in.y, (size_t)j->d.py * 2, pic->luma.samples, (size_t)pic->luma.stride * 2,
-                             (size_t)j->w * 2, j->h, hipMemcpyHostToDevice);
+  hipStream_t ps = j->pool_stream;
+  hipError_t e = hipMemcpy2DAsync(j->d.in.y, (size_t)j->d.py * 2, pic->luma.samples, (size_t)pic->luma.stride * 2,
+                                  (size_t)j->w * 2, j->h, hipMemcpyHostToDevice, ps);
   if (e != hipSuccess) return hip_fail(e, "upload luma");
   if (j->wc) {
     if (!pic->cb.samples || !pic->cr.samples || pic->cb.width != j->wc || pic->cb.height != j->hc)
       return fail(VVCLF_ERR_INVALID, "job_set_input: chroma geometry differs");
-    e = hipMemcpy2D(j->d.in.cb, (size_t)j->d.pc * 2, pic->cb.samples, (size_t)pic->cb.stride * 2, (size_t)j->wc * 2,
-                    j->hc, hipMemcpyHostToDevice);
+    e = hipMemcpy2DAsync(j->d.in.cb, (size_t)j->d.pc * 2, pic->cb.samples, (size_t)pic->cb.stride * 2,
+                         (size_t)j->wc * 2, j->hc, hipMemcpyHostToDevice, ps);
     if (e == hipSuccess)
-      e = hipMemcpy2D(j->d.in.cr, (size_t)j->d.pc * 2, pic->cr.samples, (size_t)pic->cr.stride * 2,
-                      (size_t)j->wc * 2, j->hc, hipMemcpyHostToDevice);
+      e = hipMemcpy2DAsync(j->d.in.cr, (size_t)j->d.pc * 2, pic->cr.samples, (size_t)pic->cr.stride * 2,
+                           (size_t)j->wc * 2, j->hc, hipMemcpyHostToDevice, ps);
     if (e != hipSuccess) return hip_fail(e, "upload chroma");
   }
+  // the host planes may be reused as soon as this returns
+  e = hipStreamSynchronize(ps);
+  if (e != hipSuccess) return hip_fail(e, "upload synchronize");
   return VVCLF_OK;
 }
 
@@ -314,6 +346,7 @@ int vvclf_job_run(vvclf_job* j, void* stream) {
   hipSetDevice(j->ctx->device);
   hipError_t e = launch_plain(j->d, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "launch");
+  j->fused_out = false;
   return VVCLF_OK;
 }
 
@@ -387,6 +420,7 @@ int vvclf_job_run_prof(vvclf_job* j, void* stream, vvclf_prof* p) {
   hipSetDevice(j->ctx->device);
   hipError_t e = launch_plain(j->d, (hipStream_t)stream, &p->ev[(size_t)p->used * (PLAIN_STAGES + 1)]);
   if (e != hipSuccess) return hip_fail(e, "launch");
+  j->fused_out = false;
   p->used++;
   return VVCLF_OK;
 }

@@ -562,6 +562,44 @@ void build_luma_bank(const int32_t coeff[25][12], const uint8_t clip[25][12], ui
 }
 }  // namespace
 
+// The filter banks of an ALF binding (src/alf.rs:564-644): 16 fixed sets,
+// one per APS slot, the chroma alternatives and the CC filters.
+void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t) {
+  uint32_t n_banks = 16 + b.n_luma_apses;
+  t.alf_bank.assign((size_t)n_banks * ALF_BANK_WORDS, 0);
+  for (int s = 0; s < 16; s++) {
+    int32_t coeff[25][12];
+    uint8_t clip[25][12];
+    for (int c = 0; c < 25; c++)
+      for (int k = 0; k < 12; k++) {
+        coeff[c][k] = kFixCoeff[kClassToFilt[s][c]][k];
+        clip[c][k] = 0;  // fixed sets: clipIdx 0 (src/alf.rs:578-580)
+      }
+    build_luma_bank(coeff, clip, ac.bit_depth, &t.alf_bank[(size_t)s * ALF_BANK_WORDS]);
+  }
+  for (uint32_t i = 0; i < b.n_luma_apses; i++) {
+    const vvclf_alf_aps* a = b.luma_apses[i];
+    if (!a || !a->alf_luma_filter_signal_flag) continue;
+    build_luma_bank(a->luma_coeff, a->luma_clip_idx, ac.bit_depth, &t.alf_bank[(size_t)(16 + i) * ALF_BANK_WORDS]);
+  }
+  t.alf_cbank.assign(ALF_CBANK_WORDS, 0);
+  const vvclf_alf_aps* ca = b.chroma_aps;
+  bool chroma_ok = ac.chroma_format_idc != 0 && ca && ca->alf_chroma_filter_signal_flag;
+  uint32_t n_alt = chroma_ok ? std::min(ca->n_chroma_coeff, (uint32_t)VVCLF_ALF_MAX_CHROMA_ALT) : 0;
+  for (uint32_t a = 0; a < n_alt; a++)
+    for (int k = 0; k < 6; k++)
+      t.alf_cbank[a * 6 + k] = tap_word(ca->chroma_coeff[a][k], clip_word(ac.bit_depth, ca->chroma_clip_idx[a][k]));
+  t.alf_ccbank.assign(2 * ALF_CCBANK_WORDS, 0);
+  const vvclf_alf_aps* ccs[2] = {b.cc_cb_aps, b.cc_cr_aps};
+  for (int c = 0; c < 2; c++) {
+    if (!ccs[c] || ac.chroma_format_idc == 0) continue;
+    uint32_t n = std::min(c == 0 ? ccs[c]->n_cc_cb_coeff : ccs[c]->n_cc_cr_coeff, (uint32_t)VVCLF_ALF_MAX_CC_FILTERS);
+    for (uint32_t k = 0; k < n; k++)
+      for (int j = 0; j < 7; j++)
+        t.alf_ccbank[c * ALF_CCBANK_WORDS + k * 7 + j] = c == 0 ? ccs[c]->cc_cb_coeff[k][j] : ccs[c]->cc_cr_coeff[k][j];
+  }
+}
+
 int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t, uint32_t stages) {
   // ---- SAO: src/sao.rs:389-497
   t.do_sao = false;
@@ -638,45 +676,18 @@ int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t,
       t.hctb = ap.pic_height_in_ctbs_y;
       t.alf_ctb_log2 = ac.ctb_log2_size_y;
       t.alf_bd = ac.bit_depth;
-      uint32_t n_banks = 16 + b.n_luma_apses;
-      if (n_banks > 255) return fail(VVCLF_ERR_INVALID, "alf: too many luma APS slots");
-      t.alf_bank.assign((size_t)n_banks * ALF_BANK_WORDS, 0);
-      for (int s = 0; s < 16; s++) {
-        int32_t coeff[25][12];
-        uint8_t clip[25][12];
-        for (int c = 0; c < 25; c++)
-          for (int k = 0; k < 12; k++) {
-            coeff[c][k] = kFixCoeff[kClassToFilt[s][c]][k];
-            clip[c][k] = 0;  // fixed sets: clipIdx 0 (src/alf.rs:578-580)
-          }
-        build_luma_bank(coeff, clip, ac.bit_depth, &t.alf_bank[(size_t)s * ALF_BANK_WORDS]);
-      }
+      if (16 + b.n_luma_apses > 255) return fail(VVCLF_ERR_INVALID, "alf: too many luma APS slots");
+      build_alf_banks(ac, b, t);
       std::vector<bool> slot_ok(b.n_luma_apses, false);
-      for (uint32_t i = 0; i < b.n_luma_apses; i++) {
-        const vvclf_alf_aps* a = b.luma_apses[i];
-        if (!a || !a->alf_luma_filter_signal_flag) continue;
-        slot_ok[i] = true;
-        build_luma_bank(a->luma_coeff, a->luma_clip_idx, ac.bit_depth,
-                        &t.alf_bank[(size_t)(16 + i) * ALF_BANK_WORDS]);
-      }
-      t.alf_cbank.assign(ALF_CBANK_WORDS, 0);
+      for (uint32_t i = 0; i < b.n_luma_apses; i++) slot_ok[i] = b.luma_apses[i] && b.luma_apses[i]->alf_luma_filter_signal_flag;
       const vvclf_alf_aps* ca = b.chroma_aps;
       bool chroma_ok = ac.chroma_format_idc != 0 && ca && ca->alf_chroma_filter_signal_flag;
       uint32_t n_alt = chroma_ok ? std::min(ca->n_chroma_coeff, (uint32_t)VVCLF_ALF_MAX_CHROMA_ALT) : 0;
-      for (uint32_t a = 0; a < n_alt; a++)
-        for (int k = 0; k < 6; k++)
-          t.alf_cbank[a * 6 + k] = tap_word(ca->chroma_coeff[a][k], clip_word(ac.bit_depth, ca->chroma_clip_idx[a][k]));
-      t.alf_ccbank.assign(2 * ALF_CCBANK_WORDS, 0);
       const vvclf_alf_aps* ccs[2] = {b.cc_cb_aps, b.cc_cr_aps};
       uint32_t n_cc[2] = {0, 0};
-      for (int c = 0; c < 2; c++) {
-        if (!ccs[c] || ac.chroma_format_idc == 0) continue;
-        n_cc[c] = std::min(c == 0 ? ccs[c]->n_cc_cb_coeff : ccs[c]->n_cc_cr_coeff, (uint32_t)VVCLF_ALF_MAX_CC_FILTERS);
-        for (uint32_t k = 0; k < n_cc[c]; k++)
-          for (int j = 0; j < 7; j++)
-            t.alf_ccbank[c * ALF_CCBANK_WORDS + k * 7 + j] =
-                c == 0 ? ccs[c]->cc_cb_coeff[k][j] : ccs[c]->cc_cr_coeff[k][j];
-      }
+      for (int c = 0; c < 2; c++)
+        if (ccs[c] && ac.chroma_format_idc != 0)
+          n_cc[c] = std::min(c == 0 ? ccs[c]->n_cc_cb_coeff : ccs[c]->n_cc_cr_coeff, (uint32_t)VVCLF_ALF_MAX_CC_FILTERS);
       t.alf_ctb.assign(n, 0xffffffffu);
       t.alf_rect.resize(n);
       uint32_t ctb = 1u << ac.ctb_log2_size_y;

@@ -43,8 +43,29 @@ void clear_error();
 int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc, uint32_t Hc, host_tables& t);
 int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t, uint32_t stages);
 void build_lmcs_lut(const vvclf_lmcs_binding& b, std::vector<uint16_t>& lut);
+void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t);
 
 static constexpr int PLAIN_STAGES = 8;
+// Device tables shared by many jobs (a serving stream's resident LMCS / ALF
+// state); a job built with them uploads only its own per-picture tables.
+struct resident_tables {
+  const uint16_t* lut = nullptr;     // 65536 entries
+  const uint32_t* bank = nullptr;    // luma banks
+  const uint32_t* cbank = nullptr;   // chroma bank
+  const int32_t* ccbank = nullptr;   // CC banks
+};
+
+struct job_opts {
+  hipStream_t stream = nullptr;  // stream-ordered allocation and copies (memory pool)
+  const resident_tables* res = nullptr;
+};
+
+int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvclf_job** out, uint64_t* dry_stats,
+                      const job_opts* opts = nullptr, host_tables* keep = nullptr);
+hipError_t launch_fused(int tw, int th, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles, uint32_t grid,
+                        hipStream_t s);
+int fused_max_blocks_per_cu(int tw, int th);
+
 hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev = nullptr);
 
 }  // namespace vvclf
@@ -70,4 +91,6 @@ struct vvclf_job {
   uint64_t bytes_planes, bytes_side;
   uint32_t w, h, wc, hc;
   uint32_t stages;
+  bool fused_out = false;  // last run was the fused form (result in `out`)
+  hipStream_t pool_stream = nullptr;  // set when allocated from the stream-ordered pool
 };

@@ -69,6 +69,12 @@ static constexpr int ALF_CBANK_WORDS = 8 * 6;
 // CC bank: [4 filters][7 taps] as i32 per component.
 static constexpr int ALF_CCBANK_WORDS = 4 * 7;
 
+// One output tile of the fused batched form: job index and tile origin.
+struct tile_ref {
+  uint32_t job;
+  uint16_t x0, y0;
+};
+
 struct dev_planes {
   uint16_t* y;
   uint16_t* cb;
