@@ -391,6 +391,11 @@ int vvclf_batch_run(vvclf_batch* batch, void* stream);
 /* Capture the batch launch in a hipGraph once, then replay it. */
 int vvclf_batch_graph_launch(vvclf_batch* batch, void* stream);
 int vvclf_batch_get_tune(const vvclf_batch* batch, vvclf_tune* out);
+/* Times `reps` launches (graph replays when use_graph) on `stream` with HIP
+ * events bracketing them, after `warmup` untimed launches; returns the mean
+ * milliseconds per launch. */
+int vvclf_batch_time(vvclf_batch* batch, void* stream, uint32_t warmup, uint32_t reps, int use_graph,
+                     double* ms_per_launch);
 /* Pick a tile shape for the given jobs by timing candidates on the device. */
 int vvclf_autotune(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, vvclf_tune* out);
 

@@ -182,7 +182,7 @@ class SynthSpec(C.Structure):
                 ("min_cu_w", u32), ("min_cu_h", u32), ("split_scale", f32), ("p_intra", f32), ("p_mv_change", f32),
                 ("p_coded", f32), ("qp_min", u32), ("qp_max", u32), ("rich", u32), ("lmcs", u32), ("sao", u32),
                 ("alf", u32), ("p_alf_luma", f32), ("n_luma_aps", u32), ("p_fixed_set", f32), ("chroma_alf", u32),
-                ("cc_alf", u32), ("p_cc", f32), ("tiles", u32)]
+                ("cc_alf", u32), ("p_cc", f32), ("tiles", u32), ("aps_seed", u64)]
 
 
 STRUCTS = {"vvclf_plane": Plane, "vvclf_picture": Picture, "vvclf_deblock_cu": DeblockCu,
@@ -234,6 +234,7 @@ SIGNATURES = {
     "vvclf_batch_graph_launch": (C.c_int, [VP, VP]),
     "vvclf_batch_get_tune": (C.c_int, [VP, P(Tune)]),
     "vvclf_autotune": (C.c_int, [VP, P(VP), u32, P(Tune)]),
+    "vvclf_batch_time": (C.c_int, [VP, VP, u32, u32, C.c_int, P(C.c_double)]),
     "vvclf_server_create": (C.c_int, [VP, P(ServerConfig), P(VP)]),
     "vvclf_server_destroy": (C.c_int, [VP]),
     "vvclf_server_open_stream": (C.c_int, [VP, P(u32)]),

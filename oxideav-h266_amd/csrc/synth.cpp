@@ -356,8 +356,9 @@ int vvclf_synth_make(const vvclf_synth_spec* spec, vvclf_synth_frame** out) {
     f->mv = mv;
     f->grid.cells = f->mv.data();
   }
-  // LMCS
-  if (s.lmcs) lmcs_random(f->lmcs, bd, r);
+  // LMCS and APS tables: from their own seed when the stream shares them
+  rng ra{s.aps_seed ? s.aps_seed * 0x9E3779B97F4A7C15ull + 17 : r.next()};
+  if (s.lmcs) lmcs_random(f->lmcs, bd, ra);
   // SAO
   f->sao.resize((size_t)wctb * hctb);
   for (auto& p : f->sao) {
@@ -378,8 +379,8 @@ int vvclf_synth_make(const vvclf_synth_spec* spec, vvclf_synth_frame** out) {
     a.n_luma_coeff = 25;
     for (int c = 0; c < 25; c++)
       for (int k = 0; k < 12; k++) {
-        a.luma_coeff[c][k] = r.range(-128, 127);
-        a.luma_clip_idx[c][k] = (uint8_t)r.range(0, 3);
+        a.luma_coeff[c][k] = ra.range(-128, 127);
+        a.luma_clip_idx[c][k] = (uint8_t)ra.range(0, 3);
       }
   }
   for (auto& a : f->luma_aps) f->luma_aps_ptr.push_back(&a);
@@ -392,8 +393,8 @@ int vvclf_synth_make(const vvclf_synth_spec* spec, vvclf_synth_frame** out) {
     a.alf_chroma_num_alt_filters_minus1 = 3;
     for (int i = 0; i < 4; i++)
       for (int k = 0; k < 6; k++) {
-        a.chroma_coeff[i][k] = r.range(-128, 127);
-        a.chroma_clip_idx[i][k] = (uint8_t)r.range(0, 3);
+        a.chroma_coeff[i][k] = ra.range(-128, 127);
+        a.chroma_clip_idx[i][k] = (uint8_t)ra.range(0, 3);
       }
   }
   memset(&f->cc_aps, 0, sizeof(f->cc_aps));
@@ -404,12 +405,12 @@ int vvclf_synth_make(const vvclf_synth_spec* spec, vvclf_synth_frame** out) {
     for (int i = 0; i < 4; i++)
       for (int k = 0; k < 7; k++) {
         // power-of-two mapped coefficients (src/aps.rs:314-326): 0, +-1 .. +-64
-        int m = r.range(0, 7);
+        int m = ra.range(0, 7);
         int v = m == 0 ? 0 : (1 << (m - 1));
-        a.cc_cb_coeff[i][k] = r.chance(0.5) ? -v : v;
-        m = r.range(0, 7);
+        a.cc_cb_coeff[i][k] = ra.chance(0.5) ? -v : v;
+        m = ra.range(0, 7);
         v = m == 0 ? 0 : (1 << (m - 1));
-        a.cc_cr_coeff[i][k] = r.chance(0.5) ? -v : v;
+        a.cc_cr_coeff[i][k] = ra.chance(0.5) ? -v : v;
       }
   }
   f->alf.resize((size_t)wctb * hctb);

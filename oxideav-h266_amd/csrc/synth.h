@@ -33,6 +33,8 @@ typedef struct vvclf_synth_spec {
   uint32_t cc_alf;     /* CC-ALF for Cb and Cr */
   float p_cc;          /* per-CTB CC-ALF enable */
   uint32_t tiles;      /* 2x2 tile grid with loop filtering across tiles off */
+  uint64_t aps_seed;   /* nonzero: APS / LMCS tables drawn from this seed, so
+                          pictures of one stream share them */
 } vvclf_synth_spec;
 
 typedef struct vvclf_synth_frame vvclf_synth_frame;

@@ -173,6 +173,29 @@ int vvclf_batch_get_tune(const vvclf_batch* b, vvclf_tune* out) {
   return VVCLF_OK;
 }
 
+int vvclf_batch_time(vvclf_batch* b, void* stream, uint32_t warmup, uint32_t reps, int use_graph, double* ms) {
+  if (!b || !ms || reps == 0) return fail(VVCLF_ERR_INVALID, "batch_time: bad argument");
+  hipSetDevice(b->ctx->device);
+  auto once = [&]() { return use_graph ? vvclf_batch_graph_launch(b, stream) : vvclf_batch_run(b, stream); };
+  for (uint32_t i = 0; i < warmup; i++)
+    if (int rc = once()) return rc;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail(VVCLF_ERR_HIP, "events");
+  hipError_t e = hipEventRecord(e0, (hipStream_t)stream);
+  int rc = VVCLF_OK;
+  for (uint32_t i = 0; i < reps && !rc && e == hipSuccess; i++) rc = once();
+  if (e == hipSuccess) e = hipEventRecord(e1, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "batch_time events");
+  *ms = (double)t / reps;
+  return VVCLF_OK;
+}
+
 // Times every supported tile shape and workgroups-per-CU count on the given
 // jobs (one warm launch, then the mean of three timed launches) and returns
 // the fastest.
