@@ -185,8 +185,13 @@ __device__ __forceinline__ void luma_filter_line(const int* p, const int* q, con
       rm = (pp[6] + pp[5] + pp[4] + pp[3] + pp[2] + pp[1] + 2 * (qq[2] + qq[1] + qq[0] + pp[0]) + qq[0] + qq[1] + 8) >>
            4;
     }
-    int refp = (p[mp] + p[mp - 1] + 1) >> 1;
-    int refq = (q[mq] + q[mq - 1] + 1) >> 1;
+    // refP / refQ (eqs. 1395/1396) without a runtime array index
+    int refp = (p[1] + p[0] + 1) >> 1, refq = (q[1] + q[0] + 1) >> 1;
+#pragma unroll
+    for (int k = 2; k < 8; k++) {
+      if (mp == k) refp = (p[k] + p[k - 1] + 1) >> 1;
+      if (mq == k) refq = (q[k] + q[k - 1] + 1) >> 1;
+    }
     const int F7[7] = {59, 50, 41, 32, 23, 14, 5}, F5[5] = {58, 45, 32, 19, 6}, F3[3] = {53, 32, 11};
     const int T7[7] = {6, 5, 4, 3, 2, 1, 1}, T5[5] = {6, 5, 4, 3, 2}, T3[3] = {6, 4, 2};
 #pragma unroll
@@ -252,20 +257,19 @@ __device__ __forceinline__ bool chroma_strong_row(const int* c, int beta, int tc
 }
 
 // Strong / weak chroma line filter (src/deblock.rs:1803-1926, :1986-2074).
-// n[] receives new values at the same indices as c[]; lo/hi give the index
-// range written.
-__device__ __forceinline__ void chroma_filter_line(const int* c, bool strong, bool short_p, int tc, int* n, int& lo,
-                                                   int& hi) {
+// n[] receives new values at the same indices as c[]; the returned mask has
+// bit i set when sample i is written.
+__device__ __forceinline__ uint32_t chroma_filter_line(const int* c, bool strong, bool short_p, int tc, int* n) {
   int p3 = c[0], p2 = c[1], p1 = c[2], p0 = c[3], q0 = c[4], q1 = c[5], q2 = c[6], q3 = c[7];
+#pragma unroll
+  for (int i = 0; i < 8; i++) n[i] = c[i];
   if (strong) {
     if (short_p) {
       n[3] = clampi((3 * p1 + 2 * p0 + q0 + q1 + q2 + 4) >> 3, p0 - tc, p0 + tc);
       n[4] = clampi((2 * p1 + p0 + 2 * q0 + q1 + q2 + q3 + 4) >> 3, q0 - tc, q0 + tc);
       n[5] = clampi((p1 + p0 + q0 + 2 * q1 + q2 + 2 * q3 + 4) >> 3, q1 - tc, q1 + tc);
       n[6] = clampi((p0 + q0 + q1 + 2 * q2 + 3 * q3 + 4) >> 3, q2 - tc, q2 + tc);
-      lo = 3;
-      hi = 6;
-      return;
+      return 0x78u;
     }
     n[3] = clampi((p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2 + 4) >> 3, p0 - tc, p0 + tc);
     n[2] = clampi((2 * p3 + p2 + 2 * p1 + p0 + q0 + q1 + 4) >> 3, p1 - tc, p1 + tc);
@@ -273,15 +277,12 @@ __device__ __forceinline__ void chroma_filter_line(const int* c, bool strong, bo
     n[4] = clampi((p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3 + 4) >> 3, q0 - tc, q0 + tc);
     n[5] = clampi((p1 + p0 + q0 + 2 * q1 + q2 + 2 * q3 + 4) >> 3, q1 - tc, q1 + tc);
     n[6] = clampi((p0 + q0 + q1 + 2 * q2 + 3 * q3 + 4) >> 3, q2 - tc, q2 + tc);
-    lo = 1;
-    hi = 6;
-    return;
+    return 0x7eu;
   }
   int delta = clampi((((q0 - p0) * 4) + p1 - q1 + 4) >> 3, -tc, tc);
   n[3] = p0 + delta;
   n[4] = q0 - delta;
-  lo = 3;
-  hi = 4;
+  return 0x18u;
 }
 
 // ---------------------------------------------------------------------------
@@ -372,6 +373,98 @@ __device__ __forceinline__ int alf_class_from_sums(int sh, int sv, int sd0, int 
   int tr = TRANSPOSE[dir1 * 2 + (dir2 >> 1)];
   if (dir_s != 0) filt += (((dir1 & 1) << 1) + dir_s) * 5;
   return filt | (tr << 5);
+}
+
+}  // namespace vvclf
+
+namespace vvclf {
+
+// ---------------------------------------------------------------------------
+// ALF luma for one 4x4 block from a 10x10 register window
+// W[r][c] = sample(x4 - 3 + c, y4 - 3 + r), already clamped to the fetch
+// rectangle.  VB selects the block's position against the ALF virtual
+// boundary (4 rows above the CTB bottom):
+//   0: away from it; 1: block at CTB row CtbSizeY - 8; 2: at CtbSizeY - 4.
+// Classification (src/alf.rs:822-1010): the VB padding of row reads becomes a
+// clamp of the window row index (VB 1: rows > y4+3 read y4+3; VB 2: rows <
+// y4 read y4); CARVE is carve_out_active, which shortens the sum window.
+// Filtering (src/alf.rs:652-724): the Table 45 vertical offsets are
+// (min(1,m), min(2,m), min(3,m)) with m per block row; shift 10 where m = 0.
+// All row / column indices are compile-time constants after unrolling.
+// ---------------------------------------------------------------------------
+template <int VB, bool CARVE>
+__device__ __forceinline__ int alf_block_class(const int (&W)[10][10], int bd) {
+  constexpr int jmin = (VB == 2 && CARVE) ? 0 : -2;
+  constexpr int jmax = (VB == 1 && CARVE) ? 3 : 5;
+  constexpr int ac = ((VB == 1 || VB == 2) && CARVE) ? 3 : 2;
+  int sh = 0, sv = 0, sd0 = 0, sd1 = 0;
+#pragma unroll
+  for (int jj = jmin; jj <= jmax; jj++) {
+#pragma unroll
+    for (int ii = -2; ii <= 5; ii++) {
+      if (((ii + 4) & 1) != ((jj + 4) & 1)) continue;
+      // window row indices of rows y4+jj, y4+jj-1, y4+jj+1 after VB padding
+      int rc = jj + 3, ru = jj + 2, rd = jj + 4;
+      if (VB == 1) { rc = rc > 6 ? 6 : rc; ru = ru > 6 ? 6 : ru; rd = rd > 6 ? 6 : rd; }
+      if (VB == 2) { rc = rc < 3 ? 3 : rc; ru = ru < 3 ? 3 : ru; rd = rd < 3 ? 3 : rd; }
+      const int cc = ii + 3;
+      const int c2 = W[rc][cc] * 2;
+      sh += abs(c2 - W[rc][cc - 1] - W[rc][cc + 1]);
+      sv += abs(c2 - W[ru][cc] - W[rd][cc]);
+      sd0 += abs(c2 - W[ru][cc - 1] - W[rd][cc + 1]);
+      sd1 += abs(c2 - W[ru][cc + 1] - W[rd][cc - 1]);
+    }
+  }
+  return alf_class_from_sums(sh, sv, sd0, sd1, ac, bd);
+}
+
+template <int VB>
+__device__ __forceinline__ void alf_block_filter(const int (&W)[10][10], const uint32_t (&fw)[12], int maxv,
+                                                 int (&out)[4][4]) {
+  constexpr int MT[3][4] = {{3, 3, 3, 3}, {3, 2, 1, 0}, {0, 1, 2, 3}};
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int m = MT[VB][r];
+    const int y1 = m < 1 ? m : 1, y2 = m < 2 ? m : 2, y3 = m < 3 ? m : 3;
+    const int sft = m == 0 ? 10 : 7;
+    const int R = r + 3;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int C = c + 3;
+      const int cur = W[R][C];
+      int sum = alf_pair(fw[0], W[R + y3][C], W[R - y3][C], cur);
+      sum += alf_pair(fw[1], W[R + y2][C + 1], W[R - y2][C - 1], cur);
+      sum += alf_pair(fw[2], W[R + y2][C], W[R - y2][C], cur);
+      sum += alf_pair(fw[3], W[R + y2][C - 1], W[R - y2][C + 1], cur);
+      sum += alf_pair(fw[4], W[R + y1][C + 2], W[R - y1][C - 2], cur);
+      sum += alf_pair(fw[5], W[R + y1][C + 1], W[R - y1][C - 1], cur);
+      sum += alf_pair(fw[6], W[R + y1][C], W[R - y1][C], cur);
+      sum += alf_pair(fw[7], W[R + y1][C - 1], W[R - y1][C + 1], cur);
+      sum += alf_pair(fw[8], W[R + y1][C - 2], W[R - y1][C + 2], cur);
+      sum += alf_pair(fw[9], W[R][C + 3], W[R][C - 3], cur);
+      sum += alf_pair(fw[10], W[R][C + 2], W[R][C - 2], cur);
+      sum += alf_pair(fw[11], W[R][C + 1], W[R][C - 1], cur);
+      out[r][c] = clampi(cur + ((sum + (1 << (sft - 1))) >> sft), 0, maxv);
+    }
+  }
+}
+
+// Classification + filter of one block: picks the VB / carve variant.
+// `bank` holds the CTB's 25 x 4 x 12 tap words; yb = y4 - yCtb.
+__device__ __forceinline__ void alf_block(const int (&W)[10][10], const uint32_t* bank, int yb, int ctb, bool carve,
+                                          int bd, int maxv, int (&out)[4][4]) {
+  int cls;
+  int vb = yb == ctb - 8 ? 1 : (yb == ctb - 4 ? 2 : 0);
+  if (vb == 0) cls = alf_block_class<0, false>(W, bd);
+  else if (vb == 1) cls = carve ? alf_block_class<1, true>(W, bd) : alf_block_class<1, false>(W, bd);
+  else cls = carve ? alf_block_class<2, true>(W, bd) : alf_block_class<2, false>(W, bd);
+  const uint32_t* f = bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
+  uint32_t fw[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) fw[k] = f[k];
+  if (vb == 0) alf_block_filter<0>(W, fw, maxv, out);
+  else if (vb == 1) alf_block_filter<1>(W, fw, maxv, out);
+  else alf_block_filter<2>(W, fw, maxv, out);
 }
 
 }  // namespace vvclf

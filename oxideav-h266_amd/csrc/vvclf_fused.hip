@@ -120,7 +120,8 @@ __device__ void lds_deblock_luma(const dev_job& j, lwin& L, int gx0, int gx1, in
 // VERT: edge columns sx in [sx0, sx1] (x = 8 sx), segment rows sy in [sy0, sy1] (y = 2 sy).
 // HOR : edge rows sy in [sy0, sy1] (y = 8 sy), segment columns sx in [sx0, sx1] (x = 2 sx).
 template <bool VERT>
-__device__ void lds_deblock_chroma(const dev_job& j, lwin* C2, int sx0, int sx1, int sy0, int sy1, int maxv) {
+__device__ void lds_deblock_chroma(const dev_job& j, lwin cb, uint16_t* cr_p, int sx0, int sx1, int sy0, int sy1,
+                                   int maxv) {
   const int tid = threadIdx.x;
   const int ncol = sx1 - sx0 + 1, nrow = sy1 - sy0 + 1;
   const int nseg = ncol * nrow;
@@ -137,7 +138,8 @@ __device__ void lds_deblock_chroma(const dev_job& j, lwin* C2, int sx0, int sx1,
       e = (VERT ? j.e_cv[comp] : j.e_ch[comp])[(size_t)sy * mw + sx];
     }
     if (e != 0) {  // uniform across the pair
-      lwin& L = C2[comp];
+      lwin L = cb;  // Cb and Cr windows share their geometry
+      if (comp) L.p = cr_p;
       int x0, y0;
       if (VERT) { x0 = 8 * sx; y0 = 2 * sy + line; } else { x0 = 2 * sx + line; y0 = 8 * sy; }
       int c[8];
@@ -151,11 +153,13 @@ __device__ void lds_deblock_chroma(const dev_job& j, lwin* C2, int sx0, int sx1,
       else { eligible = mflq == 3 && (mflp == 3 || mflp == 1); short_p = mflp == 1; }
       bool mine = eligible && chroma_strong_row(c, beta, tc, short_p);
       bool other = __shfl_xor((int)mine, 1) != 0;
-      int n[8], lo, hi;
-      chroma_filter_line(c, eligible && mine && other, short_p, tc, n, lo, hi);
-      bool wp = !(e & E_PLTP_BIT), wq = !(e & E_PLTQ_BIT);
-      for (int i = lo; i <= hi; i++) {
-        if ((i < 4 && !wp) || (i >= 4 && !wq)) continue;
+      int n[8];
+      uint32_t wm = chroma_filter_line(c, eligible && mine && other, short_p, tc, n);
+      if (e & E_PLTP_BIT) wm &= 0xf0u;
+      if (e & E_PLTQ_BIT) wm &= 0x0fu;
+#pragma unroll
+      for (int i = 1; i < 7; i++) {
+        if (!((wm >> i) & 1)) continue;
         int x = VERT ? x0 - 4 + i : x0, y = VERT ? y0 : y0 - 4 + i;
         if (L.inside(x, y)) L.wr(x, y, clampi(n[i], 0, maxv));
       }
@@ -251,7 +255,7 @@ __device__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y
     if (chroma && j.do_dbk_chroma) {
       int sx0 = max(0, cdiv_i(cx0 - 5, 8)), sx1 = min(j.cvw - 1, fdiv(cx1 + 5, 8));  // x in [cx0-5, cx1+5]
       int sy0 = Cw[0].oy / 2, sy1 = min(j.cvh - 1, (Cw[0].oy + Cw[0].h - 1) / 2);
-      lds_deblock_chroma<true>(j, Cw, sx0, sx1, sy0, sy1, maxv);
+      lds_deblock_chroma<true>(j, Cw[0], Cw[1].p, sx0, sx1, sy0, sy1, maxv);
     }
     __syncthreads();
     {
@@ -262,7 +266,7 @@ __device__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y
     if (chroma && j.do_dbk_chroma) {
       int sy0 = max(0, cdiv_i(cy0 - 5, 8)), sy1 = min(j.chh - 1, fdiv(cy1 + 5, 8));  // y in [cy0-5, cy1+5]
       int sx0 = max(0, fdiv(cx0 - 3, 2)), sx1 = min(j.chw - 1, fdiv(min(j.wc, cx1 + 3) - 1, 2));
-      lds_deblock_chroma<false>(j, Cw, sx0, sx1, sy0, sy1, maxv);
+      lds_deblock_chroma<false>(j, Cw[0], Cw[1].p, sx0, sx1, sy0, sy1, maxv);
     }
     __syncthreads();
   }
@@ -310,62 +314,24 @@ __device__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y
           }
         continue;
       }
-      // sample(x, y) of the reference: clamp to the fetch rectangle, then read
-      // the SAO'd window (the clamped position always lies inside it).
-      auto A = [&](int x, int y) -> int { return S.rd(clampi(x, rx0, rx1), clampi(y, ry0, ry1)); };
-      const int y_ctb = ry * ctb, yb = y4 - y_ctb;
-      const bool carve = !(y_ctb + ctb >= H) || (H - y_ctb) > ctb - 4;
-      int min_y = -2, max_y = 5, ac = 2;
-      if (yb == ctb - 8 && carve) { max_y = 3; ac = 3; }
-      else if (yb == ctb - 4 && carve) { min_y = 0; ac = 3; }
-      const int vb = y_ctb + ctb - 4;
-      const bool has_top = yb >= ctb - 4, has_bot = (ctb - 4 - yb) > 0 && (ctb - 4 - yb) < 5;
-      auto pad = [&](int yy) -> int {
-        if (has_top && yy < vb) yy = vb;
-        if (has_bot && yy > vb - 1) yy = vb - 1;
-        return yy;
-      };
-      int sh = 0, sv = 0, sd0 = 0, sd1 = 0;
-      for (int jj = min_y; jj <= max_y; jj++) {
-        const int yc = pad(y4 + jj), yu = pad(y4 + jj - 1), yd = pad(y4 + jj + 1);
-        for (int ii = ((jj & 1) ? -1 : -2); ii <= 5; ii += 2) {
-          const int xc = x4 + ii;
-          const int c2 = A(xc, yc) * 2;
-          sh += abs(c2 - A(xc - 1, yc) - A(xc + 1, yc));
-          sv += abs(c2 - A(xc, yu) - A(xc, yd));
-          sd0 += abs(c2 - A(xc - 1, yu) - A(xc + 1, yd));
-          sd1 += abs(c2 - A(xc + 1, yu) - A(xc - 1, yd));
-        }
-      }
-      const int cls = alf_class_from_sums(sh, sv, sd0, sd1, ac, j.alf_bd);
-      const uint32_t* f = s.bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
-      uint32_t fw[12];
+      // W[r][c] = sample(x4-3+c, y4-3+r): clamp to the fetch rectangle, then
+      // read the SAO'd window (the clamped position always lies inside it).
+      int W[10][10];
 #pragma unroll
-      for (int k = 0; k < 12; k++) fw[k] = f[k];
-      for (int rr = 0; rr < 4; rr++) {
-        const int y = y4 + rr;
-        if (y >= y1) break;
-        int sft, d1, d2, d3;
-        alf_t45(y - y_ctb, ctb, sft, d1, d2, d3);
-        for (int c = 0; c < 4; c++) {
-          const int x = x4 + c;
-          if (x >= x1) break;
-          const int cur = A(x, y);
-          int sum = alf_pair(fw[0], A(x, y + d3), A(x, y - d3), cur);
-          sum += alf_pair(fw[1], A(x + 1, y + d2), A(x - 1, y - d2), cur);
-          sum += alf_pair(fw[2], A(x, y + d2), A(x, y - d2), cur);
-          sum += alf_pair(fw[3], A(x - 1, y + d2), A(x + 1, y - d2), cur);
-          sum += alf_pair(fw[4], A(x + 2, y + d1), A(x - 2, y - d1), cur);
-          sum += alf_pair(fw[5], A(x + 1, y + d1), A(x - 1, y - d1), cur);
-          sum += alf_pair(fw[6], A(x, y + d1), A(x, y - d1), cur);
-          sum += alf_pair(fw[7], A(x - 1, y + d1), A(x + 1, y - d1), cur);
-          sum += alf_pair(fw[8], A(x - 2, y + d1), A(x + 2, y - d1), cur);
-          sum += alf_pair(fw[9], A(x + 3, y), A(x - 3, y), cur);
-          sum += alf_pair(fw[10], A(x + 2, y), A(x - 2, y), cur);
-          sum += alf_pair(fw[11], A(x + 1, y), A(x - 1, y), cur);
-          dst[(size_t)y * j.py + x] = (uint16_t)clampi(cur + ((sum + (1 << (sft - 1))) >> sft), 0, amax);
-        }
+      for (int rr = 0; rr < 10; rr++) {
+        const uint16_t* row = S.p + (clampi(y4 - 3 + rr, ry0, ry1) - S.oy) * S.pitch - S.ox;
+#pragma unroll
+        for (int cc = 0; cc < 10; cc++) W[rr][cc] = row[clampi(x4 - 3 + cc, rx0, rx1)];
       }
+      const int y_ctb = ry * ctb;
+      const bool carve = !(y_ctb + ctb >= H) || (H - y_ctb) > ctb - 4;
+      int outv[4][4];
+      alf_block(W, s.bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outv);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (y4 + rr < y1 && x4 + c < x1) dst[(size_t)(y4 + rr) * j.py + x4 + c] = (uint16_t)outv[rr][c];
     }
   }
   // chroma ALF + CC-ALF: one sample per thread
@@ -377,7 +343,8 @@ __device__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y
     for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) {
       const int c = i / n, k = i % n;
       const int x = cx0 + k % cwid, y = cy0 + k / cwid;
-      const lwin& SC = Sc[c];
+      lwin SC = Sc[0];  // Cb and Cr windows share their geometry
+      if (c) SC.p = Sc[1].p;
       int out = SC.rd(x, y);
       if (in_grid) {
         const int alt = (int)((cw >> (c == 0 ? 8 : 16)) & 0xff);

@@ -170,12 +170,14 @@ __global__ void __launch_bounds__(256) k_dbk_chroma(dev_job j) {
   bool mine = eligible && chroma_strong_row(c, beta, tc, short_p);
   bool other = __shfl_xor((int)mine, 1) != 0;
   bool strong = eligible && mine && other;
-  int n[8], lo, hi;
-  chroma_filter_line(c, strong, short_p, tc, n, lo, hi);
+  int n[8];
+  uint32_t wm = chroma_filter_line(c, strong, short_p, tc, n);
   const int maxv = (1 << j.bd) - 1;
-  bool wp = !(e & E_PLTP_BIT), wq = !(e & E_PLTQ_BIT);
-  for (int i = lo; i <= hi; i++) {
-    if ((i < 4 && !wp) || (i >= 4 && !wq)) continue;
+  if (e & E_PLTP_BIT) wm &= 0xf0u;
+  if (e & E_PLTQ_BIT) wm &= 0x0fu;
+#pragma unroll
+  for (int i = 1; i < 7; i++) {
+    if (!((wm >> i) & 1)) continue;
     int x = VERT ? x0 - 4 + i : x0, y = VERT ? y0 : y0 - 4 + i;
     if (x < 0 || y < 0 || x >= v.w || y >= v.h) continue;
     v.p[(size_t)y * v.pitch + x] = (uint16_t)clampi(n[i], 0, maxv);
@@ -286,71 +288,15 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
 #pragma unroll
     for (int cc = 0; cc < 10; cc++) W[rr][cc] = row[clampi(x4 - 3 + cc, cx0, cx1)];
   }
-  // classification (src/alf.rs:822-1010)
   const int y_ctb = ry * ctb;
-  const int yb = y4 - y_ctb;  // y4 within the CTB
-  const int ph = j.h;
-  const bool carve = !(y_ctb + ctb >= ph) || (ph - y_ctb) > ctb - 4;
-  int min_y = -2, max_y = 5, ac = 2;
-  if (yb == ctb - 8 && carve) { max_y = 3; ac = 3; }
-  else if (yb == ctb - 4 && carve) { min_y = 0; ac = 3; }
-  const int vb = y_ctb + ctb - 4;
-  const bool has_top = yb >= ctb - 4;
-  const bool has_bot = (ctb - 4 - yb) > 0 && (ctb - 4 - yb) < 5;
-  // padded row -> window row index
-  auto prow = [&](int yy) -> int {
-    if (has_top && yy < vb) yy = vb;
-    if (has_bot && yy > vb - 1) yy = vb - 1;
-    return yy - (y4 - 3);
-  };
-  int sh = 0, sv = 0, sd0 = 0, sd1 = 0;
-  for (int jj = min_y; jj <= max_y; jj++) {
-    int rc = prow(y4 + jj), ru = prow(y4 + jj - 1), rd = prow(y4 + jj + 1);
+  const bool carve = !(y_ctb + ctb >= j.h) || (j.h - y_ctb) > ctb - 4;
+  int outv[4][4];
+  alf_block(W, bank, y4 - y_ctb, ctb, carve, j.alf_bd, (1 << j.alf_bd) - 1, outv);
 #pragma unroll
-    for (int ii = -2; ii <= 5; ii++) {
-      if (((ii & 1) != (jj & 1))) continue;
-      int cc = ii + 3;
-      int c2 = W[rc][cc] * 2;
-      sh += abs(c2 - W[rc][cc - 1] - W[rc][cc + 1]);
-      sv += abs(c2 - W[ru][cc] - W[rd][cc]);
-      sd0 += abs(c2 - W[ru][cc - 1] - W[rd][cc + 1]);
-      sd1 += abs(c2 - W[ru][cc + 1] - W[rd][cc - 1]);
-    }
-  }
-  int cls = alf_class_from_sums(sh, sv, sd0, sd1, ac, j.alf_bd);
-  const uint32_t* f = bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
-  uint32_t fw[12];
+  for (int rr = 0; rr < 4; rr++)
 #pragma unroll
-  for (int k = 0; k < 12; k++) fw[k] = f[k];
-  const int maxv = (1 << j.alf_bd) - 1;
-  for (int rr = 0; rr < 4; rr++) {
-    int y = y4 + rr;
-    if (y >= src.h) break;
-    int sft, y1, y2, y3;
-    alf_t45(y - y_ctb, ctb, sft, y1, y2, y3);
-    int R = rr + 3;
-    for (int c = 0; c < 4; c++) {
-      int x = x4 + c;
-      if (x >= src.w) break;
-      int C = c + 3;
-      int cur = W[R][C];
-      int sum = 0;
-      sum += alf_pair(fw[0], W[R + y3][C], W[R - y3][C], cur);
-      sum += alf_pair(fw[1], W[R + y2][C + 1], W[R - y2][C - 1], cur);
-      sum += alf_pair(fw[2], W[R + y2][C], W[R - y2][C], cur);
-      sum += alf_pair(fw[3], W[R + y2][C - 1], W[R - y2][C + 1], cur);
-      sum += alf_pair(fw[4], W[R + y1][C + 2], W[R - y1][C - 2], cur);
-      sum += alf_pair(fw[5], W[R + y1][C + 1], W[R - y1][C - 1], cur);
-      sum += alf_pair(fw[6], W[R + y1][C], W[R - y1][C], cur);
-      sum += alf_pair(fw[7], W[R + y1][C - 1], W[R - y1][C + 1], cur);
-      sum += alf_pair(fw[8], W[R + y1][C - 2], W[R - y1][C + 2], cur);
-      sum += alf_pair(fw[9], W[R][C + 3], W[R][C - 3], cur);
-      sum += alf_pair(fw[10], W[R][C + 2], W[R][C - 2], cur);
-      sum += alf_pair(fw[11], W[R][C + 1], W[R][C - 1], cur);
-      int val = cur + ((sum + (1 << (sft - 1))) >> sft);
-      dst.p[(size_t)y * dst.pitch + x] = (uint16_t)clampi(val, 0, maxv);
-    }
-  }
+    for (int c = 0; c < 4; c++)
+      if (y4 + rr < src.h && x4 + c < src.w) dst.p[(size_t)(y4 + rr) * dst.pitch + x4 + c] = (uint16_t)outv[rr][c];
 }
 
 // ---------------------------------------------------------------------------
