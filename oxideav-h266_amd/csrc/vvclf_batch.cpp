@@ -14,9 +14,6 @@ using namespace vvclf;
 
 namespace vvclf {
 size_t fused_lds_bytes(int tw, int th);
-hipError_t launch_fused(int tw, int th, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles, uint32_t grid,
-                        hipStream_t s);
-int fused_max_blocks_per_cu(int tw, int th);
 }  // namespace vvclf
 
 struct vvclf_batch {
@@ -26,6 +23,7 @@ struct vvclf_batch {
   dev_job* d_jobs = nullptr;
   tile_ref* d_tiles = nullptr;
   uint32_t n_tiles = 0, grid = 0;
+  bool hbd = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -81,7 +79,9 @@ int vvclf_batch_create(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, 
   if (!shape_fits(js, t.tile_w, t.tile_h))
     return fail(VVCLF_ERR_INVALID, "batch: tile %ux%u straddles an ALF CTB", t.tile_w, t.tile_h);
   hipSetDevice(ctx->device);
-  int occ = fused_max_blocks_per_cu((int)t.tile_w, (int)t.tile_h);
+  bool hbd = false;
+  for (auto* j : js) hbd = hbd || job_needs_hbd(j->d);
+  int occ = fused_max_blocks_per_cu((int)t.tile_w, (int)t.tile_h, hbd);
   if (occ <= 0) return fail(VVCLF_ERR_HIP, "batch: fused kernel does not fit on a CU");
   if (t.wgs_per_cu == 0 || (int)t.wgs_per_cu > occ) t.wgs_per_cu = (uint32_t)occ;
   t.threads = 256;
@@ -103,6 +103,7 @@ int vvclf_batch_create(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, 
   b->ctx = ctx;
   b->jobs = js;
   b->tune = t;
+  b->hbd = hbd;
   b->n_tiles = (uint32_t)tiles.size();
   uint32_t grid = (uint32_t)cu_count(ctx->device) * t.wgs_per_cu;
   grid = std::min(grid, (b->n_tiles + 7) & ~7u);
@@ -133,8 +134,8 @@ int vvclf_batch_destroy(vvclf_batch* b) {
 int vvclf_batch_run(vvclf_batch* b, void* stream) {
   if (!b) return fail(VVCLF_ERR_INVALID, "batch_run: NULL");
   hipSetDevice(b->ctx->device);
-  hipError_t e = launch_fused((int)b->tune.tile_w, (int)b->tune.tile_h, b->d_jobs, b->d_tiles, b->n_tiles, b->grid,
-                              (hipStream_t)stream);
+  hipError_t e = launch_fused((int)b->tune.tile_w, (int)b->tune.tile_h, b->hbd, b->d_jobs, b->d_tiles, b->n_tiles,
+                              b->grid, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "fused launch");
   for (auto* j : b->jobs) j->fused_out = true;
   return VVCLF_OK;
@@ -152,8 +153,8 @@ int vvclf_batch_graph_launch(vvclf_batch* b, void* stream) {
     if (e != hipSuccess) return hip_fail(e, "graph stream");
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      hipError_t le = launch_fused((int)b->tune.tile_w, (int)b->tune.tile_h, b->d_jobs, b->d_tiles, b->n_tiles,
-                                   b->grid, cs);
+      hipError_t le = launch_fused((int)b->tune.tile_w, (int)b->tune.tile_h, b->hbd, b->d_jobs, b->d_tiles,
+                                   b->n_tiles, b->grid, cs);
       e = hipStreamEndCapture(cs, &b->graph);
       if (le != hipSuccess) e = le;
     }
@@ -209,9 +210,11 @@ int vvclf_autotune(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, vvcl
     return fail(VVCLF_ERR_HIP, "autotune: events");
   float best = 1e30f;
   vvclf_tune bt{};
+  bool hbd = false;
+  for (auto* j : js) hbd = hbd || job_needs_hbd(j->d);
   for (auto& sh : kShapes) {
     if (!shape_fits(js, sh[0], sh[1])) continue;
-    int occ = fused_max_blocks_per_cu(sh[0], sh[1]);
+    int occ = fused_max_blocks_per_cu(sh[0], sh[1], hbd);
     for (int w = 1; w <= occ; w++) {
       vvclf_tune t{(uint32_t)sh[0], (uint32_t)sh[1], 256, (uint32_t)w};
       vvclf_batch* b = nullptr;

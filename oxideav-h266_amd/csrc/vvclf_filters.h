@@ -467,4 +467,136 @@ __device__ __forceinline__ void alf_block(const int (&W)[10][10], const uint32_t
   else alf_block_filter<2>(W, fw, maxv, out);
 }
 
+// ---------------------------------------------------------------------------
+// Packed ALF luma block, for sample bit depths <= 12.
+// The block's neighbourhood is held as u16 pairs: P[r][k] holds the samples at
+// x4-4+2k (low half) and x4-3+2k (high half) of row y4-3+r, so window column c
+// (x = x4-3+c, c = 0..9) is half (c+1)&1 of word (c+1)>>1.  Two horizontally
+// adjacent output samples are filtered together: sample differences (|d| <
+// 2^12), their clips and the sum of a symmetric tap pair (|s| < 2^13) fit in
+// i16 lanes; the products accumulate in 32 bits through v_dot2.
+// Classification and filter follow the same equations as alf_block_class /
+// alf_block_filter above (src/alf.rs:822-1010, :1052-1180).
+// ---------------------------------------------------------------------------
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// sample at window row r, column c
+__device__ __forceinline__ uint32_t pw_at(const uint32_t (&P)[10][6], int r, int c) {
+  const int q = c + 1;
+  return (q & 1) ? (P[r][q >> 1] >> 16) : (P[r][q >> 1] & 0xffffu);
+}
+// samples at window columns c, c+1 of row r as one pair
+__device__ __forceinline__ uint32_t pw_pair(const uint32_t (&P)[10][6], int r, int c) {
+  const int q = c + 1;
+  return (q & 1) ? __builtin_amdgcn_alignbit(P[r][(q >> 1) + 1], P[r][q >> 1], 16) : P[r][q >> 1];
+}
+__device__ __forceinline__ uint32_t absdiff_acc(uint32_t a, uint32_t b, uint32_t acc) {
+  return acc + (a > b ? a - b : b - a);
+}
+
+template <int VB, bool CARVE>
+__device__ __forceinline__ int alf_pk_class(const uint32_t (&P)[10][6], int bd) {
+  constexpr int jmin = (VB == 2 && CARVE) ? 0 : -2;
+  constexpr int jmax = (VB == 1 && CARVE) ? 3 : 5;
+  constexpr int ac = ((VB == 1 || VB == 2) && CARVE) ? 3 : 2;
+  uint32_t sh = 0, sv = 0, sd0 = 0, sd1 = 0;
+#pragma unroll
+  for (int jj = jmin; jj <= jmax; jj++) {
+#pragma unroll
+    for (int ii = -2; ii <= 5; ii++) {
+      if (((ii + 4) & 1) != ((jj + 4) & 1)) continue;
+      int rc = jj + 3, ru = jj + 2, rd = jj + 4;
+      if (VB == 1) { rc = rc > 6 ? 6 : rc; ru = ru > 6 ? 6 : ru; rd = rd > 6 ? 6 : rd; }
+      if (VB == 2) { rc = rc < 3 ? 3 : rc; ru = ru < 3 ? 3 : ru; rd = rd < 3 ? 3 : rd; }
+      const int cc = ii + 3;
+      const uint32_t c2 = pw_at(P, rc, cc) * 2;
+      sh = absdiff_acc(c2, pw_at(P, rc, cc - 1) + pw_at(P, rc, cc + 1), sh);
+      sv = absdiff_acc(c2, pw_at(P, ru, cc) + pw_at(P, rd, cc), sv);
+      sd0 = absdiff_acc(c2, pw_at(P, ru, cc - 1) + pw_at(P, rd, cc + 1), sd0);
+      sd1 = absdiff_acc(c2, pw_at(P, ru, cc + 1) + pw_at(P, rd, cc - 1), sd1);
+    }
+  }
+  return alf_class_from_sums((int)sh, (int)sv, (int)sd0, (int)sd1, ac, bd);
+}
+
+// One tap pair of the 7x7 diamond applied to two output samples: cur holds
+// the pair of centre samples, a / b the pairs at the two symmetric positions;
+// w is the tap word (coefficient i16 | clip << 16).
+__device__ __forceinline__ void alf_pk_tap(uint32_t c0, uint32_t c1, uint32_t clip, s16x2 cur, uint32_t a, uint32_t b,
+                                           int& acc0, int& acc1) {
+  const s16x2 cl = as_s16x2(clip);
+  const s16x2 hi = {cl.x, cl.x}, lo = {cl.y, cl.y};
+  const s16x2 da = __builtin_elementwise_min(__builtin_elementwise_max(as_s16x2(a) - cur, lo), hi);
+  const s16x2 db = __builtin_elementwise_min(__builtin_elementwise_max(as_s16x2(b) - cur, lo), hi);
+  const s16x2 s = da + db;
+  acc0 = __builtin_amdgcn_sdot2(s, as_s16x2(c0), acc0, false);
+  acc1 = __builtin_amdgcn_sdot2(s, as_s16x2(c1), acc1, false);
+}
+
+// Diamond tap geometry: tap k reads (R + dy*Y, C + dx) and (R - dy*Y, C - dx)
+// where Y is the row distance allowed by the virtual boundary for |dy|.
+__device__ constexpr int kTapDy[12] = {3, 2, 2, 2, 1, 1, 1, 1, 1, 0, 0, 0};
+__device__ constexpr int kTapDx[12] = {0, 1, 0, -1, 2, 1, 0, -1, -2, 3, 2, 1};
+
+// out[r][h]: filtered samples (x4+2h, x4+2h+1) of row y4+r as a u16 pair.
+// Taps are the outer loop so that only one tap's constants are live.
+template <int VB>
+__device__ __forceinline__ void alf_pk_filter(const uint32_t (&P)[10][6], const uint32_t* f, int maxv,
+                                              uint32_t (&out)[4][2]) {
+  constexpr int MT[3][4] = {{3, 3, 3, 3}, {3, 2, 1, 0}, {0, 1, 2, 3}};
+  int acc0[4][2], acc1[4][2];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) acc0[r][h] = acc1[r][h] = MT[VB][r] == 0 ? 512 : 64;
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    const uint32_t w = f[k], kk = w >> 16;
+    const uint32_t c0 = w & 0xffffu, c1 = w << 16, clip = kk | ((0u - kk) << 16);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int m = MT[VB][r];
+      const int dy = kTapDy[k] < m ? kTapDy[k] : m;
+      const int R = r + 3;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int C = 3 + 2 * h;
+        alf_pk_tap(c0, c1, clip, as_s16x2(pw_pair(P, R, C)), pw_pair(P, R + dy, C + kTapDx[k]),
+                   pw_pair(P, R - dy, C - kTapDx[k]), acc0[r][h], acc1[r][h]);
+      }
+    }
+  }
+  const s16x2 zero = {0, 0}, vmax = {(short)maxv, (short)maxv};
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int sft = MT[VB][r] == 0 ? 10 : 7;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      // the offsets saturate to i16; the saturating add and the clip to
+      // [0, maxv] then give the same result as the exact sum
+      const s16x2 d = __builtin_amdgcn_cvt_pk_i16(acc0[r][h] >> sft, acc1[r][h] >> sft);
+      const s16x2 o = __builtin_elementwise_add_sat(as_s16x2(pw_pair(P, r + 3, 3 + 2 * h)), d);
+      out[r][h] = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(o, zero), vmax));
+    }
+  }
+}
+
+// Classification + filter of one block from the packed window; `bank` holds
+// the CTB's 25 x 4 x 12 tap words, yb = y4 - yCtb.
+__device__ __forceinline__ void alf_block_pk(const uint32_t (&P)[10][6], const uint32_t* bank, int yb, int ctb,
+                                             bool carve, int bd, int maxv, uint32_t (&out)[4][2]) {
+  int cls;
+  const int vb = yb == ctb - 8 ? 1 : (yb == ctb - 4 ? 2 : 0);
+  if (vb == 0) cls = alf_pk_class<0, false>(P, bd);
+  else if (vb == 1) cls = carve ? alf_pk_class<1, true>(P, bd) : alf_pk_class<1, false>(P, bd);
+  else cls = carve ? alf_pk_class<2, true>(P, bd) : alf_pk_class<2, false>(P, bd);
+  const uint32_t* f = bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
+  if (vb == 0) alf_pk_filter<0>(P, f, maxv, out);
+  else if (vb == 1) alf_pk_filter<1>(P, f, maxv, out);
+  else alf_pk_filter<2>(P, f, maxv, out);
+}
+
 }  // namespace vvclf

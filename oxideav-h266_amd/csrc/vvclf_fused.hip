@@ -23,14 +23,69 @@ template <int TW, int TH>
 struct fused_lds {
   static constexpr int LW = TW + 32, LH = TH + 32;          // luma input region
   static constexpr int CW = TW / 2 + 8, CH = TH / 2 + 8;    // chroma input region
-  static constexpr int SW = TW + 6, SH = TH + 6;            // SAO'd luma
-  static constexpr int SCW = TW / 2 + 4, SCH = TH / 2 + 4;  // SAO'd chroma
+  static constexpr int SW = TW + 8, SH = TH + 6;            // SAO'd luma, origin (x0-4, y0-3)
+  static constexpr int SCW = TW / 2 + 4, SCH = TH / 2 + 4;  // SAO'd chroma, origin (cx0-2, cy0-2)
+  // bounds of the edge-descriptor rectangles a tile's deblocking visits
+  static constexpr int MLV_W = TW / 4 + 6, MLV_H = TH / 4 + 9;  // luma vertical edges
+  static constexpr int MLH_W = TW / 4 + 3, MLH_H = TH / 4 + 6;  // luma horizontal edges
+  static constexpr int MCV_W = TW / 16 + 3, MCV_H = TH / 4 + 5; // chroma vertical edges
+  static constexpr int MCH_W = TW / 4 + 5, MCH_H = TH / 16 + 3; // chroma horizontal edges
+  struct maps_t {
+    uint32_t lv[MLV_H * MLV_W], lh[MLH_H * MLH_W], cv[2][MCV_H * MCV_W], ch[2][MCH_H * MCH_W];
+  };
+  struct sao_out_t {
+    uint16_t S[SH * SW];
+    uint16_t Sc[2][SCH * SCW];
+  };
   uint16_t L[LH * LW];
   uint16_t Cc[2][CH * CW];
-  uint16_t S[SH * SW];
-  uint16_t Sc[2][SCH * SCW];
+  // the edge descriptors are dead once deblocking is done, before SAO writes
+  union {
+    maps_t m;
+    sao_out_t o;
+  };
   uint32_t bank[ALF_BANK_WORDS];
+  uint32_t cbank[ALF_CBANK_WORDS];
+  int32_t ccbank[2 * ALF_CCBANK_WORDS];
+  uint64_t saop[9][3];  // SAO parameters of the (up to) 3x3 CTBs around the tile
+  uint64_t saor[9];     // and their sample rectangles
   uint16_t lut[1024];
+};
+
+// A rectangle of an edge-descriptor map (map units), staged in LDS with
+// origin (x0, y0).
+struct mrect {
+  int x0, y0, nx, ny;
+};
+
+// Global -> register -> LDS staging of up to PER*256 words of a map
+// rectangle; issue() for every rectangle first, then commit(), so that all
+// loads of a tile are in flight together.
+template <int PER>
+struct rect_stage {
+  uint32_t v[PER];
+  __device__ __forceinline__ void issue(const uint32_t* map, int mw, const mrect& r) {
+    const int n = r.nx > 0 && r.ny > 0 ? r.nx * r.ny : 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = (int)threadIdx.x + k * 256;
+      if (i < n) {
+        const int yy = i / r.nx, xx = i - yy * r.nx;
+        v[k] = map[(size_t)(r.y0 + yy) * mw + r.x0 + xx];
+      }
+    }
+  }
+  __device__ __forceinline__ void commit(uint32_t* dst, int pitch, const mrect& r) const {
+    const int n = r.nx > 0 && r.ny > 0 ? r.nx * r.ny : 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = (int)threadIdx.x + k * 256;
+      if (i < n) {
+        const int yy = i / r.nx, xx = i - yy * r.nx;
+        dst[yy * pitch + xx] = v[k];
+      }
+    }
+  }
 };
 
 // A rectangular LDS window holding picture samples [ox, ox+w) x [oy, oy+h)
@@ -67,20 +122,21 @@ __device__ __forceinline__ luma_terms shfl_terms4(const luma_terms& t, int src) 
 // VERT: edge columns gx in [gx0, gx1], segment rows gy in [gy0, gy1].
 // HOR : edge rows gy in [gy0, gy1], segment columns gx in [gx0, gx1].
 template <bool VERT>
-__device__ void lds_deblock_luma(const dev_job& j, lwin& L, int gx0, int gx1, int gy0, int gy1, int maxv) {
+__device__ __forceinline__ void lds_deblock_luma(const dev_job& j, lwin& L, const uint32_t* map, int mpitch, const mrect& mr,
+                                 int maxv) {
   const int tid = threadIdx.x;
-  const int ncol = gx1 - gx0 + 1, nrow = gy1 - gy0 + 1;
+  const int ncol = mr.nx, nrow = mr.ny;
   const int nseg = ncol * nrow;
-  const uint32_t* map = VERT ? j.e_lv : j.e_lh;
   const int lane = tid & 63;
   for (int base = 0; base < nseg; base += blockDim.x / 4) {
     const int seg = base + (tid >> 2), line = tid & 3;
     uint32_t e = 0;
     int gx = 0, gy = 0;
     if (seg < nseg) {
-      gx = gx0 + seg % ncol;
-      gy = gy0 + seg / ncol;
-      e = map[(size_t)gy * j.gw + gx];
+      const int ry = seg / ncol, rx = seg - ry * ncol;
+      gx = mr.x0 + rx;
+      gy = mr.y0 + ry;
+      e = map[ry * mpitch + rx];
     }
     if (e != 0) {  // uniform across the quad
       const int x0 = VERT ? 4 * gx : 4 * gx + line, y0 = VERT ? 4 * gy + line : 4 * gy;
@@ -120,22 +176,21 @@ __device__ void lds_deblock_luma(const dev_job& j, lwin& L, int gx0, int gx1, in
 // VERT: edge columns sx in [sx0, sx1] (x = 8 sx), segment rows sy in [sy0, sy1] (y = 2 sy).
 // HOR : edge rows sy in [sy0, sy1] (y = 8 sy), segment columns sx in [sx0, sx1] (x = 2 sx).
 template <bool VERT>
-__device__ void lds_deblock_chroma(const dev_job& j, lwin cb, uint16_t* cr_p, int sx0, int sx1, int sy0, int sy1,
-                                   int maxv) {
+__device__ __forceinline__ void lds_deblock_chroma(lwin cb, uint16_t* cr_p, const uint32_t* map0, const uint32_t* map1, int mpitch,
+                                   const mrect& mr, int bd_dbk, int maxv) {
   const int tid = threadIdx.x;
-  const int ncol = sx1 - sx0 + 1, nrow = sy1 - sy0 + 1;
+  const int ncol = mr.nx, nrow = mr.ny;
   const int nseg = ncol * nrow;
-  const int mw = VERT ? j.cvw : j.chw;
   for (int base = 0; base < 2 * nseg; base += blockDim.x / 2) {
     const int item = base + (tid >> 1), line = tid & 1;
     uint32_t e = 0;
     int sx = 0, sy = 0, comp = 0;
     if (item < 2 * nseg) {
-      comp = item / nseg;
-      int seg = item % nseg;
-      sx = sx0 + seg % ncol;
-      sy = sy0 + seg / ncol;
-      e = (VERT ? j.e_cv[comp] : j.e_ch[comp])[(size_t)sy * mw + sx];
+      comp = item >= nseg ? 1 : 0;
+      const int seg = item - comp * nseg, ry = seg / ncol, rx = seg - ry * ncol;
+      sx = mr.x0 + rx;
+      sy = mr.y0 + ry;
+      e = (comp ? map1 : map0)[ry * mpitch + rx];
     }
     if (e != 0) {  // uniform across the pair
       lwin L = cb;  // Cb and Cr windows share their geometry
@@ -146,7 +201,7 @@ __device__ void lds_deblock_chroma(const dev_job& j, lwin cb, uint16_t* cr_p, in
 #pragma unroll
       for (int i = 0; i < 8; i++) c[i] = VERT ? L.rd(x0 - 4 + i, y0) : L.rd(x0, y0 - 4 + i);
       int beta, tc;
-      edge_thresholds(e, j.bd_dbk, beta, tc);
+      edge_thresholds(e, bd_dbk, beta, tc);
       int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
       bool eligible, short_p;
       if (VERT) { eligible = mflp == 3 && mflq == 3; short_p = false; }
@@ -167,9 +222,19 @@ __device__ void lds_deblock_chroma(const dev_job& j, lwin cb, uint16_t* cr_p, in
   }
 }
 
+// SAO parameters of the CTBs around a tile: CTB columns [bx, bx + 3) and
+// rows [by, by + 3) are staged in LDS when they cover the tile's SAO reach
+// (`ok`); otherwise (SAO CTBs smaller than a third of the reach) the
+// parameters are read from the job's tables.
+struct sao_tab {
+  int bx, by;
+  bool ok;
+};
+
 // SAO of one component from window `src` into window `dst` (dst's extent is
 // the region produced).  sub = 1 for chroma.
-__device__ void lds_sao(const dev_job& j, const lwin& src, lwin& dst, int comp, int pw, int ph) {
+__device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)[3], const uint64_t* saor, const sao_tab& tb,
+                        const lwin& src, lwin& dst, int comp, int pw, int ph) {
   const int sub = comp ? 1 : 0;
   const int log2c = j.sao_ctb_log2 - sub;
   const bool active = j.do_sao && (comp == 0 ? j.sao_luma : j.sao_chroma);
@@ -184,7 +249,8 @@ __device__ void lds_sao(const dev_job& j, const lwin& src, lwin& dst, int comp, 
     const int rx = x >> log2c, ry = y >> log2c;
     if (active && rx < j.swctb && ry < j.shctb) {
       const size_t ci = (size_t)ry * j.swctb + rx;
-      const uint64_t prm = j.sao[ci * 3 + comp];
+      const int cell = (ry - tb.by) * 3 + (rx - tb.bx);
+      const uint64_t prm = tb.ok ? saop[cell][comp] : j.sao[ci * 3 + comp];
       const int type = (int)(prm & 3);
       if (type == 2) {
         int h0, v0, h1, v1;
@@ -192,7 +258,7 @@ __device__ void lds_sao(const dev_job& j, const lwin& src, lwin& dst, int comp, 
         const int nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
         bool outside = nx0 < 0 || ny0 < 0 || nx1 < 0 || ny1 < 0 || nx0 >= pw || ny0 >= ph || nx1 >= pw || ny1 >= ph;
         if (j.sao_rect) {
-          uint64_t r = j.sao_rect[ci];
+          uint64_t r = tb.ok ? saor[cell] : j.sao_rect[ci];
           int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff),
               cy1 = (int)(r >> 48);
           if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
@@ -208,130 +274,295 @@ __device__ void lds_sao(const dev_job& j, const lwin& src, lwin& dst, int comp, 
   }
 }
 
-template <int TW, int TH>
-__device__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y0) {
+// Replaces every sample of buf (storage origin (ox, oy), pitch `pitch`) in
+// [xa, xb) x [ya, yb) that lies outside the rectangle [rx0, rx1] x [ry0, ry1]
+// by the sample at the clamped position, which lies inside the rectangle:
+// reads touch only unchanged samples, so one pass suffices.  ALF reads of a
+// tile (which lies inside one CTB and so has one rectangle) are then plain
+// reads of the padded buffer.
+__device__ __forceinline__ void pad_to_rect(uint16_t* buf, int pitch, int ox, int oy, int xa, int xb, int ya, int yb, int rx0, int rx1,
+                            int ry0, int ry1) {
+  const int w = xb - xa, n = w * (yb - ya);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int y = ya + i / w, x = xa + i % w;
+    if (x >= rx0 && x <= rx1 && y >= ry0 && y <= ry1) continue;
+    buf[(y - oy) * pitch + (x - ox)] = buf[(clampi(y, ry0, ry1) - oy) * pitch + (clampi(x, rx0, rx1) - ox)];
+  }
+}
+
+template <int TW, int TH, bool HBD>
+__device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y0) {
   typedef fused_lds<TW, TH> LD;
+  constexpr int NT = 256;  // threads per workgroup
   const int W = j.w, H = j.h;
   const int x1 = min(x0 + TW, W), y1 = min(y0 + TH, H);
   const int maxv = (1 << j.bd) - 1;
   const bool chroma = j.wc > 0;
-  // ---- load (LMCS inverse folded into the load)
+  const int cx0 = x0 >> 1, cy0 = y0 >> 1, cx1 = min(x1 >> 1, j.wc), cy1 = min(y1 >> 1, j.hc);
+  // ---- geometry of everything the tile reads
   lwin L{s.L, LD::LW, max(0, x0 - 16), max(0, y0 - 16), 0, 0};
   L.w = min(W, x1 + 16) - L.ox;
   L.h = min(H, y1 + 16) - L.oy;
-  {
-    const uint16_t* src = j.in.y;
-    const bool lut_lds = j.do_lmcs && j.bd <= 10;
-    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    for (int yy = threadIdx.x >> 6; yy < L.h; yy += nw)
-    for (int xx = lane; xx < L.w; xx += 64) {
-      uint16_t v = src[(size_t)(L.oy + yy) * j.py + L.ox + xx];
-      if (j.do_lmcs) v = (lut_lds && v < 1024) ? s.lut[v] : j.lmcs_lut[v];
-      s.L[yy * LD::LW + xx] = v;
-    }
+  // chroma window (Cb; Cr has the same geometry in s.Cc[1])
+  lwin Cw{s.Cc[0], LD::CW, max(0, cx0 - 4), max(0, cy0 - 4), 0, 0};
+  Cw.w = min(j.wc, cx1 + 4) - Cw.ox;
+  Cw.h = min(j.hc, cy1 + 4) - Cw.oy;
+  const bool dbk = j.do_dbk, dbk_c = chroma && j.do_dbk && j.do_dbk_chroma;
+  mrect mlv{0, 0, 0, 0}, mlh{0, 0, 0, 0}, mcv{0, 0, 0, 0}, mch{0, 0, 0, 0};
+  if (dbk) {
+    // vertical edges x in [x0-10, x1+10] over the window rows
+    mlv.x0 = max(0, cdiv_i(x0 - 10, 4));
+    mlv.nx = min(j.gw - 1, fdiv(x1 + 10, 4)) - mlv.x0 + 1;
+    mlv.y0 = L.oy / 4;
+    mlv.ny = min(j.gh - 1, (L.oy + L.h - 1) / 4) - mlv.y0 + 1;
+    // horizontal edges y in [y0-10, y1+10] over columns [x0-4, x1+4)
+    mlh.y0 = max(0, cdiv_i(y0 - 10, 4));
+    mlh.ny = min(j.gh - 1, fdiv(y1 + 10, 4)) - mlh.y0 + 1;
+    mlh.x0 = max(0, fdiv(x0 - 4, 4));
+    mlh.nx = min(j.gw - 1, fdiv(min(W, x1 + 4) - 1, 4)) - mlh.x0 + 1;
   }
-  const int cx0 = x0 >> 1, cy0 = y0 >> 1, cx1 = min(x1 >> 1, j.wc), cy1 = min(y1 >> 1, j.hc);
-  lwin Cw[2];
-  if (chroma) {
-    for (int c = 0; c < 2; c++) {
-      Cw[c] = lwin{s.Cc[c], LD::CW, max(0, cx0 - 4), max(0, cy0 - 4), 0, 0};
-      Cw[c].w = min(j.wc, cx1 + 4) - Cw[c].ox;
-      Cw[c].h = min(j.hc, cy1 + 4) - Cw[c].oy;
-      const uint16_t* src = c ? j.in.cr : j.in.cb;
-      const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-      for (int yy = threadIdx.x >> 6; yy < Cw[c].h; yy += nw)
-        for (int xx = lane; xx < Cw[c].w; xx += 64)
-          s.Cc[c][yy * LD::CW + xx] = src[(size_t)(Cw[c].oy + yy) * j.pc + Cw[c].ox + xx];
-    }
+  if (dbk_c) {
+    // chroma vertical edges x in [cx0-5, cx1+5], horizontal y in [cy0-5, cy1+5]
+    mcv.x0 = max(0, cdiv_i(cx0 - 5, 8));
+    mcv.nx = min(j.cvw - 1, fdiv(cx1 + 5, 8)) - mcv.x0 + 1;
+    mcv.y0 = Cw.oy / 2;
+    mcv.ny = min(j.cvh - 1, (Cw.oy + Cw.h - 1) / 2) - mcv.y0 + 1;
+    mch.y0 = max(0, cdiv_i(cy0 - 5, 8));
+    mch.ny = min(j.chh - 1, fdiv(cy1 + 5, 8)) - mch.y0 + 1;
+    mch.x0 = max(0, fdiv(cx0 - 3, 2));
+    mch.nx = min(j.chw - 1, fdiv(min(j.wc, cx1 + 3) - 1, 2)) - mch.x0 + 1;
   }
-  __syncthreads();
-  // ---- deblocking: all vertical edges, then all horizontal edges
-  if (j.do_dbk) {
-    {
-      int gx0 = max(0, cdiv_i(x0 - 10, 4)), gx1 = min(j.gw - 1, fdiv(x1 + 10, 4));  // x in [x0-10, x1+10]
-      int gy0 = L.oy / 4, gy1 = min(j.gh - 1, (L.oy + L.h - 1) / 4);
-      lds_deblock_luma<true>(j, L, gx0, gx1, gy0, gy1, maxv);
-    }
-    if (chroma && j.do_dbk_chroma) {
-      int sx0 = max(0, cdiv_i(cx0 - 5, 8)), sx1 = min(j.cvw - 1, fdiv(cx1 + 5, 8));  // x in [cx0-5, cx1+5]
-      int sy0 = Cw[0].oy / 2, sy1 = min(j.cvh - 1, (Cw[0].oy + Cw[0].h - 1) / 2);
-      lds_deblock_chroma<true>(j, Cw[0], Cw[1].p, sx0, sx1, sy0, sy1, maxv);
-    }
-    __syncthreads();
-    {
-      int gy0 = max(0, cdiv_i(y0 - 10, 4)), gy1 = min(j.gh - 1, fdiv(y1 + 10, 4));  // y in [y0-10, y1+10]
-      int gx0 = max(0, fdiv(x0 - 4, 4)), gx1 = min(j.gw - 1, fdiv(min(W, x1 + 4) - 1, 4));
-      lds_deblock_luma<false>(j, L, gx0, gx1, gy0, gy1, maxv);
-    }
-    if (chroma && j.do_dbk_chroma) {
-      int sy0 = max(0, cdiv_i(cy0 - 5, 8)), sy1 = min(j.chh - 1, fdiv(cy1 + 5, 8));  // y in [cy0-5, cy1+5]
-      int sx0 = max(0, fdiv(cx0 - 3, 2)), sx1 = min(j.chw - 1, fdiv(min(j.wc, cx1 + 3) - 1, 2));
-      lds_deblock_chroma<false>(j, Cw[0], Cw[1].p, sx0, sx1, sy0, sy1, maxv);
-    }
-    __syncthreads();
+  // SAO parameter table: CTBs covering luma x in [x0-4, x1+3] (chroma reach
+  // included) and likewise y
+  sao_tab tb{0, 0, false};
+  if (j.do_sao) {
+    tb.bx = max(0, (x0 - 4) >> j.sao_ctb_log2);
+    tb.by = max(0, (y0 - 4) >> j.sao_ctb_log2);
+    tb.ok = ((x1 + 3) >> j.sao_ctb_log2) - tb.bx < 3 && ((y1 + 3) >> j.sao_ctb_log2) - tb.by < 3;
   }
-  // ---- SAO into the second buffer
-  lwin S{s.S, LD::SW, max(0, x0 - 3), max(0, y0 - 3), 0, 0};
-  S.w = min(W, x1 + 3) - S.ox;
-  S.h = min(H, y1 + 3) - S.oy;
-  lds_sao(j, L, S, 0, W, H);
-  lwin Sc[2];
-  if (chroma) {
-    for (int c = 0; c < 2; c++) {
-      Sc[c] = lwin{s.Sc[c], LD::SCW, max(0, cx0 - 2), max(0, cy0 - 2), 0, 0};
-      Sc[c].w = min(j.wc, cx1 + 2) - Sc[c].ox;
-      Sc[c].h = min(j.hc, cy1 + 2) - Sc[c].oy;
-      lds_sao(j, Cw[c], Sc[c], c + 1, j.wc, j.hc);
-    }
-  }
-  // ---- ALF: this tile lies inside one CTB
+  // ALF parameters of the tile's CTB
   const int ctb = 1 << j.ctb_log2;
   const int rx = x0 >> j.ctb_log2, ry = y0 >> j.ctb_log2;
   const bool in_grid = j.do_alf && rx < j.wctb && ry < j.hctb;
   const size_t ci = in_grid ? (size_t)ry * j.wctb + rx : 0;
   const uint32_t cw = in_grid ? j.alf_ctb[ci] : 0xffffffffu;
   const int bank_idx = (int)(cw & 0xff);
-  if (bank_idx != (int)ALF_OFF8) {
-    for (int i = threadIdx.x; i < ALF_BANK_WORDS; i += blockDim.x)
-      s.bank[i] = j.alf_bank[(size_t)bank_idx * ALF_BANK_WORDS + i];
+  const uint64_t r = in_grid ? j.alf_rect[ci] : (((uint64_t)W << 32) | ((uint64_t)H << 48));
+  // ---- load: samples (8 luma / 4 chroma per access), edge descriptors, SAO
+  // and ALF parameters; every load of a thread is issued before the first is
+  // used.  Window origins are multiples of 16 (luma) / 4 (chroma) samples and
+  // plane pitches multiples of 64, so the vector accesses are aligned; they
+  // may read up to 7 samples past the window's last column, inside the pitch.
+  {
+    constexpr int NCH = LD::LW / 8, PER = (LD::LH * NCH + NT - 1) / NT;
+    constexpr int NC4 = LD::CW / 4, PERC = (2 * LD::CH * NC4 + NT - 1) / NT;
+    constexpr int PLV = (LD::MLV_W * LD::MLV_H + NT - 1) / NT, PLH = (LD::MLH_W * LD::MLH_H + NT - 1) / NT;
+    constexpr int PCV = (LD::MCV_W * LD::MCV_H + NT - 1) / NT, PCH = (LD::MCH_W * LD::MCH_H + NT - 1) / NT;
+    constexpr int PBK = (ALF_BANK_WORDS + NT - 1) / NT;
+    const int tid = (int)threadIdx.x;
+    const int nch = (L.w + 7) >> 3, total = L.h * NCH;
+    const uint16_t* src = j.in.y + (size_t)L.oy * j.py + L.ox;
+    uint4 v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
+      if (i < total && ch < nch) v[k] = *reinterpret_cast<const uint4*>(src + (size_t)yy * j.py + 8 * ch);
+    }
+    const int nchc = (Cw.w + 3) >> 2, per_plane = Cw.h * NC4;
+    uint2 vc[PERC];
+    if (chroma) {
+#pragma unroll
+      for (int k = 0; k < PERC; k++) {
+        const int i = tid + k * NT, c = i >= per_plane ? 1 : 0, ii = i - c * per_plane;
+        const int yy = ii / NC4, ch = ii - yy * NC4;
+        const uint16_t* csrc = c ? j.in.cr : j.in.cb;
+        if (i < 2 * per_plane && ch < nchc)
+          vc[k] = *reinterpret_cast<const uint2*>(csrc + (size_t)(Cw.oy + yy) * j.pc + Cw.ox + 4 * ch);
+      }
+    }
+    rect_stage<PLV> slv;
+    rect_stage<PLH> slh;
+    rect_stage<PCV> scv0, scv1;
+    rect_stage<PCH> sch0, sch1;
+    if (dbk) {
+      slv.issue(j.e_lv, j.gw, mlv);
+      slh.issue(j.e_lh, j.gw, mlh);
+    }
+    if (dbk_c) {
+      scv0.issue(j.e_cv[0], j.cvw, mcv);
+      scv1.issue(j.e_cv[1], j.cvw, mcv);
+      sch0.issue(j.e_ch[0], j.chw, mch);
+      sch1.issue(j.e_ch[1], j.chw, mch);
+    }
+    uint32_t bk[PBK];
+    const bool alf_on = bank_idx != (int)ALF_OFF8;
+    const uint32_t* bsrc = j.alf_bank + (size_t)(alf_on ? bank_idx : 0) * ALF_BANK_WORDS;
+#pragma unroll
+    for (int k = 0; k < PBK; k++) bk[k] = (alf_on && tid + k * NT < ALF_BANK_WORDS) ? bsrc[tid + k * NT] : 0u;
+    // small tables: chroma ALF / CC-ALF banks (104 words), SAO cells (36)
+    uint64_t small = 0;
+    if (in_grid && tid < ALF_CBANK_WORDS) small = j.alf_cbank[tid];
+    else if (in_grid && tid < ALF_CBANK_WORDS + 2 * ALF_CCBANK_WORDS)
+      small = (uint32_t)j.alf_ccbank[tid - ALF_CBANK_WORDS];
+    uint64_t sv = 0;
+    const int scell = tid - 128, scx = tb.bx + (scell >> 2) % 3, scy = tb.by + (scell >> 2) / 3, sk = scell & 3;
+    const bool sao_ld = tb.ok && scell >= 0 && scell < 36 && scx < j.swctb && scy < j.shctb;
+    if (sao_ld) {
+      const size_t sci = (size_t)scy * j.swctb + scx;
+      sv = sk < 3 ? j.sao[sci * 3 + sk] : (j.sao_rect ? j.sao_rect[sci] : 0);
+    }
+    // ---- commit to LDS
+    const int lut_lds = j.bd <= 10;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
+      if (i < total && ch < nch) {
+        if (j.do_lmcs) {
+          uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const uint32_t a = w4[q] & 0xffffu, b = w4[q] >> 16;
+            const uint32_t la = (lut_lds && a < 1024) ? s.lut[a] : j.lmcs_lut[a];
+            const uint32_t lb = (lut_lds && b < 1024) ? s.lut[b] : j.lmcs_lut[b];
+            w4[q] = la | (lb << 16);
+          }
+          v[k] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        }
+        *reinterpret_cast<uint4*>(&s.L[yy * LD::LW + 8 * ch]) = v[k];
+      }
+    }
+    if (chroma) {
+#pragma unroll
+      for (int k = 0; k < PERC; k++) {
+        const int i = tid + k * NT, c = i >= per_plane ? 1 : 0, ii = i - c * per_plane;
+        const int yy = ii / NC4, ch = ii - yy * NC4;
+        if (i < 2 * per_plane && ch < nchc) *reinterpret_cast<uint2*>(&s.Cc[c][yy * LD::CW + 4 * ch]) = vc[k];
+      }
+    }
+    if (dbk) {
+      slv.commit(s.m.lv, LD::MLV_W, mlv);
+      slh.commit(s.m.lh, LD::MLH_W, mlh);
+    }
+    if (dbk_c) {
+      scv0.commit(s.m.cv[0], LD::MCV_W, mcv);
+      scv1.commit(s.m.cv[1], LD::MCV_W, mcv);
+      sch0.commit(s.m.ch[0], LD::MCH_W, mch);
+      sch1.commit(s.m.ch[1], LD::MCH_W, mch);
+    }
+    if (alf_on) {
+#pragma unroll
+      for (int k = 0; k < PBK; k++)
+        if (tid + k * NT < ALF_BANK_WORDS) s.bank[tid + k * NT] = bk[k];
+    }
+    if (in_grid && tid < ALF_CBANK_WORDS) s.cbank[tid] = (uint32_t)small;
+    else if (in_grid && tid < ALF_CBANK_WORDS + 2 * ALF_CCBANK_WORDS) s.ccbank[tid - ALF_CBANK_WORDS] = (int32_t)small;
+    if (tb.ok && scell >= 0 && scell < 36) {
+      if (sk < 3) s.saop[scell >> 2][sk] = sv;
+      else s.saor[scell >> 2] = sv;
+    }
   }
   __syncthreads();
-  uint64_t r = in_grid ? j.alf_rect[ci] : (((uint64_t)W << 32) | ((uint64_t)H << 48));
+  // ---- deblocking: all vertical edges, then all horizontal edges
+  if (dbk) {
+    lds_deblock_luma<true>(j, L, s.m.lv, LD::MLV_W, mlv, maxv);
+    if (dbk_c) lds_deblock_chroma<true>(Cw, s.Cc[1], s.m.cv[0], s.m.cv[1], LD::MCV_W, mcv, j.bd_dbk, maxv);
+    __syncthreads();
+    lds_deblock_luma<false>(j, L, s.m.lh, LD::MLH_W, mlh, maxv);
+    if (dbk_c) lds_deblock_chroma<false>(Cw, s.Cc[1], s.m.ch[0], s.m.ch[1], LD::MCH_W, mch, j.bd_dbk, maxv);
+  }
+  __syncthreads();  // deblocking done; the edge descriptors' LDS is reused
+  // ---- SAO into the second buffers.  S stores luma [x0-4, x0+TW+4) x
+  // [y0-3, y0+TH+3) and Sc chroma [cx0-2, cx0+TW/2+2) x [cy0-2, ...) at fixed
+  // origins (so a 4x4 block's ALF neighbourhood is three aligned 8-byte reads
+  // per row); the windows describe the part SAO produces (inside the picture).
+  const int sox = x0 - 4, soy = y0 - 3, scox = cx0 - 2, scoy = cy0 - 2;
+  lwin S{nullptr, LD::SW, max(0, x0 - 3), max(0, y0 - 3), 0, 0};
+  S.w = min(W, x1 + 3) - S.ox;
+  S.h = min(H, y1 + 3) - S.oy;
+  S.p = s.o.S + (S.oy - soy) * LD::SW + (S.ox - sox);
+  lds_sao(j, s.saop, s.saor, tb, L, S, 0, W, H);
+  if (chroma) {
+    for (int c = 0; c < 2; c++) {
+      lwin Sc{nullptr, LD::SCW, max(0, cx0 - 2), max(0, cy0 - 2), 0, 0};
+      Sc.w = min(j.wc, cx1 + 2) - Sc.ox;
+      Sc.h = min(j.hc, cy1 + 2) - Sc.oy;
+      Sc.p = s.o.Sc[c] + (Sc.oy - scoy) * LD::SCW + (Sc.ox - scox);
+      lwin Cs = Cw;
+      Cs.p = s.Cc[c];
+      lds_sao(j, s.saop, s.saor, tb, Cs, Sc, c + 1, j.wc, j.hc);
+    }
+  }
+  // ---- ALF: this tile lies inside one CTB
   const int rx0 = (int)(r & 0xffff), ry0 = (int)((r >> 16) & 0xffff), rx1 = (int)((r >> 32) & 0xffff) - 1,
             ry1 = (int)(r >> 48) - 1;
+  const int bw = (x1 - x0 + 3) / 4, bh = (y1 - y0 + 3) / 4;
+  const int xe = x0 + 4 * bw, ye = y0 + 4 * bh;  // block-aligned end of the tile
+  const int ccx0 = rx0 >> 1, ccy0 = ry0 >> 1, ccx1 = ((rx1 + 2) >> 1) - 1, ccy1 = ((ry1 + 2) >> 1) - 1;
+  const bool luma_interior = x0 - 3 >= rx0 && xe + 2 <= rx1 && y0 - 3 >= ry0 && ye + 2 <= ry1;
+  const bool chroma_interior = cx0 - 2 >= ccx0 && cx1 + 1 <= ccx1 && cy0 - 2 >= ccy0 && cy1 + 1 <= ccy1;
+  if (!luma_interior || (chroma && !chroma_interior)) {
+    __syncthreads();  // SAO output complete
+    if (!luma_interior) pad_to_rect(s.o.S, LD::SW, sox, soy, x0 - 3, xe + 3, y0 - 3, ye + 3, rx0, rx1, ry0, ry1);
+    if (chroma && !chroma_interior)
+      for (int c = 0; c < 2; c++)
+        pad_to_rect(s.o.Sc[c], LD::SCW, scox, scoy, cx0 - 2, cx1 + 2, cy0 - 2, cy1 + 2, ccx0, ccx1, ccy0, ccy1);
+  }
+  __syncthreads();
+  // padded luma sample (x, y), |x - [x0, xe)| <= 3, |y - [y0, ye)| <= 3
+  auto Sl = [&](int x, int y) -> int { return s.o.S[(y - soy) * LD::SW + (x - sox)]; };
   // luma: one 4x4 block per thread
   {
-    const int bw = (x1 - x0 + 3) / 4, bh = (y1 - y0 + 3) / 4;
     const int amax = (1 << j.alf_bd) - 1;
-    for (int b = threadIdx.x; b < bw * bh; b += blockDim.x) {
+    const int y_ctb = ry * ctb;
+    const bool carve = !(y_ctb + ctb >= H) || (H - y_ctb) > ctb - 4;
+    uint16_t* dst = j.out.y;
+    for (int b = threadIdx.x; b < bw * bh; b += NT) {
       const int x4 = x0 + 4 * (b % bw), y4 = y0 + 4 * (b / bw);
-      uint16_t* dst = j.out.y;
+      uint32_t outp[4][2];
       if (bank_idx == (int)ALF_OFF8) {
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+          const uint2 q = *reinterpret_cast<const uint2*>(&s.o.S[(y4 + rr - soy) * LD::SW + (x4 - sox)]);
+          outp[rr][0] = q.x;
+          outp[rr][1] = q.y;
+        }
+      } else if (HBD) {
+        int Wn[10][10];
+#pragma unroll
+        for (int rr = 0; rr < 10; rr++)
+#pragma unroll
+          for (int cc = 0; cc < 10; cc++) Wn[rr][cc] = Sl(x4 - 3 + cc, y4 - 3 + rr);
+        int outv[4][4];
+        alf_block(Wn, s.bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outv);
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+          outp[rr][0] = (uint32_t)outv[rr][0] | ((uint32_t)outv[rr][1] << 16);
+          outp[rr][1] = (uint32_t)outv[rr][2] | ((uint32_t)outv[rr][3] << 16);
+        }
+      } else {
+        uint32_t P[10][6];
+        const uint16_t* base = &s.o.S[(y4 - 3 - soy) * LD::SW + (x4 - 4 - sox)];
+#pragma unroll
+        for (int rr = 0; rr < 10; rr++) {
+          const uint2* q = reinterpret_cast<const uint2*>(base + rr * LD::SW);
+          const uint2 a = q[0], b2 = q[1], c2 = q[2];
+          P[rr][0] = a.x; P[rr][1] = a.y; P[rr][2] = b2.x; P[rr][3] = b2.y; P[rr][4] = c2.x; P[rr][5] = c2.y;
+        }
+        alf_block_pk(P, s.bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outp);
+      }
+      if (x4 + 4 <= x1) {
+#pragma unroll
         for (int rr = 0; rr < 4; rr++)
-          for (int c = 0; c < 4; c++) {
-            int x = x4 + c, y = y4 + rr;
-            if (x < x1 && y < y1) dst[(size_t)y * j.py + x] = (uint16_t)S.rd(x, y);
-          }
-        continue;
+          if (y4 + rr < y1)
+            *reinterpret_cast<uint2*>(dst + (size_t)(y4 + rr) * j.py + x4) = make_uint2(outp[rr][0], outp[rr][1]);
+      } else {
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            if (y4 + rr < y1 && x4 + c < x1)
+              dst[(size_t)(y4 + rr) * j.py + x4 + c] = (uint16_t)(outp[rr][c >> 1] >> (16 * (c & 1)));
       }
-      // W[r][c] = sample(x4-3+c, y4-3+r): clamp to the fetch rectangle, then
-      // read the SAO'd window (the clamped position always lies inside it).
-      int W[10][10];
-#pragma unroll
-      for (int rr = 0; rr < 10; rr++) {
-        const uint16_t* row = S.p + (clampi(y4 - 3 + rr, ry0, ry1) - S.oy) * S.pitch - S.ox;
-#pragma unroll
-        for (int cc = 0; cc < 10; cc++) W[rr][cc] = row[clampi(x4 - 3 + cc, rx0, rx1)];
-      }
-      const int y_ctb = ry * ctb;
-      const bool carve = !(y_ctb + ctb >= H) || (H - y_ctb) > ctb - 4;
-      int outv[4][4];
-      alf_block(W, s.bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outv);
-#pragma unroll
-      for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-          if (y4 + rr < y1 && x4 + c < x1) dst[(size_t)(y4 + rr) * j.py + x4 + c] = (uint16_t)outv[rr][c];
     }
   }
   // chroma ALF + CC-ALF: one sample per thread
@@ -339,21 +570,20 @@ __device__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y
     const int cwid = cx1 - cx0, chei = cy1 - cy0, n = cwid * chei;
     const int ctbc = ctb >> 1;
     const int amax = (1 << j.alf_bd) - 1, half = 1 << (j.alf_bd - 1);
-    const int ccx0 = rx0 >> 1, ccy0 = ry0 >> 1, ccx1 = ((rx1 + 2) >> 1) - 1, ccy1 = ((ry1 + 2) >> 1) - 1;
-    for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) {
-      const int c = i / n, k = i % n;
-      const int x = cx0 + k % cwid, y = cy0 + k / cwid;
-      lwin SC = Sc[0];  // Cb and Cr windows share their geometry
-      if (c) SC.p = Sc[1].p;
-      int out = SC.rd(x, y);
+    for (int i = threadIdx.x; i < 2 * n; i += NT) {
+      const int c = i >= n ? 1 : 0, k = i - c * n;
+      const int y = cy0 + k / cwid, x = cx0 + k % cwid;
+      const uint16_t* sc = s.o.Sc[c];
+      // padded chroma sample (xx, yy), within 2 of [cx0, cx1) x [cy0, cy1)
+      auto B = [&](int xx, int yy) -> int { return sc[(yy - scoy) * LD::SCW + (xx - scox)]; };
+      int out = B(x, y);
       if (in_grid) {
         const int alt = (int)((cw >> (c == 0 ? 8 : 16)) & 0xff);
         if (alt != (int)ALF_OFF8) {
-          const uint32_t* f = j.alf_cbank + alt * 6;
-          auto B = [&](int xx, int yy) -> int { return SC.rd(clampi(xx, ccx0, ccx1), clampi(yy, ccy0, ccy1)); };
+          const uint32_t* f = s.cbank + alt * 6;
           int sft, d1, d2;
           alf_t46(y - ry * ctbc, ctbc, sft, d1, d2);
-          const int c0 = B(x, y);
+          const int c0 = out;
           int sum = alf_pair(f[0], B(x, y + d2), B(x, y - d2), c0);
           sum += alf_pair(f[1], B(x + 1, y + d1), B(x - 1, y - d1), c0);
           sum += alf_pair(f[2], B(x, y + d1), B(x, y - d1), c0);
@@ -364,17 +594,16 @@ __device__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y
         }
         const int ccf = (int)((cw >> (c == 0 ? 24 : 28)) & 0xf);
         if (ccf != (int)ALF_OFF4) {
-          const int32_t* cf = j.alf_ccbank + c * ALF_CCBANK_WORDS + ccf * 7;
+          const int32_t* cf = s.ccbank + c * ALF_CCBANK_WORDS + ccf * 7;
           const int xl = 2 * x, yl = 2 * y;
           const int y_ctb_c = ry * ctbc;
           const bool lbb = !(yl + ctb >= H && (H - y_ctb_c * 2) <= ctb - 4);
           int yp1, yp2;
           alf_t47((y - y_ctb_c) * 2, ctb, lbb, yp1, yp2);
-          auto Lm = [&](int xx, int yy) -> int { return S.rd(clampi(xx, rx0, rx1), clampi(yy, ry0, ry1)); };
-          const int cen = Lm(xl, yl);
-          int sum = cf[0] * (Lm(xl, yl - yp1) - cen) + cf[1] * (Lm(xl - 1, yl) - cen) + cf[2] * (Lm(xl + 1, yl) - cen) +
-                    cf[3] * (Lm(xl - 1, yl + yp1) - cen) + cf[4] * (Lm(xl, yl + yp1) - cen) +
-                    cf[5] * (Lm(xl + 1, yl + yp1) - cen) + cf[6] * (Lm(xl, yl + yp2) - cen);
+          const int cen = Sl(xl, yl);
+          int sum = cf[0] * (Sl(xl, yl - yp1) - cen) + cf[1] * (Sl(xl - 1, yl) - cen) + cf[2] * (Sl(xl + 1, yl) - cen) +
+                    cf[3] * (Sl(xl - 1, yl + yp1) - cen) + cf[4] * (Sl(xl, yl + yp1) - cen) +
+                    cf[5] * (Sl(xl + 1, yl + yp1) - cen) + cf[6] * (Sl(xl, yl + yp2) - cen);
           out = clampi(out + clampi((sum + 64) >> 7, -half, half - 1), 0, amax);
         }
       }
@@ -387,7 +616,7 @@ __device__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y
 // Persistent launch: the tile list is cut into 8 contiguous chunks, one per
 // XCD (workgroups are dispatched round-robin over the 8 XCDs), so spatially
 // adjacent tiles - which share halo samples - hit the same L2.
-template <int TW, int TH>
+template <int TW, int TH, bool HBD>
 __global__ void __launch_bounds__(256) k_fused(const dev_job* __restrict__ jobs, const tile_ref* __restrict__ tiles,
                                                uint32_t n_tiles) {
   __shared__ fused_lds<TW, TH> s;
@@ -405,7 +634,7 @@ __global__ void __launch_bounds__(256) k_fused(const dev_job* __restrict__ jobs,
       cur_job = (int)tr.job;
       __syncthreads();
     }
-    fused_tile<TW, TH>(j, s, tr.x0, tr.y0);
+    fused_tile<TW, TH, HBD>(j, s, tr.x0, tr.y0);
   }
 }
 
@@ -418,28 +647,38 @@ size_t fused_lds_bytes(int tw, int th) {
   return 0;
 }
 
-int fused_max_blocks_per_cu(int tw, int th) {
+namespace {
+template <int TW, int TH>
+int occupancy_of(bool hbd) {
   int n = 0;
-  hipError_t e = hipErrorInvalidValue;
-  if (tw == 64 && th == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<64, 64>, 256, 0);
-  else if (tw == 128 && th == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<128, 64>, 256, 0);
-  else if (tw == 64 && th == 32) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<64, 32>, 256, 0);
-  else if (tw == 32 && th == 32) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<32, 32>, 256, 0);
+  hipError_t e = hbd ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<TW, TH, true>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<TW, TH, false>, 256, 0);
   return e == hipSuccess ? n : 0;
 }
+template <int TW, int TH>
+void launch_shape(bool hbd, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles, uint32_t grid,
+                  hipStream_t s) {
+  if (hbd) hipLaunchKernelGGL((k_fused<TW, TH, true>), dim3(grid), dim3(256), 0, s, jobs, tiles, n_tiles);
+  else hipLaunchKernelGGL((k_fused<TW, TH, false>), dim3(grid), dim3(256), 0, s, jobs, tiles, n_tiles);
+}
+}  // namespace
 
-hipError_t launch_fused(int tw, int th, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles, uint32_t grid,
-                        hipStream_t s) {
+int fused_max_blocks_per_cu(int tw, int th, bool hbd) {
+  if (tw == 64 && th == 64) return occupancy_of<64, 64>(hbd);
+  if (tw == 128 && th == 64) return occupancy_of<128, 64>(hbd);
+  if (tw == 64 && th == 32) return occupancy_of<64, 32>(hbd);
+  if (tw == 32 && th == 32) return occupancy_of<32, 32>(hbd);
+  return 0;
+}
+
+hipError_t launch_fused(int tw, int th, bool hbd, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles,
+                        uint32_t grid, hipStream_t s) {
   grid = (grid + 7) & ~7u;
-  if (tw == 64 && th == 64) hipLaunchKernelGGL((k_fused<64, 64>), dim3(grid), dim3(256), 0, s, jobs, tiles, n_tiles);
-  else if (tw == 128 && th == 64)
-    hipLaunchKernelGGL((k_fused<128, 64>), dim3(grid), dim3(256), 0, s, jobs, tiles, n_tiles);
-  else if (tw == 64 && th == 32)
-    hipLaunchKernelGGL((k_fused<64, 32>), dim3(grid), dim3(256), 0, s, jobs, tiles, n_tiles);
-  else if (tw == 32 && th == 32)
-    hipLaunchKernelGGL((k_fused<32, 32>), dim3(grid), dim3(256), 0, s, jobs, tiles, n_tiles);
-  else
-    return hipErrorInvalidValue;
+  if (tw == 64 && th == 64) launch_shape<64, 64>(hbd, jobs, tiles, n_tiles, grid, s);
+  else if (tw == 128 && th == 64) launch_shape<128, 64>(hbd, jobs, tiles, n_tiles, grid, s);
+  else if (tw == 64 && th == 32) launch_shape<64, 32>(hbd, jobs, tiles, n_tiles, grid, s);
+  else if (tw == 32 && th == 32) launch_shape<32, 32>(hbd, jobs, tiles, n_tiles, grid, s);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

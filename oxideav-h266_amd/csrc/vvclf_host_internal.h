@@ -62,9 +62,12 @@ struct job_opts {
 
 int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvclf_job** out, uint64_t* dry_stats,
                       const job_opts* opts = nullptr, host_tables* keep = nullptr);
-hipError_t launch_fused(int tw, int th, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles, uint32_t grid,
-                        hipStream_t s);
-int fused_max_blocks_per_cu(int tw, int th);
+// hbd selects the fused kernel variant for ALF bit depths above 12 (the
+// packed 16-bit ALF arithmetic covers <= 12).
+hipError_t launch_fused(int tw, int th, bool hbd, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles,
+                        uint32_t grid, hipStream_t s);
+int fused_max_blocks_per_cu(int tw, int th, bool hbd);
+inline bool job_needs_hbd(const dev_job& d) { return d.do_alf && d.alf_bd > 12; }
 
 hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev = nullptr);
 

@@ -245,6 +245,7 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
 // A workgroup covers one CTB-aligned tile and stages that CTB's permuted
 // filter bank in LDS.
 // ---------------------------------------------------------------------------
+template <bool HBD>
 __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
   __shared__ uint32_t bank[ALF_BANK_WORDS];
   const int tile = 1 << tile_log2, tb = tile >> 2;
@@ -279,24 +280,67 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
   uint64_t r = j.alf_rect[ci];
   const int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff) - 1,
             cy1 = (int)(r >> 48) - 1;
-  // 10x10 window, rect-clamped: W[r][c] = sample(x4-3+c, y4-3+r)
-  int W[10][10];
-#pragma unroll
-  for (int rr = 0; rr < 10; rr++) {
-    int yy = clampi(y4 - 3 + rr, cy0, cy1);
-    const uint16_t* row = src.p + (size_t)yy * src.pitch;
-#pragma unroll
-    for (int cc = 0; cc < 10; cc++) W[rr][cc] = row[clampi(x4 - 3 + cc, cx0, cx1)];
-  }
   const int y_ctb = ry * ctb;
   const bool carve = !(y_ctb + ctb >= j.h) || (j.h - y_ctb) > ctb - 4;
-  int outv[4][4];
-  alf_block(W, bank, y4 - y_ctb, ctb, carve, j.alf_bd, (1 << j.alf_bd) - 1, outv);
+  const int amax = (1 << j.alf_bd) - 1;
+  uint32_t outp[4][2];
+  if (HBD) {
+    // 10x10 window, rect-clamped: W[r][c] = sample(x4-3+c, y4-3+r)
+    int W[10][10];
 #pragma unroll
-  for (int rr = 0; rr < 4; rr++)
+    for (int rr = 0; rr < 10; rr++) {
+      int yy = clampi(y4 - 3 + rr, cy0, cy1);
+      const uint16_t* row = src.p + (size_t)yy * src.pitch;
 #pragma unroll
-    for (int c = 0; c < 4; c++)
-      if (y4 + rr < src.h && x4 + c < src.w) dst.p[(size_t)(y4 + rr) * dst.pitch + x4 + c] = (uint16_t)outv[rr][c];
+      for (int cc = 0; cc < 10; cc++) W[rr][cc] = row[clampi(x4 - 3 + cc, cx0, cx1)];
+    }
+    int outv[4][4];
+    alf_block(W, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outv);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      outp[rr][0] = (uint32_t)outv[rr][0] | ((uint32_t)outv[rr][1] << 16);
+      outp[rr][1] = (uint32_t)outv[rr][2] | ((uint32_t)outv[rr][3] << 16);
+    }
+  } else {
+    // packed window P[r][k] = samples (x4-4+2k, x4-3+2k) of row y4-3+r
+    uint32_t P[10][6];
+    if (x4 - 3 >= cx0 && x4 + 6 <= cx1 && y4 - 3 >= cy0 && y4 + 6 <= cy1) {
+      // inside the rectangle: three aligned 8-byte reads per row (x4 - 4 is
+      // a multiple of 4 and plane pitches are multiples of 64)
+#pragma unroll
+      for (int rr = 0; rr < 10; rr++) {
+        const uint2* q = reinterpret_cast<const uint2*>(src.p + (size_t)(y4 - 3 + rr) * src.pitch + x4 - 4);
+        const uint2 a = q[0], b = q[1], c = q[2];
+        P[rr][0] = a.x; P[rr][1] = a.y; P[rr][2] = b.x; P[rr][3] = b.y; P[rr][4] = c.x; P[rr][5] = c.y;
+      }
+    } else {
+#pragma unroll
+      for (int rr = 0; rr < 10; rr++) {
+        const uint16_t* row = src.p + (size_t)clampi(y4 - 3 + rr, cy0, cy1) * src.pitch;
+        uint32_t v[12];
+        v[0] = 0;
+        v[11] = 0;
+#pragma unroll
+        for (int q = 1; q <= 10; q++) v[q] = row[clampi(x4 - 4 + q, cx0, cx1)];
+#pragma unroll
+        for (int k = 0; k < 6; k++) P[rr][k] = v[2 * k] | (v[2 * k + 1] << 16);
+      }
+    }
+    alf_block_pk(P, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outp);
+  }
+  if (x4 + 4 <= src.w) {
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+      if (y4 + rr < src.h)
+        *reinterpret_cast<uint2*>(dst.p + (size_t)(y4 + rr) * dst.pitch + x4) = make_uint2(outp[rr][0], outp[rr][1]);
+  } else {
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (y4 + rr < src.h && x4 + c < src.w)
+          dst.p[(size_t)(y4 + rr) * dst.pitch + x4 + c] = (uint16_t)(outp[rr][c >> 1] >> (16 * (c & 1)));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -394,8 +438,12 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
     int tl = j.ctb_log2 < 6 ? j.ctb_log2 : 6;
     int tile = 1 << tl;
     unsigned threads = (unsigned)((tile / 4) * (tile / 4));
-    hipLaunchKernelGGL(k_alf_luma, dim3(cdiv(j.w, tile), cdiv(j.h, tile)), dim3(threads < 64 ? 64 : threads), 0, s, j,
-                       tl);
+    if (j.alf_bd > 12)
+      hipLaunchKernelGGL(k_alf_luma<true>, dim3(cdiv(j.w, tile), cdiv(j.h, tile)), dim3(threads < 64 ? 64 : threads), 0,
+                         s, j, tl);
+    else
+      hipLaunchKernelGGL(k_alf_luma<false>, dim3(cdiv(j.w, tile), cdiv(j.h, tile)), dim3(threads < 64 ? 64 : threads),
+                         0, s, j, tl);
   }
   mark(7);
   if (j.do_alf && ncomp == 3)

@@ -303,8 +303,11 @@ void launch_loop(vvclf_server* s) {
   hipDeviceProp_t p;
   if (hipGetDeviceProperties(&p, s->ctx->device) == hipSuccess) cus = p.multiProcessorCount;
   const int tw = (int)s->cfg.tune.tile_w, th = (int)s->cfg.tune.tile_h;
-  int occ = std::max(1, fused_max_blocks_per_cu(tw, th));
-  uint32_t wpc = s->cfg.tune.wgs_per_cu ? std::min<uint32_t>(s->cfg.tune.wgs_per_cu, occ) : (uint32_t)occ;
+  uint32_t wpc_of[2];
+  for (int h = 0; h < 2; h++) {
+    int occ = std::max(1, fused_max_blocks_per_cu(tw, th, h != 0));
+    wpc_of[h] = s->cfg.tune.wgs_per_cu ? std::min<uint32_t>(s->cfg.tune.wgs_per_cu, occ) : (uint32_t)occ;
+  }
   for (;;) {
     inflight fl;
     {
@@ -326,7 +329,12 @@ void launch_loop(vvclf_server* s) {
       return (uint64_t)fl.reqs[a]->job->w * fl.reqs[a]->job->h > (uint64_t)fl.reqs[b]->job->w * fl.reqs[b]->job->h;
     });
     uint32_t nt = 0;
-    for (auto* r : fl.reqs) nt += ((r->job->w + tw - 1) / tw) * ((r->job->h + th - 1) / th);
+    bool hbd = false;
+    for (auto* r : fl.reqs) {
+      nt += ((r->job->w + tw - 1) / tw) * ((r->job->h + th - 1) / th);
+      hbd = hbd || job_needs_hbd(r->job->d);
+    }
+    const uint32_t wpc = wpc_of[hbd ? 1 : 0];
     launch_slot& sl = s->slots[fl.slot];
     hipError_t e = grow_slot(sl, (uint32_t)fl.reqs.size(), nt) ? hipSuccess : hipErrorOutOfMemory;
     if (e == hipSuccess) {
@@ -344,7 +352,7 @@ void launch_loop(vvclf_server* s) {
       if (e == hipSuccess)
         e = hipMemcpyAsync(sl.d_tiles, sl.h_tiles, nt * sizeof(tile_ref), hipMemcpyHostToDevice, s->s_comp);
       uint32_t grid = std::min((uint32_t)cus * wpc, (nt + 7) & ~7u);
-      if (e == hipSuccess) e = launch_fused(tw, th, sl.d_jobs, sl.d_tiles, nt, std::max(8u, grid), s->s_comp);
+      if (e == hipSuccess) e = launch_fused(tw, th, hbd, sl.d_jobs, sl.d_tiles, nt, std::max(8u, grid), s->s_comp);
       if (e == hipSuccess) e = hipEventRecord(sl.done, s->s_comp);
     }
     std::lock_guard<std::mutex> g(s->mu);

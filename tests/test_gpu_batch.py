@@ -31,8 +31,8 @@ def make(vv, orc, cases):
     return gpu, ref
 
 
-def check_all(vv, jobs, gpu, ref, what):
-    for job, fg, fo, case in zip(jobs, gpu, ref, MIX):
+def check_all(vv, jobs, gpu, ref, what, cases=MIX):
+    for job, fg, fo, case in zip(jobs, gpu, ref, cases):
         job.download(C.byref(fg.desc.contents.pic))
         for name, a, b in zip(("luma", "cb", "cr"), fo.output_arrays(), fg.output_arrays()):
             if not np.array_equal(a, b):
@@ -53,6 +53,31 @@ def test_batch_matches_oracle(vv, orc, tw, th):
     vv.check(vv.lib.vvclf_batch_run(b, ctx.stream))
     ctx.synchronize()
     check_all(vv, jobs, gpu, ref, f"batch {tw}x{th}")
+    vv.lib.vvclf_batch_destroy(b)
+    for j in jobs:
+        j.close()
+    ctx.close()
+
+
+# A batch holding a job above 12 bits runs the fused kernel's 32-bit ALF
+# variant for all of its jobs.
+HBD_MIX = [(1, 416, 240, 21, {"bit_depth": 16}), (0, 256, 128, 22, {"bit_depth": 14, "rich": 1}),
+           (1, 208, 120, 23, {}), (3, 416, 240, 24, {"tiles": 1, "bit_depth": 16})]
+
+
+@pytest.mark.parametrize("tw,th", [(64, 64), (32, 32)])
+def test_batch_high_bit_depth(vv, orc, tw, th):
+    say(f"batch >12-bit tile {tw}x{th}")
+    gpu, ref = make(vv, orc, HBD_MIX)
+    ctx = vv.Context()
+    jobs = [vv.Job(ctx, f.desc) for f in gpu]
+    arr = (C.c_void_p * len(jobs))(*[j.h for j in jobs])
+    tune = vv.Tune(tw, th, 256, 0)
+    b = C.c_void_p()
+    vv.check(vv.lib.vvclf_batch_create(ctx.h, arr, len(jobs), C.byref(tune), C.byref(b)))
+    vv.check(vv.lib.vvclf_batch_run(b, ctx.stream))
+    ctx.synchronize()
+    check_all(vv, jobs, gpu, ref, f"batch >12-bit {tw}x{th}", HBD_MIX)
     vv.lib.vvclf_batch_destroy(b)
     for j in jobs:
         j.close()

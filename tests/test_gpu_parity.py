@@ -46,7 +46,9 @@ SMALL = [(0, 416, 240, s, {}) for s in range(3)] + \
         [(1, 416, 240, 5, {"ctb_log2": 6}), (1, 416, 240, 6, {"ctb_log2": 5, "rich": 1}),
          (1, 416, 240, 7, {"bit_depth": 8}), (1, 416, 240, 8, {"bit_depth": 12, "rich": 1}),
          (1, 416, 240, 9, {"sao": 0}), (1, 416, 240, 10, {"alf": 0}), (1, 416, 240, 11, {"lmcs": 0, "sao": 0}),
-         (3, 416, 240, 12, {"tiles": 1, "rich": 1})]
+         (3, 416, 240, 12, {"tiles": 1, "rich": 1}),
+         # above 12 bits the ALF runs in 32-bit lanes instead of packed 16-bit pairs
+         (1, 416, 240, 14, {"bit_depth": 14, "rich": 1}), (0, 256, 128, 15, {"bit_depth": 16, "tiles": 1})]
 
 
 @pytest.mark.parametrize("cfg,w,h,seed,kw", SMALL)
