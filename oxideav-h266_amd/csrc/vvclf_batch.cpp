@@ -30,22 +30,15 @@ struct vvclf_batch {
 
 namespace {
 
-const int kShapes[][2] = {{64, 64}, {128, 64}, {64, 32}, {32, 32}};
-
 bool shape_supported(uint32_t tw, uint32_t th) {
-  for (auto& s : kShapes)
+  for (auto& s : kFusedShapes)
     if ((uint32_t)s[0] == tw && (uint32_t)s[1] == th) return true;
   return false;
 }
 
-// A tile must not straddle an ALF CTB (the fused kernel stages one CTB's
-// filter bank per tile).
 bool shape_fits(const std::vector<vvclf_job*>& jobs, uint32_t tw, uint32_t th) {
-  for (auto* j : jobs) {
-    if (!j->d.do_alf) continue;
-    uint32_t ctb = 1u << j->d.ctb_log2;
-    if (tw > ctb || th > ctb || ctb % tw || ctb % th) return false;
-  }
+  for (auto* j : jobs)
+    if (!job_fits_shape(j->d, tw, th)) return false;
   return true;
 }
 
@@ -77,7 +70,7 @@ int vvclf_batch_create(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, 
   }
   if (!shape_supported(t.tile_w, t.tile_h)) return fail(VVCLF_ERR_INVALID, "batch: tile %ux%u", t.tile_w, t.tile_h);
   if (!shape_fits(js, t.tile_w, t.tile_h))
-    return fail(VVCLF_ERR_INVALID, "batch: tile %ux%u straddles an ALF CTB", t.tile_w, t.tile_h);
+    return fail(VVCLF_ERR_INVALID, "batch: tile %ux%u does not fit the jobs' ALF / SAO CTBs", t.tile_w, t.tile_h);
   hipSetDevice(ctx->device);
   bool hbd = false;
   for (auto* j : js) hbd = hbd || job_needs_hbd(j->d);
@@ -212,7 +205,7 @@ int vvclf_autotune(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, vvcl
   vvclf_tune bt{};
   bool hbd = false;
   for (auto* j : js) hbd = hbd || job_needs_hbd(j->d);
-  for (auto& sh : kShapes) {
+  for (auto& sh : kFusedShapes) {
     if (!shape_fits(js, sh[0], sh[1])) continue;
     int occ = fused_max_blocks_per_cu(sh[0], sh[1], hbd);
     for (int w = 1; w <= occ; w++) {

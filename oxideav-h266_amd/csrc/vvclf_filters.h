@@ -62,6 +62,28 @@ __device__ __forceinline__ luma_terms luma_line_terms(const int* p, const int* q
   return t;
 }
 
+// The terms of line L (0 or 3) of the lane's segment, for segments held by
+// aligned quads of lanes (DPP quad_perm [L,L,L,L]; all four lanes of a quad
+// are active together).
+template <int L>
+__device__ __forceinline__ luma_terms quad_terms(const luma_terms& t) {
+  constexpr int ctrl = L * 0x55;
+  luma_terms r;
+  r.dp = __builtin_amdgcn_update_dpp(0, t.dp, ctrl, 0xf, 0xf, false);
+  r.dq = __builtin_amdgcn_update_dpp(0, t.dq, ctrl, 0xf, 0xf, false);
+  r.sp = __builtin_amdgcn_update_dpp(0, t.sp, ctrl, 0xf, 0xf, false);
+  r.sq = __builtin_amdgcn_update_dpp(0, t.sq, ctrl, 0xf, 0xf, false);
+  r.spq = __builtin_amdgcn_update_dpp(0, t.spq, ctrl, 0xf, 0xf, false);
+  r.dpl = __builtin_amdgcn_update_dpp(0, t.dpl, ctrl, 0xf, 0xf, false);
+  r.dql = __builtin_amdgcn_update_dpp(0, t.dql, ctrl, 0xf, 0xf, false);
+  r.sp7 = __builtin_amdgcn_update_dpp(0, t.sp7, ctrl, 0xf, 0xf, false);
+  r.sq7 = __builtin_amdgcn_update_dpp(0, t.sq7, ctrl, 0xf, 0xf, false);
+  r.spm = __builtin_amdgcn_update_dpp(0, t.spm, ctrl, 0xf, 0xf, false);
+  r.sqm = __builtin_amdgcn_update_dpp(0, t.sqm, ctrl, 0xf, 0xf, false);
+  return r;
+}
+
+
 struct luma_decision {
   int dE, dEp, dEq, mflp, mflq;
 };

@@ -32,6 +32,8 @@ struct fused_lds {
   static constexpr int MCH_W = TW / 4 + 5, MCH_H = TH / 16 + 3; // chroma horizontal edges
   struct maps_t {
     uint32_t lv[MLV_H * MLV_W], lh[MLH_H * MLH_W], cv[2][MCV_H * MCV_W], ch[2][MCH_H * MCH_W];
+    // lists of the nonzero descriptors: (comp << 15) | (row << 6) | column
+    uint16_t llv[MLV_H * MLV_W], llh[MLH_H * MLH_W], lcv[2 * MCV_H * MCV_W], lch[2 * MCH_H * MCH_W];
   };
   struct sao_out_t {
     uint16_t S[SH * SW];
@@ -49,8 +51,26 @@ struct fused_lds {
   int32_t ccbank[2 * ALF_CCBANK_WORDS];
   uint64_t saop[9][3];  // SAO parameters of the (up to) 3x3 CTBs around the tile
   uint64_t saor[9];     // and their sample rectangles
+  uint64_t safe[9][3];  // per CTB and component: [x0, x1) x [y0, y1) whose 8-neighbourhoods lie
+                        // inside the picture and the CTB's rectangle (u16 fields)
+  uint32_t nlist[4];    // entries in llv, llh, lcv, lch
   uint16_t lut[1024];
+  static_assert(MLV_W <= 64 && MLH_W <= 64 && MCV_W <= 64 && MCH_W <= 64, "list column field");
+  static_assert(sizeof(maps_t) <= sizeof(sao_out_t), "maps alias the SAO output");
 };
+
+// Appends `val` for every lane with `pred` to an LDS list (one LDS atomic
+// per wave).  Called by all active lanes of the wave.
+__device__ __forceinline__ void list_append(bool pred, uint32_t val, uint16_t* list, uint32_t* cnt) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(cnt, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + below] = (uint16_t)val;
+}
 
 // A rectangle of an edge-descriptor map (map units), staged in LDS with
 // origin (x0, y0).
@@ -75,15 +95,20 @@ struct rect_stage {
       }
     }
   }
-  __device__ __forceinline__ void commit(uint32_t* dst, int pitch, const mrect& r) const {
+  // also lists the nonzero words as tag | (row << 6) | column
+  __device__ __forceinline__ void commit(uint32_t* dst, int pitch, const mrect& r, uint16_t* list, uint32_t* cnt,
+                                         uint32_t tag) const {
     const int n = r.nx > 0 && r.ny > 0 ? r.nx * r.ny : 0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       const int i = (int)threadIdx.x + k * 256;
+      int yy = 0, xx = 0;
       if (i < n) {
-        const int yy = i / r.nx, xx = i - yy * r.nx;
+        yy = i / r.nx;
+        xx = i - yy * r.nx;
         dst[yy * pitch + xx] = v[k];
       }
+      list_append(i < n && v[k] != 0, tag | (uint32_t)(yy << 6) | (uint32_t)xx, list, cnt);
     }
   }
 };
@@ -109,63 +134,73 @@ struct lwin {
 __device__ __forceinline__ int fdiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 __device__ __forceinline__ int cdiv_i(int a, int b) { return -fdiv(-a, b); }
 
-__device__ __forceinline__ luma_terms shfl_terms4(const luma_terms& t, int src) {
-  luma_terms r;
-  r.dp = __shfl(t.dp, src); r.dq = __shfl(t.dq, src); r.sp = __shfl(t.sp, src);
-  r.sq = __shfl(t.sq, src); r.spq = __shfl(t.spq, src); r.dpl = __shfl(t.dpl, src);
-  r.dql = __shfl(t.dql, src); r.sp7 = __shfl(t.sp7, src); r.sq7 = __shfl(t.sq7, src);
-  r.spm = __shfl(t.spm, src); r.sqm = __shfl(t.sqm, src);
-  return r;
-}
-
 // Luma deblocking of one direction over a grid of segments (in place in LDS).
 // VERT: edge columns gx in [gx0, gx1], segment rows gy in [gy0, gy1].
 // HOR : edge rows gy in [gy0, gy1], segment columns gx in [gx0, gx1].
-template <bool VERT>
+template <bool VERT, int PITCH>
 __device__ __forceinline__ void lds_deblock_luma(const dev_job& j, lwin& L, const uint32_t* map, int mpitch, const mrect& mr,
-                                 int maxv) {
+                                 const uint16_t* list, int nlist, int maxv) {
   const int tid = threadIdx.x;
-  const int ncol = mr.nx, nrow = mr.ny;
-  const int nseg = ncol * nrow;
-  const int lane = tid & 63;
-  for (int base = 0; base < nseg; base += blockDim.x / 4) {
-    const int seg = base + (tid >> 2), line = tid & 3;
+  for (int base = 0; base < nlist; base += blockDim.x / 4) {
+    const int it = base + (tid >> 2), line = tid & 3;
     uint32_t e = 0;
     int gx = 0, gy = 0;
-    if (seg < nseg) {
-      const int ry = seg / ncol, rx = seg - ry * ncol;
+    if (it < nlist) {
+      const int w = list[it], ry = w >> 6, rx = w & 63;
       gx = mr.x0 + rx;
       gy = mr.y0 + ry;
       e = map[ry * mpitch + rx];
     }
     if (e != 0) {  // uniform across the quad
       const int x0 = VERT ? 4 * gx : 4 * gx + line, y0 = VERT ? 4 * gy + line : 4 * gy;
+      // The line's 16 samples lie inside the window unless the edge is
+      // within 8 samples of a picture border (where the window is clipped):
+      // then reads clamp and writes outside the window are dropped.
+      constexpr int STEP = VERT ? 1 : PITCH;
+      const bool inwin = VERT ? (x0 - 8 >= L.ox && x0 + 8 <= L.ox + L.w && y0 < L.oy + L.h)
+                              : (y0 - 8 >= L.oy && y0 + 8 <= L.oy + L.h && x0 < L.ox + L.w);
+      uint16_t* c0 = L.p + (y0 - L.oy) * PITCH + (x0 - L.ox);  // q0
       int p[8], q[8];
+      if (inwin) {
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        p[i] = VERT ? L.rd(x0 - 1 - i, y0) : L.rd(x0, y0 - 1 - i);
-        q[i] = VERT ? L.rd(x0 + i, y0) : L.rd(x0, y0 + i);
+        for (int i = 0; i < 8; i++) {
+          p[i] = c0[-(i + 1) * STEP];
+          q[i] = c0[i * STEP];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          p[i] = VERT ? L.rd(x0 - 1 - i, y0) : L.rd(x0, y0 - 1 - i);
+          q[i] = VERT ? L.rd(x0 + i, y0) : L.rd(x0, y0 + i);
+        }
       }
       int beta, tc;
       edge_thresholds(e, j.bd_dbk, beta, tc);
       int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
       luma_terms mine = luma_line_terms(p, q, mflp, mflq);
-      int qb = lane & ~3;
-      luma_terms t0 = shfl_terms4(mine, qb), t3 = shfl_terms4(mine, qb + 3);
+      luma_terms t0 = quad_terms<0>(mine), t3 = quad_terms<3>(mine);
       luma_decision d = luma_decide(t0, t3, beta, tc, mflp, mflq, (e & E_CTBROW_BIT) != 0);
       int np[8], nq[8], nwp, nwq;
       luma_filter_line(p, q, d, tc, np, nq, nwp, nwq);
       if (e & E_PLTP_BIT) nwp = 0;
       if (e & E_PLTQ_BIT) nwq = 0;
+      if (inwin) {
 #pragma unroll
-      for (int i = 0; i < 7; i++) {
-        if (i < nwp) {
-          int x = VERT ? x0 - 1 - i : x0, y = VERT ? y0 : y0 - 1 - i;
-          if (L.inside(x, y)) L.wr(x, y, clampi(np[i], 0, maxv));
+        for (int i = 0; i < 7; i++) {
+          if (i < nwp) c0[-(i + 1) * STEP] = (uint16_t)clampi(np[i], 0, maxv);
+          if (i < nwq) c0[i * STEP] = (uint16_t)clampi(nq[i], 0, maxv);
         }
-        if (i < nwq) {
-          int x = VERT ? x0 + i : x0, y = VERT ? y0 : y0 + i;
-          if (L.inside(x, y)) L.wr(x, y, clampi(nq[i], 0, maxv));
+      } else {
+#pragma unroll
+        for (int i = 0; i < 7; i++) {
+          if (i < nwp) {
+            int x = VERT ? x0 - 1 - i : x0, y = VERT ? y0 : y0 - 1 - i;
+            if (L.inside(x, y)) L.wr(x, y, clampi(np[i], 0, maxv));
+          }
+          if (i < nwq) {
+            int x = VERT ? x0 + i : x0, y = VERT ? y0 : y0 + i;
+            if (L.inside(x, y)) L.wr(x, y, clampi(nq[i], 0, maxv));
+          }
         }
       }
     }
@@ -175,19 +210,17 @@ __device__ __forceinline__ void lds_deblock_luma(const dev_job& j, lwin& L, cons
 // Chroma deblocking of one direction (both components) in LDS.
 // VERT: edge columns sx in [sx0, sx1] (x = 8 sx), segment rows sy in [sy0, sy1] (y = 2 sy).
 // HOR : edge rows sy in [sy0, sy1] (y = 8 sy), segment columns sx in [sx0, sx1] (x = 2 sx).
-template <bool VERT>
+template <bool VERT, int PITCH>
 __device__ __forceinline__ void lds_deblock_chroma(lwin cb, uint16_t* cr_p, const uint32_t* map0, const uint32_t* map1, int mpitch,
-                                   const mrect& mr, int bd_dbk, int maxv) {
+                                   const mrect& mr, const uint16_t* list, int nlist, int bd_dbk, int maxv) {
   const int tid = threadIdx.x;
-  const int ncol = mr.nx, nrow = mr.ny;
-  const int nseg = ncol * nrow;
-  for (int base = 0; base < 2 * nseg; base += blockDim.x / 2) {
-    const int item = base + (tid >> 1), line = tid & 1;
+  for (int base = 0; base < nlist; base += blockDim.x / 2) {
+    const int it = base + (tid >> 1), line = tid & 1;
     uint32_t e = 0;
     int sx = 0, sy = 0, comp = 0;
-    if (item < 2 * nseg) {
-      comp = item >= nseg ? 1 : 0;
-      const int seg = item - comp * nseg, ry = seg / ncol, rx = seg - ry * ncol;
+    if (it < nlist) {
+      const int w = list[it], ry = (w >> 6) & 511, rx = w & 63;
+      comp = w >> 15;
       sx = mr.x0 + rx;
       sy = mr.y0 + ry;
       e = (comp ? map1 : map0)[ry * mpitch + rx];
@@ -197,9 +230,18 @@ __device__ __forceinline__ void lds_deblock_chroma(lwin cb, uint16_t* cr_p, cons
       if (comp) L.p = cr_p;
       int x0, y0;
       if (VERT) { x0 = 8 * sx; y0 = 2 * sy + line; } else { x0 = 2 * sx + line; y0 = 8 * sy; }
+      constexpr int STEP = VERT ? 1 : PITCH;
+      const bool inwin = VERT ? (x0 - 4 >= L.ox && x0 + 4 <= L.ox + L.w && y0 < L.oy + L.h)
+                              : (y0 - 4 >= L.oy && y0 + 4 <= L.oy + L.h && x0 < L.ox + L.w);
+      uint16_t* c0 = L.p + (y0 - L.oy) * PITCH + (x0 - L.ox);  // q0
       int c[8];
+      if (inwin) {
 #pragma unroll
-      for (int i = 0; i < 8; i++) c[i] = VERT ? L.rd(x0 - 4 + i, y0) : L.rd(x0, y0 - 4 + i);
+        for (int i = 0; i < 8; i++) c[i] = c0[(i - 4) * STEP];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) c[i] = VERT ? L.rd(x0 - 4 + i, y0) : L.rd(x0, y0 - 4 + i);
+      }
       int beta, tc;
       edge_thresholds(e, bd_dbk, beta, tc);
       int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
@@ -212,53 +254,70 @@ __device__ __forceinline__ void lds_deblock_chroma(lwin cb, uint16_t* cr_p, cons
       uint32_t wm = chroma_filter_line(c, eligible && mine && other, short_p, tc, n);
       if (e & E_PLTP_BIT) wm &= 0xf0u;
       if (e & E_PLTQ_BIT) wm &= 0x0fu;
+      if (inwin) {
 #pragma unroll
-      for (int i = 1; i < 7; i++) {
-        if (!((wm >> i) & 1)) continue;
-        int x = VERT ? x0 - 4 + i : x0, y = VERT ? y0 : y0 - 4 + i;
-        if (L.inside(x, y)) L.wr(x, y, clampi(n[i], 0, maxv));
+        for (int i = 1; i < 7; i++)
+          if ((wm >> i) & 1) c0[(i - 4) * STEP] = (uint16_t)clampi(n[i], 0, maxv);
+      } else {
+#pragma unroll
+        for (int i = 1; i < 7; i++) {
+          if (!((wm >> i) & 1)) continue;
+          int x = VERT ? x0 - 4 + i : x0, y = VERT ? y0 : y0 - 4 + i;
+          if (L.inside(x, y)) L.wr(x, y, clampi(n[i], 0, maxv));
+        }
       }
     }
   }
 }
 
 // SAO parameters of the CTBs around a tile: CTB columns [bx, bx + 3) and
-// rows [by, by + 3) are staged in LDS when they cover the tile's SAO reach
-// (`ok`); otherwise (SAO CTBs smaller than a third of the reach) the
-// parameters are read from the job's tables.
+// rows [by, by + 3), staged in LDS.  Tile shapes are chosen so that these
+// cover the tile's SAO reach (tile no larger than the SAO CTB); cells outside
+// the SAO grid and components with SAO off hold type 0.
 struct sao_tab {
   int bx, by;
-  bool ok;
 };
 
 // SAO of one component from window `src` into window `dst` (dst's extent is
-// the region produced).  sub = 1 for chroma.
-__device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)[3], const uint64_t* saor, const sao_tab& tb,
-                        const lwin& src, lwin& dst, int comp, int pw, int ph) {
+// the region produced, at most RW samples wide).  sub = 1 for chroma.  The
+// region is walked flat (row pitch RW) so that the lanes stay busy; samples
+// whose 8-neighbourhood lies inside the picture and the CTB's rectangle read
+// their edge-offset neighbours directly, the others check each neighbour.
+template <int RW>
+__device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)[3], const uint64_t* saor,
+                                        const uint64_t (*safe)[3], const sao_tab& tb, const lwin& src, lwin& dst,
+                                        int comp, int pw, int ph) {
   const int sub = comp ? 1 : 0;
   const int log2c = j.sao_ctb_log2 - sub;
-  const bool active = j.do_sao && (comp == 0 ? j.sao_luma : j.sao_chroma);
   const int maxv = (1 << j.sao_bd) - 1;
-  // rows over the waves, columns over the lanes (no integer division)
-  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int yy = threadIdx.x >> 6; yy < dst.h; yy += nw)
-  for (int xx = lane; xx < dst.w; xx += 64) {
+  const bool has_rect = j.sao_rect != nullptr;
+  const int n = RW * dst.h;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int yy = i / RW, xx = i - yy * RW;
+    if (xx >= dst.w) continue;
     const int x = dst.ox + xx, y = dst.oy + yy;
-    const int pv = src.rd(x, y);
+    const int sidx = (y - src.oy) * src.pitch + (x - src.ox);  // (x, y) lies inside src
+    const int pv = src.p[sidx];
     int out = pv;
-    const int rx = x >> log2c, ry = y >> log2c;
-    if (active && rx < j.swctb && ry < j.shctb) {
-      const size_t ci = (size_t)ry * j.swctb + rx;
-      const int cell = (ry - tb.by) * 3 + (rx - tb.bx);
-      const uint64_t prm = tb.ok ? saop[cell][comp] : j.sao[ci * 3 + comp];
-      const int type = (int)(prm & 3);
-      if (type == 2) {
-        int h0, v0, h1, v1;
-        eo_offsets((int)((prm >> 2) & 31), h0, v0, h1, v1);
-        const int nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
+    const int cell = ((y >> log2c) - tb.by) * 3 + ((x >> log2c) - tb.bx);
+    const uint64_t prm = saop[cell][comp];
+    const int type = (int)(prm & 3);
+    if (type == 1) {
+      out = sao_band_sample(pv, prm, j.sao_bd, maxv);
+    } else if (type == 2) {
+      const int cls = (int)((prm >> 2) & 3);
+      // Table 44: the second neighbour mirrors the first
+      const int h0 = (cls == 0 || cls == 2) ? -1 : (cls == 3 ? 1 : 0), v0 = cls == 0 ? 0 : -1;
+      const uint64_t sr = safe[cell][comp];
+      if (x >= (int)(sr & 0xffff) && x < (int)((sr >> 16) & 0xffff) && y >= (int)((sr >> 32) & 0xffff) &&
+          y < (int)(sr >> 48)) {
+        const int d = v0 * src.pitch + h0;
+        out = sao_edge_sample(pv, src.p[sidx + d], src.p[sidx - d], prm, maxv);
+      } else {
+        const int nx0 = x + h0, ny0 = y + v0, nx1 = x - h0, ny1 = y - v0;
         bool outside = nx0 < 0 || ny0 < 0 || nx1 < 0 || ny1 < 0 || nx0 >= pw || ny0 >= ph || nx1 >= pw || ny1 >= ph;
-        if (j.sao_rect) {
-          uint64_t r = tb.ok ? saor[cell] : j.sao_rect[ci];
+        if (has_rect) {
+          const uint64_t r = saor[cell];
           int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff),
               cy1 = (int)(r >> 48);
           if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
@@ -266,12 +325,17 @@ __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)
                     nx1 >= cx1 || ny1 >= cy1;
         }
         out = outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, src.rd(nx0, ny0), src.rd(nx1, ny1), prm, maxv);
-      } else if (type == 1) {
-        out = sao_band_sample(pv, prm, j.sao_bd, maxv);
       }
     }
     dst.wr(x, y, out);
   }
+}
+
+// Window copy (SAO off for the whole picture).
+__device__ __forceinline__ void lds_copy(const lwin& src, lwin& dst) {
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int yy = threadIdx.x >> 6; yy < dst.h; yy += nw)
+    for (int xx = lane; xx < dst.w; xx += 64) dst.wr(dst.ox + xx, dst.oy + yy, src.rd(dst.ox + xx, dst.oy + yy));
 }
 
 // Replaces every sample of buf (storage origin (ox, oy), pitch `pitch`) in
@@ -334,11 +398,10 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   }
   // SAO parameter table: CTBs covering luma x in [x0-4, x1+3] (chroma reach
   // included) and likewise y
-  sao_tab tb{0, 0, false};
+  sao_tab tb{0, 0};
   if (j.do_sao) {
     tb.bx = max(0, (x0 - 4) >> j.sao_ctb_log2);
     tb.by = max(0, (y0 - 4) >> j.sao_ctb_log2);
-    tb.ok = ((x1 + 3) >> j.sao_ctb_log2) - tb.bx < 3 && ((y1 + 3) >> j.sao_ctb_log2) - tb.by < 3;
   }
   // ALF parameters of the tile's CTB
   const int ctb = 1 << j.ctb_log2;
@@ -404,12 +467,15 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     if (in_grid && tid < ALF_CBANK_WORDS) small = j.alf_cbank[tid];
     else if (in_grid && tid < ALF_CBANK_WORDS + 2 * ALF_CCBANK_WORDS)
       small = (uint32_t)j.alf_ccbank[tid - ALF_CBANK_WORDS];
+    // SAO cells: thread 128 + 4 * cell + k loads component k (k < 3) or the
+    // rectangle (k = 3); off components and cells outside the grid read 0
     uint64_t sv = 0;
     const int scell = tid - 128, scx = tb.bx + (scell >> 2) % 3, scy = tb.by + (scell >> 2) / 3, sk = scell & 3;
-    const bool sao_ld = tb.ok && scell >= 0 && scell < 36 && scx < j.swctb && scy < j.shctb;
-    if (sao_ld) {
+    const bool sao_cell = j.do_sao && scell >= 0 && scell < 36;
+    if (sao_cell && scx < j.swctb && scy < j.shctb) {
       const size_t sci = (size_t)scy * j.swctb + scx;
-      sv = sk < 3 ? j.sao[sci * 3 + sk] : (j.sao_rect ? j.sao_rect[sci] : 0);
+      const bool on = sk == 3 || (sk == 0 ? j.sao_luma : j.sao_chroma);
+      if (on) sv = sk < 3 ? j.sao[sci * 3 + sk] : (j.sao_rect ? j.sao_rect[sci] : 0);
     }
     // ---- commit to LDS
     const int lut_lds = j.bd <= 10;
@@ -440,14 +506,14 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       }
     }
     if (dbk) {
-      slv.commit(s.m.lv, LD::MLV_W, mlv);
-      slh.commit(s.m.lh, LD::MLH_W, mlh);
+      slv.commit(s.m.lv, LD::MLV_W, mlv, s.m.llv, &s.nlist[0], 0);
+      slh.commit(s.m.lh, LD::MLH_W, mlh, s.m.llh, &s.nlist[1], 0);
     }
     if (dbk_c) {
-      scv0.commit(s.m.cv[0], LD::MCV_W, mcv);
-      scv1.commit(s.m.cv[1], LD::MCV_W, mcv);
-      sch0.commit(s.m.ch[0], LD::MCH_W, mch);
-      sch1.commit(s.m.ch[1], LD::MCH_W, mch);
+      scv0.commit(s.m.cv[0], LD::MCV_W, mcv, s.m.lcv, &s.nlist[2], 0);
+      scv1.commit(s.m.cv[1], LD::MCV_W, mcv, s.m.lcv, &s.nlist[2], 1u << 15);
+      sch0.commit(s.m.ch[0], LD::MCH_W, mch, s.m.lch, &s.nlist[3], 0);
+      sch1.commit(s.m.ch[1], LD::MCH_W, mch, s.m.lch, &s.nlist[3], 1u << 15);
     }
     if (alf_on) {
 #pragma unroll
@@ -456,19 +522,38 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     }
     if (in_grid && tid < ALF_CBANK_WORDS) s.cbank[tid] = (uint32_t)small;
     else if (in_grid && tid < ALF_CBANK_WORDS + 2 * ALF_CCBANK_WORDS) s.ccbank[tid - ALF_CBANK_WORDS] = (int32_t)small;
-    if (tb.ok && scell >= 0 && scell < 36) {
-      if (sk < 3) s.saop[scell >> 2][sk] = sv;
-      else s.saor[scell >> 2] = sv;
+    if (sao_cell) {
+      if (sk < 3) {
+        s.saop[scell >> 2][sk] = sv;
+      } else {
+        s.saor[scell >> 2] = sv;
+        // rectangles inside which a sample's whole 8-neighbourhood lies in
+        // the picture and in this CTB's rectangle
+        const uint64_t rr = j.sao_rect ? sv : ((uint64_t)W << 32) | ((uint64_t)H << 48);
+        const int qx0 = (int)(rr & 0xffff), qy0 = (int)((rr >> 16) & 0xffff), qx1 = (int)((rr >> 32) & 0xffff),
+                  qy1 = (int)(rr >> 48);
+        for (int c = 0; c < 3; c++) {
+          int ax0 = qx0, ay0 = qy0, ax1 = qx1, ay1 = qy1, pw = W, ph = H;
+          if (c) { ax0 >>= 1; ay0 >>= 1; ax1 = (ax1 + 1) >> 1; ay1 = (ay1 + 1) >> 1; pw = j.wc; ph = j.hc; }
+          const uint32_t lx = (uint32_t)max(ax0, 0) + 1, hx = (uint32_t)max(min(ax1, pw) - 1, 0);
+          const uint32_t ly = (uint32_t)max(ay0, 0) + 1, hy = (uint32_t)max(min(ay1, ph) - 1, 0);
+          s.safe[scell >> 2][c] = (uint64_t)lx | ((uint64_t)hx << 16) | ((uint64_t)ly << 32) | ((uint64_t)hy << 48);
+        }
+      }
     }
   }
   __syncthreads();
   // ---- deblocking: all vertical edges, then all horizontal edges
   if (dbk) {
-    lds_deblock_luma<true>(j, L, s.m.lv, LD::MLV_W, mlv, maxv);
-    if (dbk_c) lds_deblock_chroma<true>(Cw, s.Cc[1], s.m.cv[0], s.m.cv[1], LD::MCV_W, mcv, j.bd_dbk, maxv);
+    lds_deblock_luma<true, LD::LW>(j, L, s.m.lv, LD::MLV_W, mlv, s.m.llv, (int)s.nlist[0], maxv);
+    if (dbk_c)
+      lds_deblock_chroma<true, LD::CW>(Cw, s.Cc[1], s.m.cv[0], s.m.cv[1], LD::MCV_W, mcv, s.m.lcv, (int)s.nlist[2], j.bd_dbk,
+                               maxv);
     __syncthreads();
-    lds_deblock_luma<false>(j, L, s.m.lh, LD::MLH_W, mlh, maxv);
-    if (dbk_c) lds_deblock_chroma<false>(Cw, s.Cc[1], s.m.ch[0], s.m.ch[1], LD::MCH_W, mch, j.bd_dbk, maxv);
+    lds_deblock_luma<false, LD::LW>(j, L, s.m.lh, LD::MLH_W, mlh, s.m.llh, (int)s.nlist[1], maxv);
+    if (dbk_c)
+      lds_deblock_chroma<false, LD::CW>(Cw, s.Cc[1], s.m.ch[0], s.m.ch[1], LD::MCH_W, mch, s.m.lch, (int)s.nlist[3], j.bd_dbk,
+                                maxv);
   }
   __syncthreads();  // deblocking done; the edge descriptors' LDS is reused
   // ---- SAO into the second buffers.  S stores luma [x0-4, x0+TW+4) x
@@ -480,7 +565,8 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   S.w = min(W, x1 + 3) - S.ox;
   S.h = min(H, y1 + 3) - S.oy;
   S.p = s.o.S + (S.oy - soy) * LD::SW + (S.ox - sox);
-  lds_sao(j, s.saop, s.saor, tb, L, S, 0, W, H);
+  if (j.do_sao) lds_sao<TW + 6>(j, s.saop, s.saor, s.safe, tb, L, S, 0, W, H);
+  else lds_copy(L, S);
   if (chroma) {
     for (int c = 0; c < 2; c++) {
       lwin Sc{nullptr, LD::SCW, max(0, cx0 - 2), max(0, cy0 - 2), 0, 0};
@@ -489,7 +575,8 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       Sc.p = s.o.Sc[c] + (Sc.oy - scoy) * LD::SCW + (Sc.ox - scox);
       lwin Cs = Cw;
       Cs.p = s.Cc[c];
-      lds_sao(j, s.saop, s.saor, tb, Cs, Sc, c + 1, j.wc, j.hc);
+      if (j.do_sao) lds_sao<TW / 2 + 4>(j, s.saop, s.saor, s.safe, tb, Cs, Sc, c + 1, j.wc, j.hc);
+      else lds_copy(Cs, Sc);
     }
   }
   // ---- ALF: this tile lies inside one CTB
@@ -565,51 +652,96 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       }
     }
   }
-  // chroma ALF + CC-ALF: one sample per thread
+  // chroma ALF + CC-ALF: two horizontally adjacent samples per thread.
+  // Sample x of Sc sits at an even storage column when x - cx0 is even, so
+  // pairs (x, x+1) are aligned 32-bit reads and (x-1, x) / (x+1, x+2) one
+  // alignbit each.
   if (chroma) {
-    const int cwid = cx1 - cx0, chei = cy1 - cy0, n = cwid * chei;
+    const int cwid = cx1 - cx0, chei = cy1 - cy0, pw2 = (cwid + 1) >> 1, npair = pw2 * chei;
     const int ctbc = ctb >> 1;
     const int amax = (1 << j.alf_bd) - 1, half = 1 << (j.alf_bd - 1);
-    for (int i = threadIdx.x; i < 2 * n; i += NT) {
-      const int c = i >= n ? 1 : 0, k = i - c * n;
-      const int y = cy0 + k / cwid, x = cx0 + k % cwid;
+    const s16x2 zero2 = {0, 0}, vmax2 = {(short)amax, (short)amax};
+    for (int i = threadIdx.x; i < 2 * npair; i += NT) {
+      const int c = i >= npair ? 1 : 0, k = i - c * npair;
+      const int yy = k / pw2, x = cx0 + 2 * (k - yy * pw2), y = cy0 + yy;
       const uint16_t* sc = s.o.Sc[c];
-      // padded chroma sample (xx, yy), within 2 of [cx0, cx1) x [cy0, cy1)
-      auto B = [&](int xx, int yy) -> int { return sc[(yy - scoy) * LD::SCW + (xx - scox)]; };
-      int out = B(x, y);
+      // pair of padded chroma samples (xa, xa+1), xa - cx0 even
+      auto P2 = [&](int xa, int ya) -> uint32_t {
+        return *reinterpret_cast<const uint32_t*>(&sc[(ya - scoy) * LD::SCW + (xa - scox)]);
+      };
+      uint32_t outp = P2(x, y);
       if (in_grid) {
         const int alt = (int)((cw >> (c == 0 ? 8 : 16)) & 0xff);
-        if (alt != (int)ALF_OFF8) {
+        if (alt != (int)ALF_OFF8 && !HBD) {
           const uint32_t* f = s.cbank + alt * 6;
           int sft, d1, d2;
           alf_t46(y - ry * ctbc, ctbc, sft, d1, d2);
-          const int c0 = out;
-          int sum = alf_pair(f[0], B(x, y + d2), B(x, y - d2), c0);
-          sum += alf_pair(f[1], B(x + 1, y + d1), B(x - 1, y - d1), c0);
-          sum += alf_pair(f[2], B(x, y + d1), B(x, y - d1), c0);
-          sum += alf_pair(f[3], B(x - 1, y + d1), B(x + 1, y - d1), c0);
-          sum += alf_pair(f[4], B(x + 2, y), B(x - 2, y), c0);
-          sum += alf_pair(f[5], B(x + 1, y), B(x - 1, y), c0);
-          out = clampi(c0 + ((sum + (1 << (sft - 1))) >> sft), 0, amax);
+          const uint32_t m2 = P2(x - 2, y), m0 = outp, p2 = P2(x + 2, y);
+          const uint32_t am2 = P2(x - 2, y + d1), a0 = P2(x, y + d1), ap2 = P2(x + 2, y + d1);
+          const uint32_t bm2 = P2(x - 2, y - d1), b0 = P2(x, y - d1), bp2 = P2(x + 2, y - d1);
+          const uint32_t u2 = P2(x, y + d2), l2 = P2(x, y - d2);
+          const s16x2 cur = as_s16x2(m0);
+          int a_0 = 1 << (sft - 1), a_1 = a_0;
+          auto tap = [&](int t, uint32_t a, uint32_t b) {
+            const uint32_t w = f[t], kk = w >> 16;
+            alf_pk_tap(w & 0xffffu, w << 16, kk | ((0u - kk) << 16), cur, a, b, a_0, a_1);
+          };
+          tap(0, u2, l2);
+          tap(1, __builtin_amdgcn_alignbit(ap2, a0, 16), __builtin_amdgcn_alignbit(b0, bm2, 16));
+          tap(2, a0, b0);
+          tap(3, __builtin_amdgcn_alignbit(a0, am2, 16), __builtin_amdgcn_alignbit(bp2, b0, 16));
+          tap(4, p2, m2);
+          tap(5, __builtin_amdgcn_alignbit(p2, m0, 16), __builtin_amdgcn_alignbit(m0, m2, 16));
+          const s16x2 d = __builtin_amdgcn_cvt_pk_i16(a_0 >> sft, a_1 >> sft);
+          outp = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(__builtin_elementwise_add_sat(cur, d), zero2),
+                                                   vmax2));
+        } else if (alt != (int)ALF_OFF8) {
+          // above 12 bits: 32-bit arithmetic per sample
+          const uint32_t* f = s.cbank + alt * 6;
+          int sft, d1, d2;
+          alf_t46(y - ry * ctbc, ctbc, sft, d1, d2);
+          auto B = [&](int xx, int yy2) -> int { return sc[(yy2 - scoy) * LD::SCW + (xx - scox)]; };
+          uint32_t o2[2];
+          for (int h = 0; h < 2; h++) {
+            const int xs = x + h, c0 = B(xs, y);
+            int sum = alf_pair(f[0], B(xs, y + d2), B(xs, y - d2), c0);
+            sum += alf_pair(f[1], B(xs + 1, y + d1), B(xs - 1, y - d1), c0);
+            sum += alf_pair(f[2], B(xs, y + d1), B(xs, y - d1), c0);
+            sum += alf_pair(f[3], B(xs - 1, y + d1), B(xs + 1, y - d1), c0);
+            sum += alf_pair(f[4], B(xs + 2, y), B(xs - 2, y), c0);
+            sum += alf_pair(f[5], B(xs + 1, y), B(xs - 1, y), c0);
+            o2[h] = (uint32_t)clampi(c0 + ((sum + (1 << (sft - 1))) >> sft), 0, amax);
+          }
+          outp = o2[0] | (o2[1] << 16);
         }
         const int ccf = (int)((cw >> (c == 0 ? 24 : 28)) & 0xf);
         if (ccf != (int)ALF_OFF4) {
           const int32_t* cf = s.ccbank + c * ALF_CCBANK_WORDS + ccf * 7;
-          const int xl = 2 * x, yl = 2 * y;
+          const int yl = 2 * y;
           const int y_ctb_c = ry * ctbc;
           const bool lbb = !(yl + ctb >= H && (H - y_ctb_c * 2) <= ctb - 4);
           int yp1, yp2;
           alf_t47((y - y_ctb_c) * 2, ctb, lbb, yp1, yp2);
-          const int cen = Sl(xl, yl);
-          int sum = cf[0] * (Sl(xl, yl - yp1) - cen) + cf[1] * (Sl(xl - 1, yl) - cen) + cf[2] * (Sl(xl + 1, yl) - cen) +
-                    cf[3] * (Sl(xl - 1, yl + yp1) - cen) + cf[4] * (Sl(xl, yl + yp1) - cen) +
-                    cf[5] * (Sl(xl + 1, yl + yp1) - cen) + cf[6] * (Sl(xl, yl + yp2) - cen);
-          out = clampi(out + clampi((sum + 64) >> 7, -half, half - 1), 0, amax);
+          uint32_t o2[2];
+          for (int h = 0; h < 2; h++) {
+            const int xl = 2 * (x + h);
+            const int cen = Sl(xl, yl);
+            const int sum = cf[0] * (Sl(xl, yl - yp1) - cen) + cf[1] * (Sl(xl - 1, yl) - cen) +
+                            cf[2] * (Sl(xl + 1, yl) - cen) + cf[3] * (Sl(xl - 1, yl + yp1) - cen) +
+                            cf[4] * (Sl(xl, yl + yp1) - cen) + cf[5] * (Sl(xl + 1, yl + yp1) - cen) +
+                            cf[6] * (Sl(xl, yl + yp2) - cen);
+            const int base = (int)((outp >> (16 * h)) & 0xffffu);
+            o2[h] = (uint32_t)clampi(base + clampi((sum + 64) >> 7, -half, half - 1), 0, amax);
+          }
+          outp = o2[0] | (o2[1] << 16);
         }
       }
-      (c ? j.out.cr : j.out.cb)[(size_t)y * j.pc + x] = (uint16_t)out;
+      uint16_t* dst = (c ? j.out.cr : j.out.cb) + (size_t)y * j.pc + x;
+      if (x + 1 < cx1) *reinterpret_cast<uint32_t*>(dst) = outp;
+      else dst[0] = (uint16_t)outp;
     }
   }
+  if (threadIdx.x < 4) s.nlist[threadIdx.x] = 0;  // descriptor lists of the next tile
   __syncthreads();  // LDS is reused by the next tile
 }
 
@@ -624,6 +756,8 @@ __global__ void __launch_bounds__(256) k_fused(const dev_job* __restrict__ jobs,
   const uint32_t chunk = (n_tiles + 7) >> 3;
   const uint32_t begin = xcd * chunk, end = min(n_tiles, begin + chunk);
   int cur_job = -1;
+  if (threadIdx.x < 4) s.nlist[threadIdx.x] = 0;
+  __syncthreads();
   for (uint32_t t = begin + k; t < end; t += per_xcd) {
     const tile_ref tr = tiles[t];
     const dev_job& j = jobs[tr.job];

@@ -68,6 +68,21 @@ hipError_t launch_fused(int tw, int th, bool hbd, const dev_job* jobs, const til
                         uint32_t grid, hipStream_t s);
 int fused_max_blocks_per_cu(int tw, int th, bool hbd);
 inline bool job_needs_hbd(const dev_job& d) { return d.do_alf && d.alf_bd > 12; }
+// A fused tile lies inside one ALF CTB (one filter bank per tile) and spans at
+// most 3x3 SAO CTBs with its SAO reach (the kernel stages those parameters).
+inline bool job_fits_shape(const dev_job& d, uint32_t tw, uint32_t th) {
+  if (d.do_alf) {
+    const uint32_t ctb = 1u << d.ctb_log2;
+    if (tw > ctb || th > ctb || ctb % tw || ctb % th) return false;
+  }
+  if (d.do_sao) {
+    const uint32_t ctb = 1u << d.sao_ctb_log2;
+    if (tw > ctb || th > ctb || ctb < 8) return false;
+  }
+  return true;
+}
+// Fused tile shapes, largest first.
+constexpr int kFusedShapes[4][2] = {{128, 64}, {64, 64}, {64, 32}, {32, 32}};
 
 hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev = nullptr);
 

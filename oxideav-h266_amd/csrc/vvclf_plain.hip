@@ -77,15 +77,6 @@ __global__ void __launch_bounds__(256) k_copy(dev_job j, dev_planes from, dev_pl
 // in four consecutive lanes, which exchange the decision terms of lines 0 and
 // 3.  Block = 4 waves; each wave covers 16 segments.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ luma_terms shfl_terms(const luma_terms& t, int src) {
-  luma_terms r;
-  r.dp = __shfl(t.dp, src); r.dq = __shfl(t.dq, src); r.sp = __shfl(t.sp, src);
-  r.sq = __shfl(t.sq, src); r.spq = __shfl(t.spq, src); r.dpl = __shfl(t.dpl, src);
-  r.dql = __shfl(t.dql, src); r.sp7 = __shfl(t.sp7, src); r.sq7 = __shfl(t.sq7, src);
-  r.spm = __shfl(t.spm, src); r.sqm = __shfl(t.sqm, src);
-  return r;
-}
-
 template <bool VERT>
 __global__ void __launch_bounds__(256) k_dbk_luma(dev_job j) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -114,8 +105,7 @@ __global__ void __launch_bounds__(256) k_dbk_luma(dev_job j) {
   edge_thresholds(e, j.bd_dbk, beta, tc);
   int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
   luma_terms mine = luma_line_terms(p, q, mflp, mflq);
-  int base = lane & ~3;
-  luma_terms t0 = shfl_terms(mine, base), t3 = shfl_terms(mine, base + 3);
+  luma_terms t0 = quad_terms<0>(mine), t3 = quad_terms<3>(mine);
   luma_decision d = luma_decide(t0, t3, beta, tc, mflp, mflq, (e & E_CTBROW_BIT) != 0);
   int np[8], nq[8], nwp, nwq;
   luma_filter_line(p, q, d, tc, np, nq, nwp, nwq);

@@ -265,6 +265,11 @@ void prep_loop(vvclf_server* s, int idx) {
       o.res = &r->res->t;
       rc = job_create_stages(s->ctx, r->frame, STAGE_ALL, &r->job, nullptr, &o);
     }
+    if (!rc && !job_fits_shape(r->job->d, 32, 32)) {  // 32x32 fits every job any shape fits
+      vvclf_job_destroy(r->job);
+      r->job = nullptr;
+      rc = fail(VVCLF_ERR_UNSUPPORTED, "served picture: ALF or SAO CTBs smaller than 32 samples");
+    }
     std::lock_guard<std::mutex> g(s->mu);
     if (rc) {
       s->bytes_admitted -= r->bytes;
@@ -302,12 +307,9 @@ void launch_loop(vvclf_server* s) {
   int cus = 256;
   hipDeviceProp_t p;
   if (hipGetDeviceProperties(&p, s->ctx->device) == hipSuccess) cus = p.multiProcessorCount;
-  const int tw = (int)s->cfg.tune.tile_w, th = (int)s->cfg.tune.tile_h;
-  uint32_t wpc_of[2];
-  for (int h = 0; h < 2; h++) {
-    int occ = std::max(1, fused_max_blocks_per_cu(tw, th, h != 0));
-    wpc_of[h] = s->cfg.tune.wgs_per_cu ? std::min<uint32_t>(s->cfg.tune.wgs_per_cu, occ) : (uint32_t)occ;
-  }
+  const int cfg_tw = (int)s->cfg.tune.tile_w, cfg_th = (int)s->cfg.tune.tile_h;
+  // workgroups per CU of each (shape, >12-bit variant), 0 = not yet asked
+  uint32_t wpc_of[4][2] = {};
   for (;;) {
     inflight fl;
     {
@@ -328,13 +330,36 @@ void launch_loop(vvclf_server* s) {
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
       return (uint64_t)fl.reqs[a]->job->w * fl.reqs[a]->job->h > (uint64_t)fl.reqs[b]->job->w * fl.reqs[b]->job->h;
     });
+    // the configured tile shape, or the largest one every picture of the
+    // launch fits (32x32 fits all admitted pictures)
+    auto all_fit = [&](int a, int b) {
+      for (auto* r : fl.reqs)
+        if (!job_fits_shape(r->job->d, (uint32_t)a, (uint32_t)b)) return false;
+      return true;
+    };
+    int tw = cfg_tw, th = cfg_th;
+    if (!all_fit(tw, th)) {
+      for (auto& sh : kFusedShapes)
+        if (all_fit(sh[0], sh[1])) {
+          tw = sh[0];
+          th = sh[1];
+          break;
+        }
+    }
+    int si = 3;
+    for (int k = 0; k < 4; k++)
+      if (kFusedShapes[k][0] == tw && kFusedShapes[k][1] == th) si = k;
     uint32_t nt = 0;
     bool hbd = false;
     for (auto* r : fl.reqs) {
       nt += ((r->job->w + tw - 1) / tw) * ((r->job->h + th - 1) / th);
       hbd = hbd || job_needs_hbd(r->job->d);
     }
-    const uint32_t wpc = wpc_of[hbd ? 1 : 0];
+    uint32_t& wpc = wpc_of[si][hbd ? 1 : 0];
+    if (wpc == 0) {
+      const int occ = std::max(1, fused_max_blocks_per_cu(tw, th, hbd));
+      wpc = s->cfg.tune.wgs_per_cu ? std::min<uint32_t>(s->cfg.tune.wgs_per_cu, occ) : (uint32_t)occ;
+    }
     launch_slot& sl = s->slots[fl.slot];
     hipError_t e = grow_slot(sl, (uint32_t)fl.reqs.size(), nt) ? hipSuccess : hipErrorOutOfMemory;
     if (e == hipSuccess) {
@@ -441,6 +466,13 @@ int vvclf_server_create(vvclf_ctx* ctx, const vvclf_server_config* cfg, vvclf_se
   if (!s->cfg.max_batch) s->cfg.max_batch = 32;
   if (!s->cfg.prep_threads) s->cfg.prep_threads = 4;
   if (!s->cfg.tune.tile_w) s->cfg.tune = vvclf_tune{64, 64, 256, 0};
+  bool shape_ok = false;
+  for (auto& sh : kFusedShapes) shape_ok = shape_ok || (sh[0] == (int)s->cfg.tune.tile_w && sh[1] == (int)s->cfg.tune.tile_h);
+  if (!shape_ok) {
+    const uint32_t tw = s->cfg.tune.tile_w, th = s->cfg.tune.tile_h;
+    delete s;
+    return fail(VVCLF_ERR_INVALID, "server_create: tile %ux%u is not a fused tile shape", tw, th);
+  }
   hipSetDevice(ctx->device);
   // keep freed job memory in the device pool for reuse
   hipMemPool_t pool;

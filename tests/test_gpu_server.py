@@ -132,3 +132,42 @@ def test_injected_failure_spares_other_jobs(vv, orc):
                 assert np.array_equal(a, b)
     vv.lib.vvclf_server_destroy(s)
     ctx.close()
+
+
+def test_small_ctbs_pick_a_fitting_tile(vv, orc):
+    """A launch holding pictures with 32-sample CTBs runs 32x32 tiles (the
+    configured 64x64 tiles would straddle their ALF CTBs); all stay exact."""
+    say("server small ctbs")
+    ctx = vv.Context()
+    s = server(vv, ctx, batch=8)
+    cases = [(1, 416, 240, 31, {"ctb_log2": 5}), (0, 256, 128, 32, {}), (1, 208, 120, 33, {"ctb_log2": 5, "rich": 1}),
+             (3, 416, 240, 34, {"ctb_log2": 6})]
+    sid = C.c_uint32()
+    vv.check(vv.lib.vvclf_server_open_stream(s, C.byref(sid)))
+    frames, refs, tickets = [], [], []
+    for cfg, w, h, seed, kw in cases:
+        fg = vv.SynthFrame(cfg, w, h, seed=seed, **kw)
+        fo = vv.SynthFrame(cfg, w, h, seed=seed, **kw)
+        assert orc.oracle_apply_in_loop_filters(fo.desc) == 0
+        t = C.c_uint64()
+        vv.check(vv.lib.vvclf_server_submit(s, sid.value, fg.desc, C.byref(t)))
+        frames.append(fg)
+        refs.append(fo)
+        tickets.append(t.value)
+    for t in tickets:
+        assert vv.lib.vvclf_server_wait(s, t, 60000) == 0, vv.lib.vvclf_last_error()
+    for fg, fo, case in zip(frames, refs, cases):
+        for a, b in zip(fo.output_arrays(), fg.output_arrays()):
+            assert np.array_equal(a, b), case
+    vv.lib.vvclf_server_destroy(s)
+    ctx.close()
+
+
+def test_server_rejects_unknown_tile_shape(vv):
+    ctx = vv.Context()
+    cfg = vv.ServerConfig()
+    cfg.tune = vv.Tune(48, 48, 256, 0)
+    s = C.c_void_p()
+    assert vv.lib.vvclf_server_create(ctx.h, C.byref(cfg), C.byref(s)) == -1  # VVCLF_ERR_INVALID
+    assert b"48x48" in vv.lib.vvclf_last_error()
+    ctx.close()
