@@ -162,6 +162,9 @@ def served_run(vv, ctx, seconds, rate, n_streams=32, seed=7):
             "throughput_Mpixel_s": round(st.luma_pixels / st.wall_s / 1e6, 2) if st.wall_s else None,
             "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3), "max_ms": round(st.max_ms, 3),
             "admitted_bytes_peak": int(st.bytes_admitted_peak), "hbm_budget_bytes": int(cfg.hbm_budget_bytes),
+            "stage_p50_ms": {k: round(st.stage_p50_ms[i], 3) for i, k in
+                             enumerate(["admission", "host_tables_and_upload", "wait_for_launch", "device",
+                                        "download"])},
             "latency": "submit -> filtered samples back in host memory (includes host table derivation, "
                        "H2D and D2H copies)"}
 

@@ -418,6 +418,10 @@ typedef struct vvclf_server_stats {
   uint64_t bytes_admitted_peak;
   double p50_ms, p99_ms, max_ms, mean_ms; /* submit -> result in the caller's planes */
   double wall_s;               /* first submit -> last completion */
+  /* median time per stage of completed jobs: [0] waiting for admission,
+   * [1] host table derivation + upload, [2] waiting for a launch, [3] on the
+   * device, [4] download into the caller's planes */
+  double stage_p50_ms[5];
 } vvclf_server_stats;
 
 /* A server owns three HIP streams (upload, compute, download), a pool of

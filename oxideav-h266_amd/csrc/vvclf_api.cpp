@@ -111,6 +111,7 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
     uint64_t py = round_up(W, 64), pc = Wc ? round_up(Wc, 64) : 0;
     uint64_t side = t.lmcs_lut.size() * 2 + (t.e_lv.size() + t.e_lh.size()) * 4 +
                     (t.e_cv[0].size() + t.e_cv[1].size() + t.e_ch[0].size() + t.e_ch[1].size()) * 4 +
+                    (t.l_lv.size() + t.l_lh.size() + t.l_cv.size() + t.l_ch.size()) * 4 +
                     (t.sao.size() + t.sao_rect.size() + t.alf_rect.size()) * 8 +
                     (t.alf_ctb.size() + t.alf_bank.size() + t.alf_cbank.size() + t.alf_ccbank.size()) * 4;
     dry_stats[6] = 4 * (py * H + 2 * pc * Hc) * 2 + side;
@@ -185,6 +186,7 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   }
   size_t o_lut = sb.add(t.lmcs_lut), o_lv = sb.add(t.e_lv), o_lh = sb.add(t.e_lh);
   size_t o_cv0 = sb.add(t.e_cv[0]), o_cv1 = sb.add(t.e_cv[1]), o_ch0 = sb.add(t.e_ch[0]), o_ch1 = sb.add(t.e_ch[1]);
+  size_t o_llv = sb.add(t.l_lv), o_llh = sb.add(t.l_lh), o_lcv = sb.add(t.l_cv), o_lch = sb.add(t.l_ch);
   size_t o_sao = sb.add(t.sao), o_saor = sb.add(t.sao_rect);
   size_t o_actb = sb.add(t.alf_ctb), o_arect = sb.add(t.alf_rect), o_bank = sb.add(t.alf_bank);
   size_t o_cbank = sb.add(t.alf_cbank), o_ccbank = sb.add(t.alf_ccbank);
@@ -216,6 +218,14 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   d.e_cv[1] = (const uint32_t*)(sd + o_cv1);
   d.e_ch[0] = (const uint32_t*)(sd + o_ch0);
   d.e_ch[1] = (const uint32_t*)(sd + o_ch1);
+  d.l_lv = (const uint32_t*)(sd + o_llv);
+  d.l_lh = (const uint32_t*)(sd + o_llh);
+  d.l_cv = (const uint32_t*)(sd + o_lcv);
+  d.l_ch = (const uint32_t*)(sd + o_lch);
+  d.n_lv = (int32_t)t.l_lv.size();
+  d.n_lh = (int32_t)t.l_lh.size();
+  d.n_cv = (int32_t)t.l_cv.size();
+  d.n_ch = (int32_t)t.l_ch.size();
   d.sao = (const uint64_t*)(sd + o_sao);
   d.sao_rect = t.sao_rect.empty() ? nullptr : (const uint64_t*)(sd + o_saor);
   d.alf_ctb = (const uint32_t*)(sd + o_actb);

@@ -506,12 +506,14 @@ __device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cas
 __device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 // sample at window row r, column c
-__device__ __forceinline__ uint32_t pw_at(const uint32_t (&P)[10][6], int r, int c) {
+template <int N>
+__device__ __forceinline__ uint32_t pw_at(const uint32_t (&P)[N][6], int r, int c) {
   const int q = c + 1;
   return (q & 1) ? (P[r][q >> 1] >> 16) : (P[r][q >> 1] & 0xffffu);
 }
 // samples at window columns c, c+1 of row r as one pair
-__device__ __forceinline__ uint32_t pw_pair(const uint32_t (&P)[10][6], int r, int c) {
+template <int N>
+__device__ __forceinline__ uint32_t pw_pair(const uint32_t (&P)[N][6], int r, int c) {
   const int q = c + 1;
   return (q & 1) ? __builtin_amdgcn_alignbit(P[r][(q >> 1) + 1], P[r][q >> 1], 16) : P[r][q >> 1];
 }
@@ -563,48 +565,116 @@ __device__ __forceinline__ void alf_pk_tap(uint32_t c0, uint32_t c1, uint32_t cl
 __device__ constexpr int kTapDy[12] = {3, 2, 2, 2, 1, 1, 1, 1, 1, 0, 0, 0};
 __device__ constexpr int kTapDx[12] = {0, 1, 0, -1, 2, 1, 0, -1, -2, 3, 2, 1};
 
-// out[r][h]: filtered samples (x4+2h, x4+2h+1) of row y4+r as a u16 pair.
-// Taps are the outer loop so that only one tap's constants are live.
-template <int VB>
-__device__ __forceinline__ void alf_pk_filter(const uint32_t (&P)[10][6], const uint32_t* f, int maxv,
-                                              uint32_t (&out)[4][2]) {
+// Output rows R0 .. R0+NR-1 of the block: out[r][h] receives the filtered
+// samples (x4+2h, x4+2h+1) of row y4+r as a u16 pair.  Q[i] holds window row
+// R0 + i (N >= NR + 6 rows).  Taps are the outer loop so that only one tap's
+// constants are live.
+template <int VB, int R0, int NR, int N>
+__device__ __forceinline__ void alf_pk_filter_rows(const uint32_t (&Q)[N][6], const uint32_t* f, int maxv,
+                                                   uint32_t (&out)[4][2]) {
   constexpr int MT[3][4] = {{3, 3, 3, 3}, {3, 2, 1, 0}, {0, 1, 2, 3}};
-  int acc0[4][2], acc1[4][2];
+  int acc0[NR][2], acc1[NR][2];
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int r = 0; r < NR; r++)
 #pragma unroll
-    for (int h = 0; h < 2; h++) acc0[r][h] = acc1[r][h] = MT[VB][r] == 0 ? 512 : 64;
+    for (int h = 0; h < 2; h++) acc0[r][h] = acc1[r][h] = MT[VB][R0 + r] == 0 ? 512 : 64;
 #pragma unroll
   for (int k = 0; k < 12; k++) {
     const uint32_t w = f[k], kk = w >> 16;
     const uint32_t c0 = w & 0xffffu, c1 = w << 16, clip = kk | ((0u - kk) << 16);
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int m = MT[VB][r];
+    for (int r = 0; r < NR; r++) {
+      const int m = MT[VB][R0 + r];
       const int dy = kTapDy[k] < m ? kTapDy[k] : m;
-      const int R = r + 3;
+      const int R = r + 3;  // centre row in Q
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const int C = 3 + 2 * h;
-        alf_pk_tap(c0, c1, clip, as_s16x2(pw_pair(P, R, C)), pw_pair(P, R + dy, C + kTapDx[k]),
-                   pw_pair(P, R - dy, C - kTapDx[k]), acc0[r][h], acc1[r][h]);
+        alf_pk_tap(c0, c1, clip, as_s16x2(pw_pair(Q, R, C)), pw_pair(Q, R + dy, C + kTapDx[k]),
+                   pw_pair(Q, R - dy, C - kTapDx[k]), acc0[r][h], acc1[r][h]);
       }
     }
   }
   const s16x2 zero = {0, 0}, vmax = {(short)maxv, (short)maxv};
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int sft = MT[VB][r] == 0 ? 10 : 7;
+  for (int r = 0; r < NR; r++) {
+    const int sft = MT[VB][R0 + r] == 0 ? 10 : 7;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       // the offsets saturate to i16; the saturating add and the clip to
       // [0, maxv] then give the same result as the exact sum
       const s16x2 d = __builtin_amdgcn_cvt_pk_i16(acc0[r][h] >> sft, acc1[r][h] >> sft);
-      const s16x2 o = __builtin_elementwise_add_sat(as_s16x2(pw_pair(P, r + 3, 3 + 2 * h)), d);
-      out[r][h] = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(o, zero), vmax));
+      const s16x2 o = __builtin_elementwise_add_sat(as_s16x2(pw_pair(Q, r + 3, 3 + 2 * h)), d);
+      out[R0 + r][h] = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(o, zero), vmax));
     }
   }
 }
+
+template <int VB>
+__device__ __forceinline__ void alf_pk_filter(const uint32_t (&P)[10][6], const uint32_t* f, int maxv,
+                                              uint32_t (&out)[4][2]) {
+  alf_pk_filter_rows<VB, 0, 4, 10>(P, f, maxv, out);
+}
+
+// Classification reading the window from LDS row by row (three window rows
+// live at a time) instead of holding all ten; base points at sample
+// (x4-4, y4-3), rows PITCH samples apart, 8-byte aligned.  Same sums as
+// alf_pk_class.
+template <int PITCH>
+__device__ __forceinline__ void lds_row(const uint16_t* base, int r, uint32_t (&R)[6]) {
+  const uint2* q = reinterpret_cast<const uint2*>(base + r * PITCH);
+  const uint2 a = q[0], b = q[1], c = q[2];
+  R[0] = a.x; R[1] = a.y; R[2] = b.x; R[3] = b.y; R[4] = c.x; R[5] = c.y;
+}
+__device__ __forceinline__ uint32_t row_at(const uint32_t (&R)[6], int c) {
+  const int q = c + 1;
+  return (q & 1) ? (R[q >> 1] >> 16) : (R[q >> 1] & 0xffffu);
+}
+
+template <int VB, bool CARVE, int PITCH>
+__device__ __forceinline__ int alf_lds_class(const uint16_t* base, int bd) {
+  constexpr int jmin = (VB == 2 && CARVE) ? 0 : -2;
+  constexpr int jmax = (VB == 1 && CARVE) ? 3 : 5;
+  constexpr int ac = ((VB == 1 || VB == 2) && CARVE) ? 3 : 2;
+  uint32_t sh = 0, sv = 0, sd0 = 0, sd1 = 0;
+#pragma unroll
+  for (int jj = jmin; jj <= jmax; jj++) {
+    int rc = jj + 3, ru = jj + 2, rd = jj + 4;
+    if (VB == 1) { rc = rc > 6 ? 6 : rc; ru = ru > 6 ? 6 : ru; rd = rd > 6 ? 6 : rd; }
+    if (VB == 2) { rc = rc < 3 ? 3 : rc; ru = ru < 3 ? 3 : ru; rd = rd < 3 ? 3 : rd; }
+    uint32_t U[6], Cr[6], D[6];
+    lds_row<PITCH>(base, ru, U);
+    lds_row<PITCH>(base, rc, Cr);
+    lds_row<PITCH>(base, rd, D);
+#pragma unroll
+    for (int ii = -2; ii <= 5; ii++) {
+      if (((ii + 4) & 1) != ((jj + 4) & 1)) continue;
+      const int cc = ii + 3;
+      const uint32_t c2 = row_at(Cr, cc) * 2;
+      sh = absdiff_acc(c2, row_at(Cr, cc - 1) + row_at(Cr, cc + 1), sh);
+      sv = absdiff_acc(c2, row_at(U, cc) + row_at(D, cc), sv);
+      sd0 = absdiff_acc(c2, row_at(U, cc - 1) + row_at(D, cc + 1), sd0);
+      sd1 = absdiff_acc(c2, row_at(U, cc + 1) + row_at(D, cc - 1), sd1);
+    }
+    __asm__ volatile("" ::: "memory");  // keep each row's loads next to their use
+  }
+  return alf_class_from_sums((int)sh, (int)sv, (int)sd0, (int)sd1, ac, bd);
+}
+
+template <int PITCH>
+__device__ __forceinline__ int alf_lds_block_class(const uint16_t* base, int vb, bool carve, int bd) {
+  if (vb == 0) return alf_lds_class<0, false, PITCH>(base, bd);
+  if (vb == 1) return carve ? alf_lds_class<1, true, PITCH>(base, bd) : alf_lds_class<1, false, PITCH>(base, bd);
+  return carve ? alf_lds_class<2, true, PITCH>(base, bd) : alf_lds_class<2, false, PITCH>(base, bd);
+}
+
+// Class of a block at yb = y4 - yCtb (virtual-boundary variant picked here).
+__device__ __forceinline__ int alf_pk_block_class(const uint32_t (&P)[10][6], int vb, bool carve, int bd) {
+  if (vb == 0) return alf_pk_class<0, false>(P, bd);
+  if (vb == 1) return carve ? alf_pk_class<1, true>(P, bd) : alf_pk_class<1, false>(P, bd);
+  return carve ? alf_pk_class<2, true>(P, bd) : alf_pk_class<2, false>(P, bd);
+}
+__device__ __forceinline__ int alf_vb_kind(int yb, int ctb) { return yb == ctb - 8 ? 1 : (yb == ctb - 4 ? 2 : 0); }
 
 // Classification + filter of one block from the packed window; `bank` holds
 // the CTB's 25 x 4 x 12 tap words, yb = y4 - yCtb.

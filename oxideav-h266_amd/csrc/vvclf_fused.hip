@@ -78,37 +78,30 @@ struct mrect {
   int x0, y0, nx, ny;
 };
 
-// Global -> register -> LDS staging of up to PER*256 words of a map
-// rectangle; issue() for every rectangle first, then commit(), so that all
-// loads of a tile are in flight together.
-template <int PER>
+// Global -> register -> LDS staging of a map rectangle (at most MH x MW
+// words, LDS row pitch MW); issue() for every rectangle first, then
+// commit(), so that all loads of a tile are in flight together.  Element i
+// covers LDS row i / MW, column i % MW (compile-time divisor).
+template <int MW, int MH>
 struct rect_stage {
+  static constexpr int PER = (MW * MH + 255) / 256;
   uint32_t v[PER];
   __device__ __forceinline__ void issue(const uint32_t* map, int mw, const mrect& r) {
-    const int n = r.nx > 0 && r.ny > 0 ? r.nx * r.ny : 0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = (int)threadIdx.x + k * 256;
-      if (i < n) {
-        const int yy = i / r.nx, xx = i - yy * r.nx;
-        v[k] = map[(size_t)(r.y0 + yy) * mw + r.x0 + xx];
-      }
+      const int i = (int)threadIdx.x + k * 256, yy = i / MW, xx = i - yy * MW;
+      if (yy < r.ny && xx < r.nx) v[k] = map[(size_t)(r.y0 + yy) * mw + r.x0 + xx];
     }
   }
   // also lists the nonzero words as tag | (row << 6) | column
-  __device__ __forceinline__ void commit(uint32_t* dst, int pitch, const mrect& r, uint16_t* list, uint32_t* cnt,
+  __device__ __forceinline__ void commit(uint32_t* dst, const mrect& r, uint16_t* list, uint32_t* cnt,
                                          uint32_t tag) const {
-    const int n = r.nx > 0 && r.ny > 0 ? r.nx * r.ny : 0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = (int)threadIdx.x + k * 256;
-      int yy = 0, xx = 0;
-      if (i < n) {
-        yy = i / r.nx;
-        xx = i - yy * r.nx;
-        dst[yy * pitch + xx] = v[k];
-      }
-      list_append(i < n && v[k] != 0, tag | (uint32_t)(yy << 6) | (uint32_t)xx, list, cnt);
+      const int i = (int)threadIdx.x + k * 256, yy = i / MW, xx = i - yy * MW;
+      const bool in = yy < r.ny && xx < r.nx;
+      if (in) dst[i] = v[k];
+      list_append(in && v[k] != 0, tag | (uint32_t)(yy << 6) | (uint32_t)xx, list, cnt);
     }
   }
 };
@@ -298,34 +291,32 @@ __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)
     const int x = dst.ox + xx, y = dst.oy + yy;
     const int sidx = (y - src.oy) * src.pitch + (x - src.ox);  // (x, y) lies inside src
     const int pv = src.p[sidx];
-    int out = pv;
     const int cell = ((y >> log2c) - tb.by) * 3 + ((x >> log2c) - tb.bx);
     const uint64_t prm = saop[cell][comp];
-    const int type = (int)(prm & 3);
-    if (type == 1) {
-      out = sao_band_sample(pv, prm, j.sao_bd, maxv);
-    } else if (type == 2) {
-      const int cls = (int)((prm >> 2) & 3);
-      // Table 44: the second neighbour mirrors the first
-      const int h0 = (cls == 0 || cls == 2) ? -1 : (cls == 3 ? 1 : 0), v0 = cls == 0 ? 0 : -1;
-      const uint64_t sr = safe[cell][comp];
-      if (x >= (int)(sr & 0xffff) && x < (int)((sr >> 16) & 0xffff) && y >= (int)((sr >> 32) & 0xffff) &&
-          y < (int)(sr >> 48)) {
-        const int d = v0 * src.pitch + h0;
-        out = sao_edge_sample(pv, src.p[sidx + d], src.p[sidx - d], prm, maxv);
-      } else {
-        const int nx0 = x + h0, ny0 = y + v0, nx1 = x - h0, ny1 = y - v0;
-        bool outside = nx0 < 0 || ny0 < 0 || nx1 < 0 || ny1 < 0 || nx0 >= pw || ny0 >= ph || nx1 >= pw || ny1 >= ph;
-        if (has_rect) {
-          const uint64_t r = saor[cell];
-          int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff),
-              cy1 = (int)(r >> 48);
-          if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
-          outside = outside || nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
-                    nx1 >= cx1 || ny1 >= cy1;
-        }
-        out = outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, src.rd(nx0, ny0), src.rd(nx1, ny1), prm, maxv);
+    const uint64_t sr = safe[cell][comp];
+    const int type = (int)(prm & 3), cls = (int)((prm >> 2) & 3);
+    // Table 44 neighbour offsets (the second mirrors the first):
+    // h0 = -1, 0, -1, 1 and v0 = 0, -1, -1, -1 for classes 0..3
+    const int h0 = (int)((0x84u >> (2 * cls)) & 3u) - 1, v0 = cls == 0 ? 0 : -1;
+    const bool inner = x >= (int)(sr & 0xffff) && x < (int)((sr >> 16) & 0xffff) && y >= (int)((sr >> 32) & 0xffff) &&
+                       y < (int)(sr >> 48);
+    const bool fast = type == 2 && inner;
+    const int d = fast ? v0 * src.pitch + h0 : 0;  // in-window neighbours, else a harmless self read
+    const int eo = sao_edge_sample(pv, src.p[sidx + d], src.p[sidx - d], prm, maxv);
+    const int bo = sao_band_sample(pv, prm, j.sao_bd, maxv);
+    int out = type == 1 ? bo : (fast ? eo : pv);
+    if (type == 2 && !inner) {
+      const int nx0 = x + h0, ny0 = y + v0, nx1 = x - h0, ny1 = y - v0;
+      bool outside = nx0 < 0 || ny0 < 0 || nx1 < 0 || ny1 < 0 || nx0 >= pw || ny0 >= ph || nx1 >= pw || ny1 >= ph;
+      if (has_rect) {
+        const uint64_t r = saor[cell];
+        int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff),
+            cy1 = (int)(r >> 48);
+        if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
+        outside = outside || nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
+                  nx1 >= cx1 || ny1 >= cy1;
       }
+      out = outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, src.rd(nx0, ny0), src.rd(nx1, ny1), prm, maxv);
     }
     dst.wr(x, y, out);
   }
@@ -351,6 +342,17 @@ __device__ __forceinline__ void pad_to_rect(uint16_t* buf, int pitch, int ox, in
     const int y = ya + i / w, x = xa + i % w;
     if (x >= rx0 && x <= rx1 && y >= ry0 && y <= ry1) continue;
     buf[(y - oy) * pitch + (x - ox)] = buf[(clampi(y, ry0, ry1) - oy) * pitch + (clampi(x, rx0, rx1) - ox)];
+  }
+}
+
+// N rows of 12 samples (6 aligned pairs) starting at `base` (8-byte aligned).
+template <int PITCH, int N>
+__device__ __forceinline__ void load_pairs(const uint16_t* base, uint32_t (&P)[N][6]) {
+#pragma unroll
+  for (int rr = 0; rr < N; rr++) {
+    const uint2* q = reinterpret_cast<const uint2*>(base + rr * PITCH);
+    const uint2 a = q[0], b = q[1], c = q[2];
+    P[rr][0] = a.x; P[rr][1] = a.y; P[rr][2] = b.x; P[rr][3] = b.y; P[rr][4] = c.x; P[rr][5] = c.y;
   }
 }
 
@@ -431,6 +433,31 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
       if (i < total && ch < nch) v[k] = *reinterpret_cast<const uint4*>(src + (size_t)yy * j.py + 8 * ch);
     }
+    // tiles larger than 64x64 commit the luma first, so that fewer loads
+    // (and registers) are in flight at once
+    constexpr bool SPLIT = TW * TH > 64 * 64;
+    const int lut_lds = j.bd <= 10;
+    auto commit_luma = [&]() {
+#pragma unroll
+      for (int k = 0; k < PER; k++) {
+        const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
+        if (i < total && ch < nch) {
+          if (j.do_lmcs) {
+            uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const uint32_t a = w4[q] & 0xffffu, b = w4[q] >> 16;
+              const uint32_t la = (lut_lds && a < 1024) ? s.lut[a] : j.lmcs_lut[a];
+              const uint32_t lb = (lut_lds && b < 1024) ? s.lut[b] : j.lmcs_lut[b];
+              w4[q] = la | (lb << 16);
+            }
+            v[k] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+          }
+          *reinterpret_cast<uint4*>(&s.L[yy * LD::LW + 8 * ch]) = v[k];
+        }
+      }
+    };
+    if (SPLIT) commit_luma();
     const int nchc = (Cw.w + 3) >> 2, per_plane = Cw.h * NC4;
     uint2 vc[PERC];
     if (chroma) {
@@ -443,10 +470,10 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
           vc[k] = *reinterpret_cast<const uint2*>(csrc + (size_t)(Cw.oy + yy) * j.pc + Cw.ox + 4 * ch);
       }
     }
-    rect_stage<PLV> slv;
-    rect_stage<PLH> slh;
-    rect_stage<PCV> scv0, scv1;
-    rect_stage<PCH> sch0, sch1;
+    rect_stage<LD::MLV_W, LD::MLV_H> slv;
+    rect_stage<LD::MLH_W, LD::MLH_H> slh;
+    rect_stage<LD::MCV_W, LD::MCV_H> scv0, scv1;
+    rect_stage<LD::MCH_W, LD::MCH_H> sch0, sch1;
     if (dbk) {
       slv.issue(j.e_lv, j.gw, mlv);
       slh.issue(j.e_lh, j.gw, mlh);
@@ -478,25 +505,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       if (on) sv = sk < 3 ? j.sao[sci * 3 + sk] : (j.sao_rect ? j.sao_rect[sci] : 0);
     }
     // ---- commit to LDS
-    const int lut_lds = j.bd <= 10;
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
-      if (i < total && ch < nch) {
-        if (j.do_lmcs) {
-          uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const uint32_t a = w4[q] & 0xffffu, b = w4[q] >> 16;
-            const uint32_t la = (lut_lds && a < 1024) ? s.lut[a] : j.lmcs_lut[a];
-            const uint32_t lb = (lut_lds && b < 1024) ? s.lut[b] : j.lmcs_lut[b];
-            w4[q] = la | (lb << 16);
-          }
-          v[k] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
-        }
-        *reinterpret_cast<uint4*>(&s.L[yy * LD::LW + 8 * ch]) = v[k];
-      }
-    }
+    if (!SPLIT) commit_luma();
     if (chroma) {
 #pragma unroll
       for (int k = 0; k < PERC; k++) {
@@ -506,14 +515,14 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       }
     }
     if (dbk) {
-      slv.commit(s.m.lv, LD::MLV_W, mlv, s.m.llv, &s.nlist[0], 0);
-      slh.commit(s.m.lh, LD::MLH_W, mlh, s.m.llh, &s.nlist[1], 0);
+      slv.commit(s.m.lv, mlv, s.m.llv, &s.nlist[0], 0);
+      slh.commit(s.m.lh, mlh, s.m.llh, &s.nlist[1], 0);
     }
     if (dbk_c) {
-      scv0.commit(s.m.cv[0], LD::MCV_W, mcv, s.m.lcv, &s.nlist[2], 0);
-      scv1.commit(s.m.cv[1], LD::MCV_W, mcv, s.m.lcv, &s.nlist[2], 1u << 15);
-      sch0.commit(s.m.ch[0], LD::MCH_W, mch, s.m.lch, &s.nlist[3], 0);
-      sch1.commit(s.m.ch[1], LD::MCH_W, mch, s.m.lch, &s.nlist[3], 1u << 15);
+      scv0.commit(s.m.cv[0], mcv, s.m.lcv, &s.nlist[2], 0);
+      scv1.commit(s.m.cv[1], mcv, s.m.lcv, &s.nlist[2], 1u << 15);
+      sch0.commit(s.m.ch[0], mch, s.m.lch, &s.nlist[3], 0);
+      sch1.commit(s.m.ch[1], mch, s.m.lch, &s.nlist[3], 1u << 15);
     }
     if (alf_on) {
 #pragma unroll
@@ -603,6 +612,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     const int y_ctb = ry * ctb;
     const bool carve = !(y_ctb + ctb >= H) || (H - y_ctb) > ctb - 4;
     uint16_t* dst = j.out.y;
+#pragma unroll 1
     for (int b = threadIdx.x; b < bw * bh; b += NT) {
       const int x4 = x0 + 4 * (b % bw), y4 = y0 + 4 * (b / bw);
       uint32_t outp[4][2];
@@ -627,15 +637,30 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
           outp[rr][1] = (uint32_t)outv[rr][2] | ((uint32_t)outv[rr][3] << 16);
         }
       } else {
-        uint32_t P[10][6];
+        // classification over the 10-row neighbourhood, then the filter in
+        // two halves of two output rows, each reloading its 8 window rows:
+        // the compiler barriers keep the loads apart so that fewer
+        // registers are live at once
         const uint16_t* base = &s.o.S[(y4 - 3 - soy) * LD::SW + (x4 - 4 - sox)];
-#pragma unroll
-        for (int rr = 0; rr < 10; rr++) {
-          const uint2* q = reinterpret_cast<const uint2*>(base + rr * LD::SW);
-          const uint2 a = q[0], b2 = q[1], c2 = q[2];
-          P[rr][0] = a.x; P[rr][1] = a.y; P[rr][2] = b2.x; P[rr][3] = b2.y; P[rr][4] = c2.x; P[rr][5] = c2.y;
+        const int vb = alf_vb_kind(y4 - y_ctb, ctb);
+        const int cls = alf_lds_block_class<LD::SW>(base, vb, carve, j.alf_bd);
+        const uint32_t* f = s.bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
+        __asm__ volatile("" ::: "memory");
+        {
+          uint32_t Q[8][6];
+          load_pairs<LD::SW>(base, Q);
+          if (vb == 0) alf_pk_filter_rows<0, 0, 2, 8>(Q, f, amax, outp);
+          else if (vb == 1) alf_pk_filter_rows<1, 0, 2, 8>(Q, f, amax, outp);
+          else alf_pk_filter_rows<2, 0, 2, 8>(Q, f, amax, outp);
         }
-        alf_block_pk(P, s.bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outp);
+        __asm__ volatile("" ::: "memory");
+        {
+          uint32_t Q[8][6];
+          load_pairs<LD::SW>(base + 2 * LD::SW, Q);
+          if (vb == 0) alf_pk_filter_rows<0, 2, 2, 8>(Q, f, amax, outp);
+          else if (vb == 1) alf_pk_filter_rows<1, 2, 2, 8>(Q, f, amax, outp);
+          else alf_pk_filter_rows<2, 2, 2, 8>(Q, f, amax, outp);
+        }
       }
       if (x4 + 4 <= x1) {
 #pragma unroll
@@ -749,7 +774,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
 // XCD (workgroups are dispatched round-robin over the 8 XCDs), so spatially
 // adjacent tiles - which share halo samples - hit the same L2.
 template <int TW, int TH, bool HBD>
-__global__ void __launch_bounds__(256) k_fused(const dev_job* __restrict__ jobs, const tile_ref* __restrict__ tiles,
+__global__ void __launch_bounds__(256, 2) k_fused(const dev_job* __restrict__ jobs, const tile_ref* __restrict__ tiles,
                                                uint32_t n_tiles) {
   __shared__ fused_lds<TW, TH> s;
   const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;

@@ -447,6 +447,22 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
     }
     t.has_chroma_dbk = true;
   }
+  // compact lists of the descriptors that filter something
+  auto list_nonzero = [](const std::vector<uint32_t>& e, uint32_t w, uint32_t tag, std::vector<uint32_t>& out) {
+    for (size_t i = 0; i < e.size(); i++)
+      if (e[i]) out.push_back(tag | ((uint32_t)(i / w) << 16) | (uint32_t)(i % w));
+  };
+  t.l_lv.clear();
+  t.l_lh.clear();
+  t.l_cv.clear();
+  t.l_ch.clear();
+  list_nonzero(t.e_lv, t.gw, 0, t.l_lv);
+  list_nonzero(t.e_lh, t.gw, 0, t.l_lh);
+  if (t.has_chroma_dbk)
+    for (int c = 0; c < 2; c++) {
+      list_nonzero(t.e_cv[c], t.cvw, (uint32_t)c << 31, t.l_cv);
+      list_nonzero(t.e_ch[c], t.chw, (uint32_t)c << 31, t.l_ch);
+    }
   for (int i = 0; i < 3; i++) t.bs_hist[i] = hist[i];
   t.chroma_filtered = hist[3];
   return VVCLF_OK;
@@ -614,6 +630,10 @@ int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t,
       if (sc.bit_depth < 1 || sc.bit_depth > 16) return fail(VVCLF_ERR_INVALID, "sao: bit_depth %u", sc.bit_depth);
       if (sc.chroma_used && sc.chroma_format_idc != 0 && sc.chroma_format_idc != 1)
         return fail(VVCLF_ERR_UNSUPPORTED, "sao: chroma_format_idc %u", sc.chroma_format_idc);
+      // CTBs of 16..256 samples (VVC uses 32..128): the kernels take 8-sample
+      // chroma runs inside one CTB
+      if (sc.ctb_log2_size_y < 4 || sc.ctb_log2_size_y > 8)
+        return fail(VVCLF_ERR_UNSUPPORTED, "sao: ctb_log2_size_y %u", sc.ctb_log2_size_y);
       t.do_sao = true;
       t.sao_luma = sc.luma_used != 0;
       t.sao_chroma = sc.chroma_used != 0 && sc.chroma_format_idc != 0;

@@ -18,6 +18,8 @@ struct host_tables {
   // deblock
   uint32_t gw = 0, gh = 0, cvw = 0, cvh = 0, chw = 0, chh = 0;
   std::vector<uint32_t> e_lv, e_lh, e_cv[2], e_ch[2];
+  // nonzero descriptors: luma gy << 16 | gx; chroma comp << 31 | sy << 16 | sx
+  std::vector<uint32_t> l_lv, l_lh, l_cv, l_ch;
   bool has_chroma_dbk = false;
   uint64_t bs_hist[3] = {0, 0, 0};
   uint64_t chroma_filtered = 0;

@@ -112,6 +112,13 @@ struct dev_job {
   int32_t chw, chh;          // chroma horizontal map dims (ceil(wc/2), ceil(hc/8))
   const uint32_t* e_cv[2];   // Cb, Cr vertical edges
   const uint32_t* e_ch[2];   // Cb, Cr horizontal edges
+  // the nonzero descriptors of each direction as lists (luma gy << 16 | gx,
+  // chroma (comp - 1) << 31 | sy << 16 | sx), for the per-picture kernels
+  const uint32_t* l_lv;
+  const uint32_t* l_lh;
+  const uint32_t* l_cv;
+  const uint32_t* l_ch;
+  int32_t n_lv, n_lh, n_cv, n_ch;
   // SAO
   const uint64_t* sao;       // [hctb*wctb][3]
   const uint64_t* sao_rect;  // [hctb*wctb] (luma coords), NULL = picture

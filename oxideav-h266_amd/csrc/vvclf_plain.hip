@@ -41,24 +41,42 @@ __device__ __forceinline__ int rdc(const plane_view& v, int x, int y) {
 // ---------------------------------------------------------------------------
 // prep: 8 samples per thread, rows of every plane (blockIdx.z = component).
 // ---------------------------------------------------------------------------
+// A workgroup covers 2048 samples of PREP_ROWS rows; for luma with LMCS at
+// <= 10 bits the first 1024 entries of the inverse table are staged in LDS.
+constexpr int PREP_ROWS = 8;
 __global__ void __launch_bounds__(256) k_prep(dev_job j) {
-  int comp = blockIdx.z;
+  __shared__ uint16_t lut[1024];
+  const int comp = blockIdx.z;
   plane_view src = pick(j.in, j, comp), dst = pick(j.a, j, comp);
-  int y = blockIdx.y;
-  int x = (blockIdx.x * 256 + threadIdx.x) * 8;
-  if (y >= src.h || x >= src.w) return;
-  const uint4* s = reinterpret_cast<const uint4*>(src.p + (size_t)y * src.pitch + x);
-  uint4 v = *s;
-  if (comp == 0 && j.do_lmcs) {
-    uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      uint32_t lo = j.lmcs_lut[w[i] & 0xffff], hi = j.lmcs_lut[w[i] >> 16];
-      w[i] = lo | (hi << 16);
-    }
-    v = make_uint4(w[0], w[1], w[2], w[3]);
+  const int x = (blockIdx.x * 256 + threadIdx.x) * 8;
+  const int y0 = blockIdx.y * PREP_ROWS;
+  const bool map = comp == 0 && j.do_lmcs;
+  const bool lut_lds = map && j.bd <= 10;
+  if (lut_lds) {
+    for (int i = threadIdx.x; i < 1024; i += 256) lut[i] = j.lmcs_lut[i];
+    __syncthreads();
   }
-  *reinterpret_cast<uint4*>(dst.p + (size_t)y * dst.pitch + x) = v;
+  if (x >= src.w) return;
+  uint4 v[PREP_ROWS];
+#pragma unroll
+  for (int r = 0; r < PREP_ROWS; r++)
+    if (y0 + r < src.h) v[r] = *reinterpret_cast<const uint4*>(src.p + (size_t)(y0 + r) * src.pitch + x);
+#pragma unroll
+  for (int r = 0; r < PREP_ROWS; r++) {
+    if (y0 + r >= src.h) break;
+    if (map) {
+      uint32_t w[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const uint32_t a = w[i] & 0xffffu, b = w[i] >> 16;
+        const uint32_t lo = (lut_lds && a < 1024) ? lut[a] : j.lmcs_lut[a];
+        const uint32_t hi = (lut_lds && b < 1024) ? lut[b] : j.lmcs_lut[b];
+        w[i] = lo | (hi << 16);
+      }
+      v[r] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    *reinterpret_cast<uint4*>(dst.p + (size_t)(y0 + r) * dst.pitch + x) = v[r];
+  }
 }
 
 // plain copy between buffers (used when a stage is inactive)
@@ -79,19 +97,12 @@ __global__ void __launch_bounds__(256) k_copy(dev_job j, dev_planes from, dev_pl
 // ---------------------------------------------------------------------------
 template <bool VERT>
 __global__ void __launch_bounds__(256) k_dbk_luma(dev_job j) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int seg_local = lane >> 2, line = lane & 3;
-  int gx, gy;
-  if (VERT) {
-    gx = blockIdx.x * 16 + seg_local;  // edge column (x = 4*gx)
-    gy = blockIdx.y * 4 + wave;        // segment row
-  } else {
-    gx = blockIdx.x * 16 + seg_local;  // segment column
-    gy = blockIdx.y * 4 + wave;        // edge row (y = 4*gy)
-  }
-  if (gx >= j.gw || gy >= j.gh) return;  // whole quad leaves together
-  uint32_t e = (VERT ? j.e_lv : j.e_lh)[(size_t)gy * j.gw + gx];
-  if (e == 0) return;
+  // one list entry (a segment with a nonzero descriptor) per quad of lanes
+  const int t = blockIdx.x * 256 + threadIdx.x, line = t & 3, it = t >> 2;
+  if (it >= (VERT ? j.n_lv : j.n_lh)) return;  // whole quad leaves together
+  const uint32_t w = (VERT ? j.l_lv : j.l_lh)[it];
+  const int gx = (int)(w & 0xffff), gy = (int)(w >> 16);
+  const uint32_t e = (VERT ? j.e_lv : j.e_lh)[(size_t)gy * j.gw + gx];
   plane_view v = pick(j.a, j, 0);
   const int x0 = VERT ? 4 * gx : 4 * gx + line;  // q0 position of this line
   const int y0 = VERT ? 4 * gy + line : 4 * gy;
@@ -136,15 +147,13 @@ __global__ void __launch_bounds__(256) k_dbk_luma(dev_job j) {
 // ---------------------------------------------------------------------------
 template <bool VERT>
 __global__ void __launch_bounds__(256) k_dbk_chroma(dev_job j) {
-  const int comp = blockIdx.z + 1;
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  const int line = t & 1;
-  int mw = VERT ? j.cvw : j.chw, mh = VERT ? j.cvh : j.chh;
-  int seg = t >> 1;
-  int sx = seg % mw, sy = seg / mw;
-  if (sy >= mh) return;
-  uint32_t e = (VERT ? j.e_cv[comp - 1] : j.e_ch[comp - 1])[(size_t)sy * mw + sx];
-  if (e == 0) return;
+  // one list entry (a segment with a nonzero descriptor) per pair of lanes
+  const int t = blockIdx.x * 256 + threadIdx.x, line = t & 1, it = t >> 1;
+  if (it >= (VERT ? j.n_cv : j.n_ch)) return;  // whole pair leaves together
+  const uint32_t w = (VERT ? j.l_cv : j.l_ch)[it];
+  const int comp = (int)(w >> 31) + 1, sx = (int)(w & 0xffff), sy = (int)((w >> 16) & 0x7fff);
+  const int mw = VERT ? j.cvw : j.chw;
+  const uint32_t e = (VERT ? j.e_cv[comp - 1] : j.e_ch[comp - 1])[(size_t)sy * mw + sx];
   plane_view v = pick(j.a, j, comp);
   int x0, y0;  // q0 of this line
   if (VERT) { x0 = 8 * sx; y0 = 2 * sy + line; } else { x0 = 2 * sx + line; y0 = 8 * sy; }
@@ -180,52 +189,80 @@ __global__ void __launch_bounds__(256) k_dbk_chroma(dev_job j) {
 __global__ void __launch_bounds__(256) k_sao(dev_job j) {
   const int comp = blockIdx.z;
   plane_view src = pick(j.a, j, comp), dst = pick(j.b, j, comp);
-  int y = blockIdx.y;
-  int x0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  const int y = blockIdx.y;
+  const int x0 = (blockIdx.x * 256 + threadIdx.x) * 8;
   if (y >= src.h || x0 >= src.w) return;
   const int sub = comp ? 1 : 0;
   const int log2c = j.sao_ctb_log2 - sub;
-  bool active = comp == 0 ? j.sao_luma : j.sao_chroma;
+  const bool active = comp == 0 ? j.sao_luma : j.sao_chroma;
   const int maxv = (1 << j.sao_bd) - 1;
   const uint16_t* srow = src.p + (size_t)y * src.pitch;
   uint16_t* drow = dst.p + (size_t)y * dst.pitch;
-  for (int i = 0; i < 4; i++) {
-    int x = x0 + i;
-    if (x >= src.w) break;
-    int pv = srow[x];
-    int out = pv;
-    int rx = x >> log2c, ry = y >> log2c;
-    if (active && rx < j.swctb && ry < j.shctb) {
-      size_t ci = (size_t)ry * j.swctb + rx;
-      uint64_t prm = j.sao[ci * 3 + comp];
-      int type = (int)(prm & 3);
-      if (type == 2) {
-        int h0, v0, h1, v1;
-        eo_offsets((int)((prm >> 2) & 31), h0, v0, h1, v1);
-        int nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
-        bool outside = nx0 < 0 || ny0 < 0 || nx1 < 0 || ny1 < 0 || nx0 >= src.w || ny0 >= src.h || nx1 >= src.w ||
-                       ny1 >= src.h;
-        if (j.sao_rect) {
-          uint64_t r = j.sao_rect[ci];
-          int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff),
-              cy1 = (int)(r >> 48);
-          if (sub) {
-            cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1;
-          }
-          outside = outside || nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
-                    nx1 >= cx1 || ny1 >= cy1;
-        }
-        if (outside) {
-          out = clampi(pv, 0, maxv);
-        } else {
-          int n0 = src.p[(size_t)ny0 * src.pitch + nx0], n1 = src.p[(size_t)ny1 * src.pitch + nx1];
-          out = sao_edge_sample(pv, n0, n1, prm, maxv);
-        }
-      } else if (type == 1) {
-        out = sao_band_sample(pv, prm, j.sao_bd, maxv);
-      }
+  // rows are padded to the pitch (a multiple of 64), so the 8-sample read
+  // stays inside the plane even past its width
+  const uint4 cv = *reinterpret_cast<const uint4*>(srow + x0);
+  const bool full = x0 + 8 <= src.w;
+  // SAO CTBs are at least 8 samples wide, so the chunk lies in one CTB
+  const int rx = x0 >> log2c, ry = y >> log2c;
+  uint64_t prm = 0;
+  size_t ci = 0;
+  if (active && rx < j.swctb && ry < j.shctb) {
+    ci = (size_t)ry * j.swctb + rx;
+    prm = j.sao[ci * 3 + comp];
+  }
+  const int type = (int)(prm & 3);
+  uint32_t ow[4] = {cv.x, cv.y, cv.z, cv.w};
+  if (type == 1) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int pv = (int)((ow[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+      const uint32_t o = (uint32_t)sao_band_sample(pv, prm, j.sao_bd, maxv);
+      ow[i >> 1] = (i & 1) ? ((ow[i >> 1] & 0xffffu) | (o << 16)) : ((ow[i >> 1] & 0xffff0000u) | o);
     }
-    drow[x] = (uint16_t)out;
+  } else if (type == 2) {
+    const int cls = (int)((prm >> 2) & 3);
+    int h0, v0, h1, v1;
+    eo_offsets(cls, h0, v0, h1, v1);
+    // neighbour rows y+v0 and y+v1 (= y-v0), each with one sample either side
+    int A[10], B[10];
+    auto load_row = [&](int yy, int (&R)[10]) {
+      const int yc = clampi(yy, 0, src.h - 1);
+      const uint16_t* r = src.p + (size_t)yc * src.pitch;
+      const uint4 q = *reinterpret_cast<const uint4*>(r + x0);
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int i = 0; i < 8; i++) R[i + 1] = (int)((w[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+      R[0] = x0 > 0 ? r[x0 - 1] : R[1];
+      R[9] = x0 + 8 < src.w ? r[x0 + 8] : R[8];
+    };
+    load_row(y + v0, A);
+    load_row(y + v1, B);
+    // the chunk's neighbourhood inside the picture and the CTB rectangle?
+    int cx0 = 0, cy0 = 0, cx1 = src.w, cy1 = src.h;
+    if (j.sao_rect) {
+      const uint64_t r = j.sao_rect[ci];
+      cx0 = (int)(r & 0xffff); cy0 = (int)((r >> 16) & 0xffff); cx1 = (int)((r >> 32) & 0xffff); cy1 = (int)(r >> 48);
+      if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
+      cx0 = max(cx0, 0); cy0 = max(cy0, 0); cx1 = min(cx1, src.w); cy1 = min(cy1, src.h);
+    }
+    const bool inner = x0 - 1 >= cx0 && x0 + 8 < cx1 && y - 1 >= cy0 && y + 1 < cy1;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int pv = (int)((ow[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+      int o = sao_edge_sample(pv, A[i + 1 + h0], B[i + 1 + h1], prm, maxv);
+      if (!inner) {
+        const int x = x0 + i, nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
+        const bool outside = nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
+                             nx1 >= cx1 || ny1 >= cy1;
+        if (outside) o = clampi(pv, 0, maxv);
+      }
+      ow[i >> 1] = (i & 1) ? ((ow[i >> 1] & 0xffffu) | ((uint32_t)o << 16)) : ((ow[i >> 1] & 0xffff0000u) | (uint32_t)o);
+    }
+  }
+  if (full) {
+    *reinterpret_cast<uint4*>(drow + x0) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+  } else {
+    for (int i = 0; x0 + i < src.w; i++) drow[x0 + i] = (uint16_t)(ow[i >> 1] >> (16 * (i & 1)));
   }
 }
 
@@ -337,64 +374,106 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
 // ALF chroma + CC-ALF: one thread per chroma sample, B -> out.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
+  // two horizontally adjacent samples (x, x+1), x even, per thread
   const int comp = blockIdx.z + 1;
   plane_view src = pick(j.b, j, comp), dst = pick(j.out, j, comp);
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int x = 2 * (blockIdx.x * 64 + (threadIdx.x & 63));
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= src.w || y >= src.h) return;
-  int cur = src.p[(size_t)y * src.pitch + x];
-  int out = cur;
+  // rows are padded to the pitch (a multiple of 64): the pair read stays in
+  // the plane even when x + 1 is past the width
+  const uint16_t* srow = src.p + (size_t)y * src.pitch;
+  uint32_t outp = *reinterpret_cast<const uint32_t*>(srow + x);
   const int ctb = 1 << j.ctb_log2, ctbc = ctb >> 1;
-  const int rx = (2 * x) >> j.ctb_log2, ry = (2 * y) >> j.ctb_log2;
+  const int rx = (2 * x) >> j.ctb_log2, ry = (2 * y) >> j.ctb_log2;  // one CTB for both samples
   if (j.do_alf && rx < j.wctb && ry < j.hctb) {
     const size_t ci = (size_t)ry * j.wctb + rx;
-    uint32_t cw = j.alf_ctb[ci];
-    uint64_t r = j.alf_rect[ci];
+    const uint32_t cw = j.alf_ctb[ci];
+    const uint64_t r = j.alf_rect[ci];
     const int lx0 = (int)(r & 0xffff), ly0 = (int)((r >> 16) & 0xffff), lx1 = (int)((r >> 32) & 0xffff),
               ly1 = (int)(r >> 48);
     const int maxv = (1 << j.alf_bd) - 1;
-    int alt = (int)((cw >> (comp == 1 ? 8 : 16)) & 0xff);
+    const int alt = (int)((cw >> (comp == 1 ? 8 : 16)) & 0xff);
     if (alt != (int)ALF_OFF8) {
       const int cx0 = lx0 >> 1, cy0 = ly0 >> 1, cx1 = ((lx1 + 1) >> 1) - 1, cy1 = ((ly1 + 1) >> 1) - 1;
       const uint32_t* f = j.alf_cbank + alt * 6;
       int sft, y1, y2;
       alf_t46(y - ry * ctbc, ctbc, sft, y1, y2);
-      auto S = [&](int dx, int dy) -> int {
-        return src.p[(size_t)clampi(y + dy, cy0, cy1) * src.pitch + clampi(x + dx, cx0, cx1)];
-      };
-      int c0 = S(0, 0);
-      int sum = 0;
-      sum += alf_pair(f[0], S(0, y2), S(0, -y2), c0);
-      sum += alf_pair(f[1], S(1, y1), S(-1, -y1), c0);
-      sum += alf_pair(f[2], S(0, y1), S(0, -y1), c0);
-      sum += alf_pair(f[3], S(-1, y1), S(1, -y1), c0);
-      sum += alf_pair(f[4], S(2, 0), S(-2, 0), c0);
-      sum += alf_pair(f[5], S(1, 0), S(-1, 0), c0);
-      out = clampi(c0 + ((sum + (1 << (sft - 1))) >> sft), 0, maxv);
+      if (j.alf_bd <= 12 && x - 2 >= cx0 && x + 3 <= cx1 && y - 2 >= cy0 && y + 2 <= cy1) {
+        // neighbourhood inside the rectangle: packed pairs (see the fused form)
+        auto P2 = [&](int dx, int dy) -> uint32_t {
+          return *reinterpret_cast<const uint32_t*>(src.p + (size_t)(y + dy) * src.pitch + x + dx);
+        };
+        const uint32_t m2 = P2(-2, 0), m0 = outp, p2 = P2(2, 0);
+        const uint32_t am2 = P2(-2, y1), a0 = P2(0, y1), ap2 = P2(2, y1);
+        const uint32_t bm2 = P2(-2, -y1), b0 = P2(0, -y1), bp2 = P2(2, -y1);
+        const uint32_t u2 = P2(0, y2), l2 = P2(0, -y2);
+        const s16x2 cur = as_s16x2(m0);
+        int a_0 = 1 << (sft - 1), a_1 = a_0;
+        auto tap = [&](int t, uint32_t a, uint32_t b) {
+          const uint32_t w = f[t], kk = w >> 16;
+          alf_pk_tap(w & 0xffffu, w << 16, kk | ((0u - kk) << 16), cur, a, b, a_0, a_1);
+        };
+        tap(0, u2, l2);
+        tap(1, __builtin_amdgcn_alignbit(ap2, a0, 16), __builtin_amdgcn_alignbit(b0, bm2, 16));
+        tap(2, a0, b0);
+        tap(3, __builtin_amdgcn_alignbit(a0, am2, 16), __builtin_amdgcn_alignbit(bp2, b0, 16));
+        tap(4, p2, m2);
+        tap(5, __builtin_amdgcn_alignbit(p2, m0, 16), __builtin_amdgcn_alignbit(m0, m2, 16));
+        const s16x2 d = __builtin_amdgcn_cvt_pk_i16(a_0 >> sft, a_1 >> sft);
+        const s16x2 zero2 = {0, 0}, vmax2 = {(short)maxv, (short)maxv};
+        outp = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(__builtin_elementwise_add_sat(cur, d), zero2),
+                                                 vmax2));
+      } else {
+        uint32_t o2[2];
+        for (int h = 0; h < 2; h++) {
+          const int xs = x + h;
+          auto S = [&](int dx, int dy) -> int {
+            return src.p[(size_t)clampi(y + dy, cy0, cy1) * src.pitch + clampi(xs + dx, cx0, cx1)];
+          };
+          const int c0 = S(0, 0);
+          int sum = 0;
+          sum += alf_pair(f[0], S(0, y2), S(0, -y2), c0);
+          sum += alf_pair(f[1], S(1, y1), S(-1, -y1), c0);
+          sum += alf_pair(f[2], S(0, y1), S(0, -y1), c0);
+          sum += alf_pair(f[3], S(-1, y1), S(1, -y1), c0);
+          sum += alf_pair(f[4], S(2, 0), S(-2, 0), c0);
+          sum += alf_pair(f[5], S(1, 0), S(-1, 0), c0);
+          o2[h] = (uint32_t)clampi(c0 + ((sum + (1 << (sft - 1))) >> sft), 0, maxv);
+        }
+        outp = o2[0] | (o2[1] << 16);
+      }
     }
-    int ccf = (int)((cw >> (comp == 1 ? 24 : 28)) & 0xf);
+    const int ccf = (int)((cw >> (comp == 1 ? 24 : 28)) & 0xf);
     if (ccf != (int)ALF_OFF4) {
       const int32_t* cf = j.alf_ccbank + (comp - 1) * ALF_CCBANK_WORDS + ccf * 7;
       plane_view lum = pick(j.b, j, 0);
-      const int xl = 2 * x, yl = 2 * y;
+      const int yl = 2 * y;
       const int y_ctb_c = ry * ctbc;
       const int ph = j.h;
       const bool lbb = !(yl + ctb >= ph && (ph - y_ctb_c * 2) <= ctb - 4);
       int yp1, yp2;
       alf_t47((y - y_ctb_c) * 2, ctb, lbb, yp1, yp2);
-      auto L = [&](int dx, int dy) -> int {
-        return lum.p[(size_t)clampi(yl + dy, ly0, ly1 - 1) * lum.pitch + clampi(xl + dx, lx0, lx1 - 1)];
-      };
-      int cen = L(0, 0);
-      int sum = cf[0] * (L(0, -yp1) - cen) + cf[1] * (L(-1, 0) - cen) + cf[2] * (L(1, 0) - cen) +
-                cf[3] * (L(-1, yp1) - cen) + cf[4] * (L(0, yp1) - cen) + cf[5] * (L(1, yp1) - cen) +
-                cf[6] * (L(0, yp2) - cen);
       const int half = 1 << (j.alf_bd - 1);
-      int sc = clampi((sum + 64) >> 7, -half, half - 1);
-      out = clampi(out + sc, 0, maxv);
+      uint32_t o2[2];
+      for (int h = 0; h < 2; h++) {
+        const int xl = 2 * (x + h);
+        auto L = [&](int dx, int dy) -> int {
+          return lum.p[(size_t)clampi(yl + dy, ly0, ly1 - 1) * lum.pitch + clampi(xl + dx, lx0, lx1 - 1)];
+        };
+        const int cen = L(0, 0);
+        const int sum = cf[0] * (L(0, -yp1) - cen) + cf[1] * (L(-1, 0) - cen) + cf[2] * (L(1, 0) - cen) +
+                        cf[3] * (L(-1, yp1) - cen) + cf[4] * (L(0, yp1) - cen) + cf[5] * (L(1, yp1) - cen) +
+                        cf[6] * (L(0, yp2) - cen);
+        const int base = (int)((outp >> (16 * h)) & 0xffffu);
+        o2[h] = (uint32_t)clampi(base + clampi((sum + 64) >> 7, -half, half - 1), 0, maxv);
+      }
+      outp = o2[0] | (o2[1] << 16);
     }
   }
-  dst.p[(size_t)y * dst.pitch + x] = (uint16_t)out;
+  uint16_t* drow = dst.p + (size_t)y * dst.pitch;
+  if (x + 1 < src.w) *reinterpret_cast<uint32_t*>(drow + x) = outp;
+  else drow[x] = (uint16_t)outp;
 }
 
 // ---------------------------------------------------------------------------
@@ -411,18 +490,18 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
   };
   mark(0);
   // prep: in -> A (rows of 2048 samples per block)
-  hipLaunchKernelGGL(k_prep, dim3(cdiv(j.w, 2048), j.h, ncomp), dim3(256), 0, s, j);
+  hipLaunchKernelGGL(k_prep, dim3(cdiv(j.w, 2048), cdiv(j.h, PREP_ROWS), ncomp), dim3(256), 0, s, j);
   mark(1);
-  if (j.do_dbk) hipLaunchKernelGGL(k_dbk_luma<true>, dim3(cdiv(j.gw, 16), cdiv(j.gh, 4)), dim3(256), 0, s, j);
+  if (j.do_dbk && j.n_lv) hipLaunchKernelGGL(k_dbk_luma<true>, dim3(cdiv(4u * j.n_lv, 256)), dim3(256), 0, s, j);
   mark(2);
-  if (j.do_dbk) hipLaunchKernelGGL(k_dbk_luma<false>, dim3(cdiv(j.gw, 16), cdiv(j.gh, 4)), dim3(256), 0, s, j);
+  if (j.do_dbk && j.n_lh) hipLaunchKernelGGL(k_dbk_luma<false>, dim3(cdiv(4u * j.n_lh, 256)), dim3(256), 0, s, j);
   mark(3);
   bool dc = j.do_dbk && j.do_dbk_chroma && ncomp == 3;
-  if (dc) hipLaunchKernelGGL(k_dbk_chroma<true>, dim3(cdiv(2u * j.cvw * j.cvh, 256), 1, 2), dim3(256), 0, s, j);
+  if (dc && j.n_cv) hipLaunchKernelGGL(k_dbk_chroma<true>, dim3(cdiv(2u * j.n_cv, 256)), dim3(256), 0, s, j);
   mark(4);
-  if (dc) hipLaunchKernelGGL(k_dbk_chroma<false>, dim3(cdiv(2u * j.chw * j.chh, 256), 1, 2), dim3(256), 0, s, j);
+  if (dc && j.n_ch) hipLaunchKernelGGL(k_dbk_chroma<false>, dim3(cdiv(2u * j.n_ch, 256)), dim3(256), 0, s, j);
   mark(5);
-  if (j.do_sao) hipLaunchKernelGGL(k_sao, dim3(cdiv(j.w, 1024), j.h, ncomp), dim3(256), 0, s, j);
+  if (j.do_sao) hipLaunchKernelGGL(k_sao, dim3(cdiv(j.w, 2048), j.h, ncomp), dim3(256), 0, s, j);
   mark(6);
   if (j.do_alf) {
     int tl = j.ctb_log2 < 6 ? j.ctb_log2 : 6;
@@ -437,7 +516,7 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
   }
   mark(7);
   if (j.do_alf && ncomp == 3)
-    hipLaunchKernelGGL(k_alf_chroma, dim3(cdiv(j.wc, 64), cdiv(j.hc, 4), 2), dim3(256), 0, s, j);
+    hipLaunchKernelGGL(k_alf_chroma, dim3(cdiv(j.wc, 128), cdiv(j.hc, 4), 2), dim3(256), 0, s, j);
   mark(8);
   return hipGetLastError();
 }

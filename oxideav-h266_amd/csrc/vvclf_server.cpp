@@ -89,6 +89,7 @@ struct request {
   uint32_t stream = 0;
   vvclf_frame* frame = nullptr;
   clk::time_point t_submit, t_done;
+  clk::time_point t_prep, t_ready, t_launch, t_device;  // stage boundaries
   int status = VVCLF_ERR_AGAIN;  // AGAIN while pending
   bool done = false;
   vvclf_job* job = nullptr;
@@ -141,6 +142,7 @@ struct vvclf_server {
   // stats
   uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0, pixels = 0, launches = 0;
   std::vector<double> lat_ms;
+  std::vector<double> stage_ms[5];
   clk::time_point first_submit, last_done;
   bool any_submit = false;
 };
@@ -153,7 +155,9 @@ uint64_t job_bytes_bound(const vvclf_frame& f) {
   uint64_t py = (W + 63) / 64 * 64, wc = f.chroma_format_idc ? f.pic.cb.width : 0, hc = f.chroma_format_idc ? f.pic.cb.height : 0;
   uint64_t pc = (wc + 63) / 64 * 64;
   uint64_t planes = 4 * ((py * H + 2 * pc * hc) * 2 + 256);
-  uint64_t side = ((W / 4 + 2) * (H / 4 + 2) * 8) + ((wc / 2 + 1) * (hc / 2 + 1) * 16) + (W / 32 + 2) * (H / 32 + 2) * 64 + (1 << 20);
+  // deblock maps and their lists of nonzero entries (at most as long)
+  uint64_t side = 2 * (((W / 4 + 2) * (H / 4 + 2) * 8) + ((wc / 2 + 1) * (hc / 2 + 1) * 16)) +
+                  (W / 32 + 2) * (H / 32 + 2) * 64 + (1 << 20);
   return planes + side;
 }
 
@@ -166,7 +170,13 @@ void finish(vvclf_server* s, request* r, int status) {
   if (status == VVCLF_OK) {
     s->completed++;
     s->pixels += (uint64_t)r->frame->pic.luma.width * r->frame->pic.luma.height;
-    s->lat_ms.push_back(std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count());
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    s->lat_ms.push_back(ms(r->t_submit, r->t_done));
+    s->stage_ms[0].push_back(ms(r->t_submit, r->t_prep));
+    s->stage_ms[1].push_back(ms(r->t_prep, r->t_ready));
+    s->stage_ms[2].push_back(ms(r->t_ready, r->t_launch));
+    s->stage_ms[3].push_back(ms(r->t_launch, r->t_device));
+    s->stage_ms[4].push_back(ms(r->t_device, r->t_done));
   } else {
     s->failed++;
   }
@@ -239,6 +249,7 @@ void prep_loop(vvclf_server* s, int idx) {
             s->bytes_peak = std::max(s->bytes_peak, s->bytes_admitted);
             s->admitted++;
             r = c;
+            r->t_prep = clk::now();
             break;
           }
         }
@@ -279,6 +290,7 @@ void prep_loop(vvclf_server* s, int idx) {
       s->cv_prep.notify_all();
       continue;
     }
+    r->t_ready = clk::now();
     s->ready.push_back(r);
     s->cv_launch.notify_one();
   }
@@ -367,6 +379,7 @@ void launch_loop(vvclf_server* s) {
       for (uint32_t i = 0; i < fl.reqs.size(); i++) {
         sl.h_jobs[i] = fl.reqs[i]->job->d;
         fl.reqs[i]->job->fused_out = true;
+        fl.reqs[i]->t_launch = clk::now();
       }
       for (uint32_t oi : order) {
         const vvclf_job* j = fl.reqs[oi]->job;
@@ -411,19 +424,26 @@ void retire_loop(vvclf_server* s) {
       fl = s->running.front();
     }
     hipError_t e = hipEventSynchronize(s->slots[fl.slot].done);
-    std::vector<int> rcs(fl.reqs.size(), VVCLF_OK);
+    const clk::time_point t_dev = clk::now();
+    {
+      // the slot's tables are free once the kernel is done: the next launch
+      // may go while this one's samples come back
+      std::lock_guard<std::mutex> g(s->mu);
+      s->running.pop_front();
+      s->cv_launch.notify_all();
+    }
+    // each job is finished as soon as its own samples are back
     for (size_t i = 0; i < fl.reqs.size(); i++) {
       request* r = fl.reqs[i];
-      if (e != hipSuccess) {
-        rcs[i] = hip_fail(e, "fused kernel");
-        continue;
-      }
+      r->t_device = t_dev;
+      int rc = VVCLF_OK;
+      if (e != hipSuccess) rc = hip_fail(e, "fused kernel");
       vvclf_job* j = r->job;
       vvclf_picture& pic = r->frame->pic;
       uint16_t *y, *cb, *cr;
       uint32_t py, pc;
       vvclf_job_output(j, &y, &cb, &cr, &py, &pc);
-      hipError_t de = hipMemcpy2DAsync(pic.luma.samples, (size_t)pic.luma.stride * 2, y, (size_t)py * 2,
+      hipError_t de = e != hipSuccess ? e : hipMemcpy2DAsync(pic.luma.samples, (size_t)pic.luma.stride * 2, y, (size_t)py * 2,
                                        (size_t)j->w * 2, j->h, hipMemcpyDeviceToHost, s->s_down);
       if (de == hipSuccess && j->wc) {
         de = hipMemcpy2DAsync(pic.cb.samples, (size_t)pic.cb.stride * 2, cb, (size_t)pc * 2, (size_t)j->wc * 2, j->hc,
@@ -433,20 +453,17 @@ void retire_loop(vvclf_server* s) {
                                 j->hc, hipMemcpyDeviceToHost, s->s_down);
       }
       if (de == hipSuccess) de = hipStreamSynchronize(s->s_down);
-      if (de != hipSuccess) rcs[i] = hip_fail(de, "download");
-    }
-    std::lock_guard<std::mutex> g(s->mu);
-    s->running.pop_front();
-    for (size_t i = 0; i < fl.reqs.size(); i++) {
-      request* r = fl.reqs[i];
+      if (e == hipSuccess && de != hipSuccess) rc = hip_fail(de, "download");
       vvclf_job_destroy(r->job);
+      std::lock_guard<std::mutex> g(s->mu);
       r->job = nullptr;
       r->res.reset();
       s->bytes_admitted -= r->bytes;
       s->admitted--;
-      finish(s, r, rcs[i]);
+      finish(s, r, rc);
+      s->cv_prep.notify_all();
     }
-    s->cv_prep.notify_all();
+    std::lock_guard<std::mutex> g(s->mu);
     s->cv_launch.notify_all();
   }
 }
@@ -617,6 +634,12 @@ int vvclf_server_get_stats(vvclf_server* s, vvclf_server_stats* o) {
     o->mean_ms = sum / l.size();
   }
   if (s->any_submit && s->completed) o->wall_s = std::chrono::duration<double>(s->last_done - s->first_submit).count();
+  for (int k = 0; k < 5; k++) {
+    std::vector<double> v = s->stage_ms[k];
+    if (v.empty()) continue;
+    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+    o->stage_p50_ms[k] = v[v.size() / 2];
+  }
   return VVCLF_OK;
 }
 
