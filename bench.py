@@ -117,6 +117,24 @@ def batched_run(vv, ctx, jobs, reps, warmup=2):
             "hbm_algorithmic_GBs": round(CHAIN_BYTES_PER_PX * px / (ms.value / 1e3) / 1e9, 2)}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def traffic_for(kernel):
+    """HBM bytes per dispatch of `kernel` on this workload (FETCH_SIZE + WRITE_SIZE, calibrated), from the
+    rocprofv3 summary in profiles/, or None when not profiled."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for k, v in d.items():
+        name = k.replace("void ", "")
+        if name.startswith(kernel) and (len(name) == len(kernel) or name[len(kernel)] in "<,"):
+            return int(round(v["read_bytes"] + v["write_bytes"]))
+    return None
+
+
 def served_run(vv, ctx, seconds, rate, n_streams=32, seed=7):
     """Poisson arrivals at `rate` pictures/s over `n_streams` streams."""
     sizes = [(832, 480), (1920, 1080), (3840, 2160)]
@@ -181,6 +199,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-mixed64", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-rate", type=float, default=120.0)
     args = ap.parse_args()
@@ -222,7 +241,7 @@ def main():
         "stage_ms_per_picture": {k: round(v, 5) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": traffic_for(KERNEL_NAMES[dom]),
                      "algorithmic_bytes_per_launch": dom_bytes,
                      "avg_launch_ms": round(stage_ms[dom], 5)},
     }
@@ -232,13 +251,30 @@ def main():
         bt["speedup_vs_plain"] = round(bt["value"] / value, 2)
         bt["roofline"] = {"bound": "hbm", "kernel": "k_fused", "achieved": bt["hbm_algorithmic_GBs"],
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(bt["hbm_algorithmic_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                          "frac": round(bt["hbm_algorithmic_GBs"] / HBM_PEAK_GBS, 4),
+                          "traffic": traffic_for("k_fused<%d, %d" % tuple(int(v) for v in bt["tile"].split("x"))),
                           "algorithmic_bytes_per_launch": CHAIN_BYTES_PER_PX * px,
                           "avg_launch_ms": bt["ms_per_launch"]}
         out["batched"] = {"same_8x4k": bt}
     for j in jobs:
         j.close()
     jobs = []
+    if not args.no_dropin:
+        # the drop-in call end to end: host planes in, host planes out (PCIe
+        # and host table derivation included), one picture at a time
+        log("drop-in host-to-host")
+        ts = []
+        for rep in range(2):
+            for f in frames:
+                f.reset()
+                t0 = time.perf_counter()
+                vv.check(vv.lib.vvclf_apply_in_loop_filters(f.desc))
+                ts.append(time.perf_counter() - t0)
+        ts = ts[len(frames):]  # first pass warms the pools and staging buffers
+        out["dropin_host_to_host"] = {"value": round(N * W * H / sum(ts) / 1e6, 2), "unit": "Mpixel/s",
+                                      "ms_per_picture": round(1e3 * sum(ts) / len(ts), 3),
+                                      "note": "vvclf_apply_in_loop_filters per picture: host table derivation, "
+                                              "upload, plain kernels, download"}
     if not args.no_batched and not args.no_mixed64:
         log("batched form, 64 mixed pictures")
         sizes = [(416, 240)] * 16 + [(832, 480)] * 16 + [(1920, 1080)] * 16 + [(3840, 2160)] * 16

@@ -46,7 +46,17 @@ vvclf_ctx* default_ctx() {
   static vvclf_ctx* ctx = nullptr;
   std::lock_guard<std::mutex> g(mu);
   if (!ctx) {
-    if (vvclf_ctx_create(0, &ctx) != VVCLF_OK) ctx = nullptr;
+    if (vvclf_ctx_create(0, &ctx) != VVCLF_OK) {
+      ctx = nullptr;
+    } else {
+      // per-picture jobs allocate from the device's stream-ordered pool;
+      // freed memory stays cached for the next picture
+      hipMemPool_t pool;
+      if (hipDeviceGetDefaultMemPool(&pool, 0) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+      }
+    }
   }
   return ctx;
 }
@@ -234,12 +244,81 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   d.alf_cbank = res ? res->cbank : (const uint32_t*)(sd + o_cbank);
   d.alf_ccbank = res ? res->ccbank : (const int32_t*)(sd + o_ccbank);
   if (keep) *keep = std::move(t);
-  if ((rc = vvclf_job_set_input(j, &f->pic))) {
+  if (opts && opts->stage) {
+    hipError_t ue = upload_staged(j, f->pic, *opts->stage, opts->stream ? opts->stream : j->pool_stream);
+    if (ue != hipSuccess) {
+      vvclf_job_destroy(j);
+      return hip_fail(ue, "upload (staged)");
+    }
+  } else if ((rc = vvclf_job_set_input(j, &f->pic))) {
     vvclf_job_destroy(j);
     return rc;
   }
   *out = j;
   return VVCLF_OK;
+}
+
+bool pinned_buf::ensure(size_t n) {
+  if (n <= cap) return true;
+  release();
+  if (hipHostMalloc(&p, n, 0) != hipSuccess) {
+    p = nullptr;
+    return false;
+  }
+  cap = n;
+  return true;
+}
+
+void pinned_buf::release() {
+  if (p) hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+namespace {
+size_t set_samples(const vvclf_job* j) {
+  return (size_t)j->d.py * j->h + 2 * (size_t)j->d.pc * j->hc;
+}
+}  // namespace
+
+hipError_t upload_staged(vvclf_job* j, const vvclf_picture& pic, pinned_buf& st, hipStream_t s) {
+  if (pic.luma.width != j->w || pic.luma.height != j->h || !pic.luma.samples) return hipErrorInvalidValue;
+  if (j->wc && (!pic.cb.samples || !pic.cr.samples || pic.cb.width != j->wc || pic.cb.height != j->hc))
+    return hipErrorInvalidValue;
+  const size_t n = set_samples(j);
+  if (!st.ensure(n * 2)) return hipErrorOutOfMemory;
+  uint16_t* h = (uint16_t*)st.p;
+  for (uint32_t y = 0; y < j->h; y++)
+    memcpy(h + (size_t)y * j->d.py, pic.luma.samples + (size_t)y * pic.luma.stride, (size_t)j->w * 2);
+  uint16_t* hc = h + (size_t)j->d.py * j->h;
+  for (int c = 0; c < 2 && j->wc; c++) {
+    const vvclf_plane& pl = c ? pic.cr : pic.cb;
+    uint16_t* dst = hc + (size_t)c * j->d.pc * j->hc;
+    for (uint32_t y = 0; y < j->hc; y++)
+      memcpy(dst + (size_t)y * j->d.pc, pl.samples + (size_t)y * pl.stride, (size_t)j->wc * 2);
+  }
+  hipError_t e = hipMemcpyAsync(j->d.in.y, h, n * 2, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
+hipError_t download_staged_issue(vvclf_job* j, pinned_buf& st, hipStream_t s) {
+  const size_t n = set_samples(j);
+  if (!st.ensure(n * 2)) return hipErrorOutOfMemory;
+  return hipMemcpyAsync(st.p, job_final(j, 0), n * 2, hipMemcpyDeviceToHost, s);
+}
+
+void download_staged_finish(const vvclf_job* j, const pinned_buf& st, vvclf_picture& pic) {
+  const uint16_t* h = (const uint16_t*)st.p;
+  for (uint32_t y = 0; y < j->h; y++)
+    memcpy(pic.luma.samples + (size_t)y * pic.luma.stride, h + (size_t)y * j->d.py, (size_t)j->w * 2);
+  const uint16_t* hc = h + (size_t)j->d.py * j->h;
+  for (int c = 0; c < 2 && j->wc; c++) {
+    vvclf_plane& pl = c ? pic.cr : pic.cb;
+    const uint16_t* src = hc + (size_t)c * j->d.pc * j->hc;
+    for (uint32_t y = 0; y < j->hc; y++)
+      memcpy(pl.samples + (size_t)y * pl.stride, src + (size_t)y * j->d.pc, (size_t)j->wc * 2);
+  }
 }
 
 const uint16_t* job_final(const vvclf_job* j, int comp) {
@@ -461,11 +540,19 @@ uint64_t vvclf_job_luma_pixels(const vvclf_job* j) { return j ? (uint64_t)j->w *
 // Drop-in synchronous entry points: upload, run the requested stages, copy the
 // result back into the caller's planes (in place, like `&mut PictureBuffer`).
 // ---------------------------------------------------------------------------
+// The drop-in calls: one picture, host planes in and out.  Samples move
+// through a pinned buffer kept per calling thread (sized to the largest
+// picture seen, held until the process ends); device memory comes from the
+// context stream's pool.
 static int run_sync(vvclf_frame* f, uint32_t stages) {
   vvclf_ctx* ctx = default_ctx();
   if (!ctx) return VVCLF_ERR_HIP;
+  static thread_local pinned_buf stage;
+  job_opts o;
+  o.stream = ctx->stream;
+  o.stage = &stage;
   vvclf_job* j = nullptr;
-  int rc = job_create_stages(ctx, f, stages, &j, nullptr);
+  int rc = job_create_stages(ctx, f, stages, &j, nullptr, &o);
   if (rc) return rc;
   // Nothing to do: leave the planes untouched.
   if (!j->d.do_lmcs && !j->d.do_dbk && !j->d.do_sao && !j->d.do_alf) {
@@ -474,10 +561,11 @@ static int run_sync(vvclf_frame* f, uint32_t stages) {
   }
   rc = vvclf_job_run(j, ctx->stream);
   if (!rc) {
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) rc = hip_fail(e, "stream synchronize");
+    hipError_t e = download_staged_issue(j, stage, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) download_staged_finish(j, stage, f->pic);
+    else rc = hip_fail(e, "download");
   }
-  if (!rc) rc = vvclf_job_download(j, &f->pic);
   vvclf_job_destroy(j);
   return rc;
 }

@@ -332,6 +332,77 @@ __device__ __forceinline__ int sao_band_sample(int p, uint64_t prm, int bd, int 
   return clampi(p + off, 0, maxv);
 }
 
+// ---------------------------------------------------------------------------
+// SAO over runs of 8 samples sharing one CTB (src/sao.rs:504-626).  The four
+// offsets of a CTB component are folded into a table of i8 in a 64-bit word:
+// edge offset T[e] for the raw index e = 2 + sign(p-n0) + sign(p-n1)
+// (remapped 0,1,2,3,4 -> offsets 1,2,-,3,4, index 2 adds nothing), band
+// offset T[k] for k = band - band_position in 0..3.  Valid when every offset
+// fits in i8 (10-bit: |offset| <= 31; 12-bit: <= 124); otherwise the
+// per-sample path above is used.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int med3i(int a, int b, int c) {
+  int r;
+  __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+struct sao_run {
+  int type;      // 0 off, 1 band, 2 edge
+  int aux;       // edge class, or band position
+  uint64_t tab;  // i8 offsets (see above)
+  bool fits;     // all offsets fit in i8
+};
+
+__device__ __forceinline__ sao_run sao_run_of(uint64_t prm) {
+  sao_run r;
+  r.type = (int)(prm & 3);
+  r.aux = (int)((prm >> 2) & 31);
+  const int o1 = sao_offset(1, prm), o2 = sao_offset(2, prm), o3 = sao_offset(3, prm), o4 = sao_offset(4, prm);
+  r.fits = max(max(abs(o1), abs(o2)), max(abs(o3), abs(o4))) <= 127;
+  auto b = [](int v, int k) -> uint64_t { return (uint64_t)(uint8_t)(int8_t)v << (8 * k); };
+  r.tab = r.type == 2 ? (b(o1, 0) | b(o2, 1) | b(o3, 3) | b(o4, 4)) : (b(o1, 0) | b(o2, 1) | b(o3, 2) | b(o4, 3));
+  return r;
+}
+
+// Edge offset of samples x8..x8+7 of row y: A, P, B hold rows y-1, y, y+1 at
+// x8-1 .. x8+8 (index i+1 is sample x8+i).  Neighbours by class (Table 44).
+template <int CLS>
+__device__ __forceinline__ void sao_eo8(const int (&A)[10], const int (&P)[10], const int (&B)[10], uint64_t tab,
+                                        int maxv, int (&out)[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int p = P[i + 1];
+    const int n0 = CLS == 0 ? P[i] : (CLS == 1 ? A[i + 1] : (CLS == 2 ? A[i] : A[i + 2]));
+    const int n1 = CLS == 0 ? P[i + 2] : (CLS == 1 ? B[i + 1] : (CLS == 2 ? B[i + 2] : B[i]));
+    const int e = 2 + med3i(p - n0, -1, 1) + med3i(p - n1, -1, 1);
+    const int off = (int)(int8_t)(uint8_t)(tab >> (8 * e));
+    out[i] = min(max(p + off, 0), maxv);
+  }
+}
+
+__device__ __forceinline__ void sao_eo8_any(int cls, const int (&A)[10], const int (&P)[10], const int (&B)[10],
+                                            uint64_t tab, int maxv, int (&out)[8]) {
+  switch (cls & 3) {
+    case 0: sao_eo8<0>(A, P, B, tab, maxv, out); break;
+    case 1: sao_eo8<1>(A, P, B, tab, maxv, out); break;
+    case 2: sao_eo8<2>(A, P, B, tab, maxv, out); break;
+    default: sao_eo8<3>(A, P, B, tab, maxv, out); break;
+  }
+}
+
+// Band offset of 8 samples P[1..8].
+__device__ __forceinline__ void sao_bo8(const int (&P)[10], int pos, uint64_t tab, int bd, int maxv, int (&out)[8]) {
+  const int sh = bd >= 5 ? bd - 5 : 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int p = P[i + 1];
+    const int k = ((p >> sh) - pos) & 31;
+    const int off = k < 4 ? (int)(int8_t)(uint8_t)(tab >> (8 * k)) : 0;
+    out[i] = min(max(p + off, 0), maxv);
+  }
+}
+
 // Table 44 neighbour offsets per EO class.
 __device__ __forceinline__ void eo_offsets(int cls, int& h0, int& v0, int& h1, int& v1) {
   switch (cls & 3) {

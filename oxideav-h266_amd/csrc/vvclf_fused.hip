@@ -14,6 +14,8 @@
 // samples another edge of that direction uses; the host checks this).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "vvclf_filters.h"
 #include "vvclf_internal.h"
 
@@ -272,10 +274,12 @@ struct sao_tab {
 };
 
 // SAO of one component from window `src` into window `dst` (dst's extent is
-// the region produced, at most RW samples wide).  sub = 1 for chroma.  The
-// region is walked flat (row pitch RW) so that the lanes stay busy; samples
-// whose 8-neighbourhood lies inside the picture and the CTB's rectangle read
-// their edge-offset neighbours directly, the others check each neighbour.
+// the region produced, at most RW samples wide).  sub = 1 for chroma.  Work is
+// split into runs of 8 samples at x = 8k (one CTB per run, SAO CTBs being at
+// least 8 samples wide), walked flat so that the lanes stay busy.  Runs with
+// every offset in i8 use the table form (sao_eo8 / sao_bo8); edge-offset
+// samples whose 8-neighbourhood is not inside the picture and the CTB's
+// rectangle take the per-sample check.
 template <int RW>
 __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)[3], const uint64_t* saor,
                                         const uint64_t (*safe)[3], const sao_tab& tb, const lwin& src, lwin& dst,
@@ -284,41 +288,92 @@ __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)
   const int log2c = j.sao_ctb_log2 - sub;
   const int maxv = (1 << j.sao_bd) - 1;
   const bool has_rect = j.sao_rect != nullptr;
-  const int n = RW * dst.h;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    const int yy = i / RW, xx = i - yy * RW;
-    if (xx >= dst.w) continue;
-    const int x = dst.ox + xx, y = dst.oy + yy;
-    const int sidx = (y - src.oy) * src.pitch + (x - src.ox);  // (x, y) lies inside src
-    const int pv = src.p[sidx];
-    const int cell = ((y >> log2c) - tb.by) * 3 + ((x >> log2c) - tb.bx);
+  const int ka = dst.ox >> 3, nk = ((dst.ox + dst.w - 1) >> 3) - ka + 1;
+  constexpr int NK = (RW + 14) / 8;  // runs per row, at most
+  for (int i = threadIdx.x; i < NK * dst.h; i += 256) {
+    const int yy = i / NK, kk = i - yy * NK;
+    if (kk >= nk) continue;
+    const int y = dst.oy + yy, x8 = (ka + kk) * 8;
+    const int xa = max(x8, dst.ox), xb = min(x8 + 8, dst.ox + dst.w);  // samples produced
+    const int cell = ((y >> log2c) - tb.by) * 3 + ((x8 >> log2c) - tb.bx);
     const uint64_t prm = saop[cell][comp];
-    const uint64_t sr = safe[cell][comp];
-    const int type = (int)(prm & 3), cls = (int)((prm >> 2) & 3);
-    // Table 44 neighbour offsets (the second mirrors the first):
-    // h0 = -1, 0, -1, 1 and v0 = 0, -1, -1, -1 for classes 0..3
-    const int h0 = (int)((0x84u >> (2 * cls)) & 3u) - 1, v0 = cls == 0 ? 0 : -1;
-    const bool inner = x >= (int)(sr & 0xffff) && x < (int)((sr >> 16) & 0xffff) && y >= (int)((sr >> 32) & 0xffff) &&
-                       y < (int)(sr >> 48);
-    const bool fast = type == 2 && inner;
-    const int d = fast ? v0 * src.pitch + h0 : 0;  // in-window neighbours, else a harmless self read
-    const int eo = sao_edge_sample(pv, src.p[sidx + d], src.p[sidx - d], prm, maxv);
-    const int bo = sao_band_sample(pv, prm, j.sao_bd, maxv);
-    int out = type == 1 ? bo : (fast ? eo : pv);
-    if (type == 2 && !inner) {
-      const int nx0 = x + h0, ny0 = y + v0, nx1 = x - h0, ny1 = y - v0;
-      bool outside = nx0 < 0 || ny0 < 0 || nx1 < 0 || ny1 < 0 || nx0 >= pw || ny0 >= ph || nx1 >= pw || ny1 >= ph;
-      if (has_rect) {
-        const uint64_t r = saor[cell];
-        int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff),
-            cy1 = (int)(r >> 48);
-        if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
-        outside = outside || nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
-                  nx1 >= cx1 || ny1 >= cy1;
+    const sao_run run = sao_run_of(prm);
+    // row yr at x8-1 .. x8+8 (direct reads when inside the window, else clamped)
+    const bool direct = x8 - 1 >= src.ox && x8 + 9 <= src.ox + src.w;
+    auto row = [&](int yr, int (&R)[10]) {
+      if (direct && yr >= src.oy && yr < src.oy + src.h) {
+        const uint16_t* q = src.p + (yr - src.oy) * src.pitch + (x8 - src.ox);
+        const uint2 a = *reinterpret_cast<const uint2*>(q), b = *reinterpret_cast<const uint2*>(q + 4);
+        const uint32_t w[4] = {a.x, a.y, b.x, b.y};
+#pragma unroll
+        for (int k = 0; k < 8; k++) R[k + 1] = (int)((w[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+        R[0] = q[-1];
+        R[9] = q[8];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 10; k++) R[k] = src.rd(x8 - 1 + k, yr);
       }
-      out = outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, src.rd(nx0, ny0), src.rd(nx1, ny1), prm, maxv);
+    };
+    int P[10], o[8];
+    row(y, P);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = P[k + 1];
+    if (run.type == 1) {
+      if (run.fits) {
+        sao_bo8(P, run.aux, run.tab, j.sao_bd, maxv, o);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) o[k] = sao_band_sample(P[k + 1], prm, j.sao_bd, maxv);
+      }
+    } else if (run.type == 2) {
+      const int cls = run.aux & 3;
+      int A[10], B[10];
+      if (cls != 0) {
+        row(y - 1, A);
+        row(y + 1, B);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 10; k++) A[k] = B[k] = P[k];
+      }
+      const uint64_t sr = safe[cell][comp];
+      const bool inner = xa >= (int)(sr & 0xffff) && xb <= (int)((sr >> 16) & 0xffff) &&
+                         y >= (int)((sr >> 32) & 0xffff) && y < (int)(sr >> 48);
+      if (inner && run.fits) {
+        sao_eo8_any(cls, A, P, B, run.tab, maxv, o);
+      } else {
+        int h0, v0, h1, v1;
+        eo_offsets(cls, h0, v0, h1, v1);
+        int cx0 = 0, cy0 = 0, cx1 = pw, cy1 = ph;
+        if (has_rect) {
+          const uint64_t r = saor[cell];
+          cx0 = (int)(r & 0xffff); cy0 = (int)((r >> 16) & 0xffff); cx1 = (int)((r >> 32) & 0xffff); cy1 = (int)(r >> 48);
+          if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
+          cx0 = max(cx0, 0); cy0 = max(cy0, 0); cx1 = min(cx1, pw); cy1 = min(cy1, ph);
+        }
+        auto slow = [&](auto cls_tag) {
+          constexpr int CLS = decltype(cls_tag)::value;
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            const int pv = P[k + 1];
+            const int n0 = CLS == 0 ? P[k] : (CLS == 1 ? A[k + 1] : (CLS == 2 ? A[k] : A[k + 2]));
+            const int n1 = CLS == 0 ? P[k + 2] : (CLS == 1 ? B[k + 1] : (CLS == 2 ? B[k + 2] : B[k]));
+            const int x = x8 + k, nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
+            const bool outside = nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
+                                 nx1 >= cx1 || ny1 >= cy1;
+            o[k] = outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, n0, n1, prm, maxv);
+          }
+        };
+        switch (cls) {
+          case 0: slow(std::integral_constant<int, 0>{}); break;
+          case 1: slow(std::integral_constant<int, 1>{}); break;
+          case 2: slow(std::integral_constant<int, 2>{}); break;
+          default: slow(std::integral_constant<int, 3>{}); break;
+        }
+      }
     }
-    dst.wr(x, y, out);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (x8 + k >= xa && x8 + k < xb) dst.wr(x8 + k, y, o[k]);
   }
 }
 

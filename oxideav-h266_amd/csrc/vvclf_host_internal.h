@@ -57,10 +57,29 @@ struct resident_tables {
   const int32_t* ccbank = nullptr;   // CC banks
 };
 
+// Pinned host staging buffer, grown on demand; the owner frees it.
+struct pinned_buf {
+  void* p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t n);
+  void release();
+};
+
 struct job_opts {
   hipStream_t stream = nullptr;  // stream-ordered allocation and copies (memory pool)
   const resident_tables* res = nullptr;
+  pinned_buf* stage = nullptr;   // upload through this pinned buffer
 };
+
+// Host planes <-> a job's contiguous sample set (luma, Cb, Cr at the device
+// pitches) through a pinned buffer: CPU row copies plus one linear DMA.
+// upload_staged returns after the DMA completed (the buffer may be reused);
+// download_staged_issue only queues the DMA into `st`, download_staged_finish
+// copies the rows out once `s` has reached it.
+const uint16_t* job_final(const vvclf_job* j, int comp);  // the final sample set's plane
+hipError_t upload_staged(vvclf_job* j, const vvclf_picture& pic, pinned_buf& st, hipStream_t s);
+hipError_t download_staged_issue(vvclf_job* j, pinned_buf& st, hipStream_t s);
+void download_staged_finish(const vvclf_job* j, const pinned_buf& st, vvclf_picture& pic);
 
 int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvclf_job** out, uint64_t* dry_stats,
                       const job_opts* opts = nullptr, host_tables* keep = nullptr);

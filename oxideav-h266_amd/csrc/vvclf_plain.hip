@@ -12,6 +12,8 @@
 // all segments of a pass run concurrently with the sequential result.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "vvclf_filters.h"
 #include "vvclf_internal.h"
 
@@ -191,17 +193,23 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
   plane_view src = pick(j.a, j, comp), dst = pick(j.b, j, comp);
   const int y = blockIdx.y;
   const int x0 = (blockIdx.x * 256 + threadIdx.x) * 8;
-  if (y >= src.h || x0 >= src.w) return;
+  if (y >= src.h || x0 >= src.w) return;  // lanes leave in x order: a lane's right neighbour lane is
+                                          // present whenever x0 + 8 is inside the picture
   const int sub = comp ? 1 : 0;
   const int log2c = j.sao_ctb_log2 - sub;
   const bool active = comp == 0 ? j.sao_luma : j.sao_chroma;
   const int maxv = (1 << j.sao_bd) - 1;
-  const uint16_t* srow = src.p + (size_t)y * src.pitch;
   uint16_t* drow = dst.p + (size_t)y * dst.pitch;
-  // rows are padded to the pitch (a multiple of 64), so the 8-sample read
-  // stays inside the plane even past its width
-  const uint4 cv = *reinterpret_cast<const uint4*>(srow + x0);
-  const bool full = x0 + 8 <= src.w;
+  // Rows y-1, y, y+1 (clamped) as 8-sample reads, all issued with the
+  // parameter read; rows are padded to the pitch (a multiple of 64), so the
+  // reads stay inside the plane past its width.  Edge-offset classes 1..3
+  // use the outer rows; adjacent rows' workgroups share them in L2.
+  const uint16_t* ra = src.p + (size_t)max(y - 1, 0) * src.pitch;
+  const uint16_t* rp = src.p + (size_t)y * src.pitch;
+  const uint16_t* rb = src.p + (size_t)min(y + 1, src.h - 1) * src.pitch;
+  const uint4 qa = *reinterpret_cast<const uint4*>(ra + x0);
+  const uint4 qp = *reinterpret_cast<const uint4*>(rp + x0);
+  const uint4 qb = *reinterpret_cast<const uint4*>(rb + x0);
   // SAO CTBs are at least 8 samples wide, so the chunk lies in one CTB
   const int rx = x0 >> log2c, ry = y >> log2c;
   uint64_t prm = 0;
@@ -210,33 +218,38 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
     ci = (size_t)ry * j.swctb + rx;
     prm = j.sao[ci * 3 + comp];
   }
-  const int type = (int)(prm & 3);
-  uint32_t ow[4] = {cv.x, cv.y, cv.z, cv.w};
-  if (type == 1) {
+  // samples x0-1 and x0+8 of each row: from the neighbouring lanes (which
+  // hold the adjacent chunks), read directly at the wave's ends
+  const int lane = threadIdx.x & 63;
+  auto unpack = [&](const uint4& q, const uint16_t* r, int (&R)[10]) {
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int pv = (int)((ow[i >> 1] >> (16 * (i & 1))) & 0xffffu);
-      const uint32_t o = (uint32_t)sao_band_sample(pv, prm, j.sao_bd, maxv);
-      ow[i >> 1] = (i & 1) ? ((ow[i >> 1] & 0xffffu) | (o << 16)) : ((ow[i >> 1] & 0xffff0000u) | o);
+    for (int i = 0; i < 8; i++) R[i + 1] = (int)((w[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+    int left = __shfl_up(R[8], 1), right = __shfl_down(R[1], 1);
+    if (lane == 0) left = x0 > 0 ? r[x0 - 1] : R[1];
+    if (lane == 63 || x0 + 8 >= src.w) right = x0 + 8 < src.w ? r[x0 + 8] : R[8];
+    R[0] = x0 > 0 ? left : R[1];
+    R[9] = right;
+  };
+  int A[10], P[10], B[10];
+  unpack(qa, ra, A);
+  unpack(qp, rp, P);
+  unpack(qb, rb, B);
+  const sao_run run = sao_run_of(prm);
+  int o[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) o[i] = P[i + 1];
+  if (run.type == 1) {
+    if (run.fits) {
+      sao_bo8(P, run.aux, run.tab, j.sao_bd, maxv, o);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) o[i] = sao_band_sample(P[i + 1], prm, j.sao_bd, maxv);
     }
-  } else if (type == 2) {
-    const int cls = (int)((prm >> 2) & 3);
+  } else if (run.type == 2) {
+    const int cls = run.aux & 3;
     int h0, v0, h1, v1;
     eo_offsets(cls, h0, v0, h1, v1);
-    // neighbour rows y+v0 and y+v1 (= y-v0), each with one sample either side
-    int A[10], B[10];
-    auto load_row = [&](int yy, int (&R)[10]) {
-      const int yc = clampi(yy, 0, src.h - 1);
-      const uint16_t* r = src.p + (size_t)yc * src.pitch;
-      const uint4 q = *reinterpret_cast<const uint4*>(r + x0);
-      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-      for (int i = 0; i < 8; i++) R[i + 1] = (int)((w[i >> 1] >> (16 * (i & 1))) & 0xffffu);
-      R[0] = x0 > 0 ? r[x0 - 1] : R[1];
-      R[9] = x0 + 8 < src.w ? r[x0 + 8] : R[8];
-    };
-    load_row(y + v0, A);
-    load_row(y + v1, B);
     // the chunk's neighbourhood inside the picture and the CTB rectangle?
     int cx0 = 0, cy0 = 0, cx1 = src.w, cy1 = src.h;
     if (j.sao_rect) {
@@ -246,23 +259,40 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
       cx0 = max(cx0, 0); cy0 = max(cy0, 0); cx1 = min(cx1, src.w); cy1 = min(cy1, src.h);
     }
     const bool inner = x0 - 1 >= cx0 && x0 + 8 < cx1 && y - 1 >= cy0 && y + 1 < cy1;
+    if (inner && run.fits) {
+      sao_eo8_any(cls, A, P, B, run.tab, maxv, o);
+    } else {
+      // per sample, with the picture / rectangle check of each neighbour
+      auto slow = [&](auto cls_tag) {
+        constexpr int CLS = decltype(cls_tag)::value;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int pv = (int)((ow[i >> 1] >> (16 * (i & 1))) & 0xffffu);
-      int o = sao_edge_sample(pv, A[i + 1 + h0], B[i + 1 + h1], prm, maxv);
-      if (!inner) {
-        const int x = x0 + i, nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
-        const bool outside = nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
-                             nx1 >= cx1 || ny1 >= cy1;
-        if (outside) o = clampi(pv, 0, maxv);
+        for (int i = 0; i < 8; i++) {
+          const int pv = P[i + 1];
+          const int n0 = CLS == 0 ? P[i] : (CLS == 1 ? A[i + 1] : (CLS == 2 ? A[i] : A[i + 2]));
+          const int n1 = CLS == 0 ? P[i + 2] : (CLS == 1 ? B[i + 1] : (CLS == 2 ? B[i + 2] : B[i]));
+          const int x = x0 + i, nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
+          const bool outside = nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
+                               nx1 >= cx1 || ny1 >= cy1;
+          o[i] = outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, n0, n1, prm, maxv);
+        }
+      };
+      switch (cls) {
+        case 0: slow(std::integral_constant<int, 0>{}); break;
+        case 1: slow(std::integral_constant<int, 1>{}); break;
+        case 2: slow(std::integral_constant<int, 2>{}); break;
+        default: slow(std::integral_constant<int, 3>{}); break;
       }
-      ow[i >> 1] = (i & 1) ? ((ow[i >> 1] & 0xffffu) | ((uint32_t)o << 16)) : ((ow[i >> 1] & 0xffff0000u) | (uint32_t)o);
     }
   }
-  if (full) {
+  if (x0 + 8 <= src.w) {
+    uint32_t ow[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) ow[q] = (uint32_t)o[2 * q] | ((uint32_t)o[2 * q + 1] << 16);
     *reinterpret_cast<uint4*>(drow + x0) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
   } else {
-    for (int i = 0; x0 + i < src.w; i++) drow[x0 + i] = (uint16_t)(ow[i >> 1] >> (16 * (i & 1)));
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      if (x0 + i < src.w) drow[x0 + i] = (uint16_t)o[i];
   }
 }
 

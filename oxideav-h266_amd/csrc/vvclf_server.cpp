@@ -136,6 +136,11 @@ struct vvclf_server {
   bool stop = false;
   std::vector<std::thread> threads;
   std::vector<hipStream_t> prep_streams;
+  // pinned staging: one buffer per prep thread (uploads), two for the
+  // retire thread (downloads, double-buffered)
+  std::vector<pinned_buf> prep_stage;
+  pinned_buf down_stage[2];
+  hipEvent_t down_done[2] = {nullptr, nullptr};
   hipStream_t s_comp = nullptr, s_down = nullptr;
   launch_slot slots[2];
   int next_slot = 0;
@@ -274,6 +279,7 @@ void prep_loop(vvclf_server* s, int idx) {
       job_opts o;
       o.stream = ps;
       o.res = &r->res->t;
+      o.stage = &s->prep_stage[idx];
       rc = job_create_stages(s->ctx, r->frame, STAGE_ALL, &r->job, nullptr, &o);
     }
     if (!rc && !job_fits_shape(r->job->d, 32, 32)) {  // 32x32 fits every job any shape fits
@@ -432,36 +438,38 @@ void retire_loop(vvclf_server* s) {
       s->running.pop_front();
       s->cv_launch.notify_all();
     }
-    // each job is finished as soon as its own samples are back
-    for (size_t i = 0; i < fl.reqs.size(); i++) {
+    // each job is finished as soon as its own samples are back: the DMA of
+    // job i+1 into one pinned buffer overlaps the row copies of job i out of
+    // the other
+    const size_t n = fl.reqs.size();
+    std::vector<int> rc(n, e == hipSuccess ? VVCLF_OK : hip_fail(e, "fused kernel"));
+    auto issue = [&](size_t i) {
+      if (rc[i] != VVCLF_OK) return;
+      hipError_t de = download_staged_issue(fl.reqs[i]->job, s->down_stage[i & 1], s->s_down);
+      if (de == hipSuccess) de = hipEventRecord(s->down_done[i & 1], s->s_down);
+      if (de != hipSuccess) rc[i] = hip_fail(de, "download");
+    };
+    auto complete = [&](size_t i) {
       request* r = fl.reqs[i];
       r->t_device = t_dev;
-      int rc = VVCLF_OK;
-      if (e != hipSuccess) rc = hip_fail(e, "fused kernel");
-      vvclf_job* j = r->job;
-      vvclf_picture& pic = r->frame->pic;
-      uint16_t *y, *cb, *cr;
-      uint32_t py, pc;
-      vvclf_job_output(j, &y, &cb, &cr, &py, &pc);
-      hipError_t de = e != hipSuccess ? e : hipMemcpy2DAsync(pic.luma.samples, (size_t)pic.luma.stride * 2, y, (size_t)py * 2,
-                                       (size_t)j->w * 2, j->h, hipMemcpyDeviceToHost, s->s_down);
-      if (de == hipSuccess && j->wc) {
-        de = hipMemcpy2DAsync(pic.cb.samples, (size_t)pic.cb.stride * 2, cb, (size_t)pc * 2, (size_t)j->wc * 2, j->hc,
-                              hipMemcpyDeviceToHost, s->s_down);
-        if (de == hipSuccess)
-          de = hipMemcpy2DAsync(pic.cr.samples, (size_t)pic.cr.stride * 2, cr, (size_t)pc * 2, (size_t)j->wc * 2,
-                                j->hc, hipMemcpyDeviceToHost, s->s_down);
+      if (rc[i] == VVCLF_OK) {
+        hipError_t de = hipEventSynchronize(s->down_done[i & 1]);
+        if (de == hipSuccess) download_staged_finish(r->job, s->down_stage[i & 1], r->frame->pic);
+        else rc[i] = hip_fail(de, "download");
       }
-      if (de == hipSuccess) de = hipStreamSynchronize(s->s_down);
-      if (e == hipSuccess && de != hipSuccess) rc = hip_fail(de, "download");
       vvclf_job_destroy(r->job);
       std::lock_guard<std::mutex> g(s->mu);
       r->job = nullptr;
       r->res.reset();
       s->bytes_admitted -= r->bytes;
       s->admitted--;
-      finish(s, r, rc);
+      finish(s, r, rc[i]);
       s->cv_prep.notify_all();
+    };
+    if (n) issue(0);
+    for (size_t i = 0; i < n; i++) {
+      if (i + 1 < n) issue(i + 1);
+      complete(i);
     }
     std::lock_guard<std::mutex> g(s->mu);
     s->cv_launch.notify_all();
@@ -500,7 +508,9 @@ int vvclf_server_create(vvclf_ctx* ctx, const vvclf_server_config* cfg, vvclf_se
   hipError_t e = hipStreamCreateWithFlags(&s->s_comp, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_down, hipStreamNonBlocking);
   for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&s->slots[i].done, hipEventDisableTiming);
+  for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&s->down_done[i], hipEventDisableTiming);
   s->prep_streams.resize(s->cfg.prep_threads);
+  s->prep_stage.resize(s->cfg.prep_threads);
   for (auto& ps : s->prep_streams)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ps, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -535,6 +545,11 @@ int vvclf_server_destroy(vvclf_server* s) {
     if (sl.done) hipEventDestroy(sl.done);
   }
   hipDeviceSynchronize();
+  for (auto& b : s->prep_stage) b.release();
+  for (int k = 0; k < 2; k++) {
+    s->down_stage[k].release();
+    if (s->down_done[k]) hipEventDestroy(s->down_done[k]);
+  }
   for (auto ps : s->prep_streams) hipStreamDestroy(ps);
   hipStreamDestroy(s->s_comp);
   hipStreamDestroy(s->s_down);
