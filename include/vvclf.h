@@ -352,12 +352,15 @@ int vvclf_job_set_input(vvclf_job* job, const vvclf_picture* pic);
 /* Plain per-picture form: separate stage kernels on `stream` (hipStream_t,
  * NULL = default stream).  Asynchronous. */
 int vvclf_job_run(vvclf_job* job, void* stream);
+/* The same per-picture kernel sequence, captured into a HIP graph on the
+ * first call and replayed from then on (fewer launch gaps). */
+int vvclf_job_graph_launch(vvclf_job* job, void* stream);
 /* Stage timing for the plain form: a pool of HIP events for up to
  * `max_runs` runs.  vvclf_job_run_prof records events between the stage
  * kernels on the run's stream (no synchronisation); vvclf_prof_stage_ms waits
  * for the recorded runs and returns, per stage, the summed milliseconds:
- * [prep, deblock luma V, deblock luma H, deblock chroma V, deblock chroma H,
- *  SAO, ALF luma, ALF chroma + CC-ALF]. */
+ * [prep, deblock V (luma + chroma), deblock H (luma + chroma), SAO, ALF luma,
+ *  ALF chroma + CC-ALF, 0, 0]. */
 typedef struct vvclf_prof vvclf_prof;
 int vvclf_prof_create(vvclf_ctx* ctx, uint32_t max_runs, vvclf_prof** out);
 int vvclf_prof_destroy(vvclf_prof* prof);

@@ -220,6 +220,7 @@ SIGNATURES = {
     "vvclf_job_destroy": (C.c_int, [VP]),
     "vvclf_job_set_input": (C.c_int, [VP, P(Picture)]),
     "vvclf_job_run": (C.c_int, [VP, VP]),
+    "vvclf_job_graph_launch": (C.c_int, [VP, VP]),
     "vvclf_job_download": (C.c_int, [VP, P(Picture)]),
     "vvclf_job_output": (C.c_int, [VP, P(VP), P(VP), P(VP), P(u32), P(u32)]),
     "vvclf_job_device_bytes": (u64, [VP]),
@@ -395,6 +396,9 @@ class Job:
 
     def run(self, stream=None):
         check(lib.vvclf_job_run(self.h, stream))
+
+    def graph_launch(self, stream=None):
+        check(lib.vvclf_job_graph_launch(self.h, stream))
 
     def download(self, pic_ptr):
         check(lib.vvclf_job_download(self.h, pic_ptr))

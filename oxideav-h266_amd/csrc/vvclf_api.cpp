@@ -393,6 +393,8 @@ int vvclf_frame_check(const vvclf_frame* frame, uint64_t stats[8]) {
 int vvclf_job_destroy(vvclf_job* j) {
   if (!j) return VVCLF_OK;
   hipSetDevice(j->ctx->device);
+  if (j->exec) hipGraphExecDestroy(j->exec);
+  if (j->graph) hipGraphDestroy(j->graph);
   if (j->pool_stream) {  // back to the stream-ordered pool, no device-wide sync
     if (j->dev_planes) hipFreeAsync(j->dev_planes, j->pool_stream);
     if (j->dev_side) hipFreeAsync(j->dev_side, j->pool_stream);
@@ -435,6 +437,31 @@ int vvclf_job_run(vvclf_job* j, void* stream) {
   hipSetDevice(j->ctx->device);
   hipError_t e = launch_plain(j->d, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "launch");
+  j->fused_out = false;
+  return VVCLF_OK;
+}
+
+int vvclf_job_graph_launch(vvclf_job* j, void* stream) {
+  if (!j) return fail(VVCLF_ERR_INVALID, "job_graph_launch: NULL job");
+  hipSetDevice(j->ctx->device);
+  hipError_t e;
+  if (!j->exec) {
+    // capture on a private stream so the caller's stream may be the legacy one
+    hipStream_t cs;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "graph stream");
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipError_t le = launch_plain(j->d, cs);
+      e = hipStreamEndCapture(cs, &j->graph);
+      if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&j->exec, j->graph, nullptr, nullptr, 0);
+    hipStreamDestroy(cs);
+    if (e != hipSuccess) return hip_fail(e, "graph capture");
+  }
+  e = hipGraphLaunch(j->exec, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   j->fused_out = false;
   return VVCLF_OK;
 }
@@ -517,7 +544,7 @@ int vvclf_job_run_prof(vvclf_job* j, void* stream, vvclf_prof* p) {
 int vvclf_prof_stage_ms(vvclf_prof* p, float ms[8], uint32_t* runs) {
   if (!p || !ms) return fail(VVCLF_ERR_INVALID, "prof_stage_ms: NULL argument");
   hipSetDevice(p->device);
-  for (int s = 0; s < PLAIN_STAGES; s++) ms[s] = 0.f;
+  for (int s = 0; s < 8; s++) ms[s] = 0.f;
   for (uint32_t r = 0; r < p->used; r++) {
     hipEvent_t* ev = &p->ev[(size_t)r * (PLAIN_STAGES + 1)];
     hipError_t e = hipEventSynchronize(ev[PLAIN_STAGES]);

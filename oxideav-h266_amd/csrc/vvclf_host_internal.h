@@ -47,7 +47,7 @@ int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t,
 void build_lmcs_lut(const vvclf_lmcs_binding& b, std::vector<uint16_t>& lut);
 void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t);
 
-static constexpr int PLAIN_STAGES = 8;
+static constexpr int PLAIN_STAGES = 6;  // prep, deblock V, deblock H, SAO, ALF luma, ALF chroma
 // Device tables shared by many jobs (a serving stream's resident LMCS / ALF
 // state); a job built with them uploads only its own per-picture tables.
 struct resident_tables {
@@ -132,4 +132,6 @@ struct vvclf_job {
   uint32_t stages;
   bool fused_out = false;  // last run was the fused form (result in `out`)
   hipStream_t pool_stream = nullptr;  // set when allocated from the stream-ordered pool
+  hipGraph_t graph = nullptr;         // the plain sequence, captured on first graph launch
+  hipGraphExec_t exec = nullptr;
 };

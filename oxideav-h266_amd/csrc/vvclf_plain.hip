@@ -98,9 +98,9 @@ __global__ void __launch_bounds__(256) k_copy(dev_job j, dev_planes from, dev_pl
 // 3.  Block = 4 waves; each wave covers 16 segments.
 // ---------------------------------------------------------------------------
 template <bool VERT>
-__global__ void __launch_bounds__(256) k_dbk_luma(dev_job j) {
+__device__ __forceinline__ void dbk_luma_lane(const dev_job& j, int t) {
   // one list entry (a segment with a nonzero descriptor) per quad of lanes
-  const int t = blockIdx.x * 256 + threadIdx.x, line = t & 3, it = t >> 2;
+  const int line = t & 3, it = t >> 2;
   if (it >= (VERT ? j.n_lv : j.n_lh)) return;  // whole quad leaves together
   const uint32_t w = (VERT ? j.l_lv : j.l_lh)[it];
   const int gx = (int)(w & 0xffff), gy = (int)(w >> 16);
@@ -148,9 +148,9 @@ __global__ void __launch_bounds__(256) k_dbk_luma(dev_job j) {
 // lanes 2k, 2k+1.  blockIdx.z = Cb / Cr.
 // ---------------------------------------------------------------------------
 template <bool VERT>
-__global__ void __launch_bounds__(256) k_dbk_chroma(dev_job j) {
+__device__ __forceinline__ void dbk_chroma_lane(const dev_job& j, int t) {
   // one list entry (a segment with a nonzero descriptor) per pair of lanes
-  const int t = blockIdx.x * 256 + threadIdx.x, line = t & 1, it = t >> 1;
+  const int line = t & 1, it = t >> 1;
   if (it >= (VERT ? j.n_cv : j.n_ch)) return;  // whole pair leaves together
   const uint32_t w = (VERT ? j.l_cv : j.l_ch)[it];
   const int comp = (int)(w >> 31) + 1, sx = (int)(w & 0xffff), sy = (int)((w >> 16) & 0x7fff);
@@ -183,6 +183,14 @@ __global__ void __launch_bounds__(256) k_dbk_chroma(dev_job j) {
     if (x < 0 || y < 0 || x >= v.w || y >= v.h) continue;
     v.p[(size_t)y * v.pitch + x] = (uint16_t)clampi(n[i], 0, maxv);
   }
+}
+
+// One launch per direction: workgroups [0, luma_blocks) filter luma
+// segments, the rest chroma segments (the planes are independent).
+template <bool VERT>
+__global__ void __launch_bounds__(256) k_dbk(dev_job j, int luma_blocks) {
+  if ((int)blockIdx.x < luma_blocks) dbk_luma_lane<VERT>(j, blockIdx.x * 256 + threadIdx.x);
+  else dbk_chroma_lane<VERT>(j, (blockIdx.x - luma_blocks) * 256 + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -522,17 +530,21 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
   // prep: in -> A (rows of 2048 samples per block)
   hipLaunchKernelGGL(k_prep, dim3(cdiv(j.w, 2048), cdiv(j.h, PREP_ROWS), ncomp), dim3(256), 0, s, j);
   mark(1);
-  if (j.do_dbk && j.n_lv) hipLaunchKernelGGL(k_dbk_luma<true>, dim3(cdiv(4u * j.n_lv, 256)), dim3(256), 0, s, j);
+  // deblocking: all vertical edges (luma and chroma in one launch), then all
+  // horizontal edges
+  const bool dc = j.do_dbk && j.do_dbk_chroma && ncomp == 3;
+  {
+    const unsigned lb = j.do_dbk ? cdiv(4u * j.n_lv, 256) : 0, cb = dc ? cdiv(2u * j.n_cv, 256) : 0;
+    if (lb + cb) hipLaunchKernelGGL(k_dbk<true>, dim3(lb + cb), dim3(256), 0, s, j, (int)lb);
+  }
   mark(2);
-  if (j.do_dbk && j.n_lh) hipLaunchKernelGGL(k_dbk_luma<false>, dim3(cdiv(4u * j.n_lh, 256)), dim3(256), 0, s, j);
+  {
+    const unsigned lb = j.do_dbk ? cdiv(4u * j.n_lh, 256) : 0, cb = dc ? cdiv(2u * j.n_ch, 256) : 0;
+    if (lb + cb) hipLaunchKernelGGL(k_dbk<false>, dim3(lb + cb), dim3(256), 0, s, j, (int)lb);
+  }
   mark(3);
-  bool dc = j.do_dbk && j.do_dbk_chroma && ncomp == 3;
-  if (dc && j.n_cv) hipLaunchKernelGGL(k_dbk_chroma<true>, dim3(cdiv(2u * j.n_cv, 256)), dim3(256), 0, s, j);
-  mark(4);
-  if (dc && j.n_ch) hipLaunchKernelGGL(k_dbk_chroma<false>, dim3(cdiv(2u * j.n_ch, 256)), dim3(256), 0, s, j);
-  mark(5);
   if (j.do_sao) hipLaunchKernelGGL(k_sao, dim3(cdiv(j.w, 2048), j.h, ncomp), dim3(256), 0, s, j);
-  mark(6);
+  mark(4);
   if (j.do_alf) {
     int tl = j.ctb_log2 < 6 ? j.ctb_log2 : 6;
     int tile = 1 << tl;
@@ -544,10 +556,10 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
       hipLaunchKernelGGL(k_alf_luma<false>, dim3(cdiv(j.w, tile), cdiv(j.h, tile)), dim3(threads < 64 ? 64 : threads),
                          0, s, j, tl);
   }
-  mark(7);
+  mark(5);
   if (j.do_alf && ncomp == 3)
     hipLaunchKernelGGL(k_alf_chroma, dim3(cdiv(j.wc, 128), cdiv(j.hc, 4), 2), dim3(256), 0, s, j);
-  mark(8);
+  mark(6);
   return hipGetLastError();
 }
 

@@ -141,6 +141,11 @@ def test_job_runs_are_repeatable(vv, orc):
         job.run()
         job.download(C.byref(f.desc.contents.pic))
         outs.append(f.output_arrays())
+    # the captured-graph form of the same sequence, replayed
+    for _ in range(3):
+        job.graph_launch()
+        job.download(C.byref(f.desc.contents.pic))
+        outs.append(f.output_arrays())
     for o in outs:
         assert_same(fo.output_arrays(), o, "job run")
     job.close()
