@@ -243,6 +243,8 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   d.alf_bank = res ? res->bank : (const uint32_t*)(sd + o_bank);
   d.alf_cbank = res ? res->cbank : (const uint32_t*)(sd + o_cbank);
   d.alf_ccbank = res ? res->ccbank : (const int32_t*)(sd + o_ccbank);
+  j->h_alf_ctb = t.alf_ctb;
+  j->h_alf_rect = t.alf_rect;
   if (keep) *keep = std::move(t);
   if (opts && opts->stage) {
     hipError_t ue = upload_staged(j, f->pic, *opts->stage, opts->stream ? opts->stream : j->pool_stream);

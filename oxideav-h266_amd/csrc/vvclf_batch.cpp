@@ -88,7 +88,7 @@ int vvclf_batch_create(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, 
   for (uint32_t oi : order) {
     const vvclf_job* j = js[oi];
     for (uint32_t y = 0; y < j->h; y += t.tile_h)
-      for (uint32_t x = 0; x < j->w; x += t.tile_w) tiles.push_back(tile_ref{oi, (uint16_t)x, (uint16_t)y});
+      for (uint32_t x = 0; x < j->w; x += t.tile_w) tiles.push_back(make_tile(j, oi, x, y));
   }
   std::vector<dev_job> dj(n_jobs);
   for (uint32_t i = 0; i < n_jobs; i++) dj[i] = js[i]->d;

@@ -13,6 +13,39 @@ namespace vvclf {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Loads and stores through a pointer known to address global memory (HBM).
+// A pointer read from memory (the job records of the batched kernel) is
+// generic to the compiler, which then emits flat accesses; those also count
+// against the LDS / scalar-load counter, so that every wait for an LDS read
+// or a scalar load would also wait for all HBM traffic in flight.
+#define VVCLF_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const VVCLF_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(VVCLF_GLOBAL T*)p = v;
+}
+// uint2 / uint4 are classes whose copy constructor takes a generic reference,
+// so they move as native vectors here
+typedef uint32_t vu32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t vu32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 gld(const uint2* p) {
+  const vu32x2 v = *(const VVCLF_GLOBAL vu32x2*)p;
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 gld(const uint4* p) {
+  const vu32x4 v = *(const VVCLF_GLOBAL vu32x4*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(uint2* p, uint2 v) {
+  *(VVCLF_GLOBAL vu32x2*)p = vu32x2{v.x, v.y};
+}
+__device__ __forceinline__ void gst(uint4* p, uint4 v) {
+  *(VVCLF_GLOBAL vu32x4*)p = vu32x4{v.x, v.y, v.z, v.w};
+}
+
 // Table 43 (spec data): beta' and tC' indexed by the clamped Q.
 __constant__ static const int16_t kBeta[64] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,
                                               6,  7,  8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24,

@@ -41,8 +41,9 @@ struct fused_lds {
     uint16_t S[SH * SW];
     uint16_t Sc[2][SCH * SCW];
   };
-  uint16_t L[LH * LW];
-  uint16_t Cc[2][CH * CW];
+  // L / Cc also hold the tile's output between the ALF and the final stores
+  alignas(16) uint16_t L[LH * LW];
+  alignas(16) uint16_t Cc[2][CH * CW];
   // the edge descriptors are dead once deblocking is done, before SAO writes
   union {
     maps_t m;
@@ -59,7 +60,18 @@ struct fused_lds {
   uint16_t lut[1024];
   static_assert(MLV_W <= 64 && MLH_W <= 64 && MCV_W <= 64 && MCH_W <= 64, "list column field");
   static_assert(sizeof(maps_t) <= sizeof(sao_out_t), "maps alias the SAO output");
+  static_assert(TW <= LW && TH <= LH && TW / 2 <= CW && TH / 2 <= CH, "output buffers fit the input windows");
+  static_assert(TW % 16 == 0, "16-byte output chunks");
 };
+
+// The thread index, opaque to the compiler: index arithmetic derived from it
+// is then recomputed where it is used instead of being hoisted out of the
+// tile loop, where it would hold dozens of registers for the whole kernel.
+__device__ __forceinline__ int ltid() {
+  int t;
+  __asm__ volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((int)threadIdx.x));
+  return t;
+}
 
 // Appends `val` for every lane with `pred` to an LDS list (one LDS atomic
 // per wave).  Called by all active lanes of the wave.
@@ -69,7 +81,7 @@ __device__ __forceinline__ void list_append(bool pred, uint32_t val, uint16_t* l
   const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
   const int leader = __ffsll((unsigned long long)m) - 1;
   uint32_t base = 0;
-  if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(cnt, (uint32_t)__popcll(m));
+  if ((int)(ltid() & 63) == leader) base = atomicAdd(cnt, (uint32_t)__popcll(m));
   base = __shfl(base, leader);
   if (pred) list[base + below] = (uint16_t)val;
 }
@@ -88,11 +100,12 @@ template <int MW, int MH>
 struct rect_stage {
   static constexpr int PER = (MW * MH + 255) / 256;
   uint32_t v[PER];
+  // map == nullptr: nothing to load (every element 0)
   __device__ __forceinline__ void issue(const uint32_t* map, int mw, const mrect& r) {
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = (int)threadIdx.x + k * 256, yy = i / MW, xx = i - yy * MW;
-      if (yy < r.ny && xx < r.nx) v[k] = map[(size_t)(r.y0 + yy) * mw + r.x0 + xx];
+      const int i = ltid() + k * 256, yy = i / MW, xx = i - yy * MW;
+      v[k] = (map && yy < r.ny && xx < r.nx) ? gld(&map[(size_t)(r.y0 + yy) * mw + r.x0 + xx]) : 0u;
     }
   }
   // also lists the nonzero words as tag | (row << 6) | column
@@ -100,7 +113,7 @@ struct rect_stage {
                                          uint32_t tag) const {
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = (int)threadIdx.x + k * 256, yy = i / MW, xx = i - yy * MW;
+      const int i = ltid() + k * 256, yy = i / MW, xx = i - yy * MW;
       const bool in = yy < r.ny && xx < r.nx;
       if (in) dst[i] = v[k];
       list_append(in && v[k] != 0, tag | (uint32_t)(yy << 6) | (uint32_t)xx, list, cnt);
@@ -135,7 +148,7 @@ __device__ __forceinline__ int cdiv_i(int a, int b) { return -fdiv(-a, b); }
 template <bool VERT, int PITCH>
 __device__ __forceinline__ void lds_deblock_luma(const dev_job& j, lwin& L, const uint32_t* map, int mpitch, const mrect& mr,
                                  const uint16_t* list, int nlist, int maxv) {
-  const int tid = threadIdx.x;
+  const int tid = ltid();
   for (int base = 0; base < nlist; base += blockDim.x / 4) {
     const int it = base + (tid >> 2), line = tid & 3;
     uint32_t e = 0;
@@ -208,7 +221,7 @@ __device__ __forceinline__ void lds_deblock_luma(const dev_job& j, lwin& L, cons
 template <bool VERT, int PITCH>
 __device__ __forceinline__ void lds_deblock_chroma(lwin cb, uint16_t* cr_p, const uint32_t* map0, const uint32_t* map1, int mpitch,
                                    const mrect& mr, const uint16_t* list, int nlist, int bd_dbk, int maxv) {
-  const int tid = threadIdx.x;
+  const int tid = ltid();
   for (int base = 0; base < nlist; base += blockDim.x / 2) {
     const int it = base + (tid >> 1), line = tid & 1;
     uint32_t e = 0;
@@ -290,7 +303,7 @@ __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)
   const bool has_rect = j.sao_rect != nullptr;
   const int ka = dst.ox >> 3, nk = ((dst.ox + dst.w - 1) >> 3) - ka + 1;
   constexpr int NK = (RW + 14) / 8;  // runs per row, at most
-  for (int i = threadIdx.x; i < NK * dst.h; i += 256) {
+  for (int i = ltid(); i < NK * dst.h; i += 256) {
     const int yy = i / NK, kk = i - yy * NK;
     if (kk >= nk) continue;
     const int y = dst.oy + yy, x8 = (ka + kk) * 8;
@@ -379,8 +392,8 @@ __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)
 
 // Window copy (SAO off for the whole picture).
 __device__ __forceinline__ void lds_copy(const lwin& src, lwin& dst) {
-  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int yy = threadIdx.x >> 6; yy < dst.h; yy += nw)
+  const int lane = ltid() & 63, nw = blockDim.x >> 6;
+  for (int yy = ltid() >> 6; yy < dst.h; yy += nw)
     for (int xx = lane; xx < dst.w; xx += 64) dst.wr(dst.ox + xx, dst.oy + yy, src.rd(dst.ox + xx, dst.oy + yy));
 }
 
@@ -393,7 +406,7 @@ __device__ __forceinline__ void lds_copy(const lwin& src, lwin& dst) {
 __device__ __forceinline__ void pad_to_rect(uint16_t* buf, int pitch, int ox, int oy, int xa, int xb, int ya, int yb, int rx0, int rx1,
                             int ry0, int ry1) {
   const int w = xb - xa, n = w * (yb - ya);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = ltid(); i < n; i += blockDim.x) {
     const int y = ya + i / w, x = xa + i % w;
     if (x >= rx0 && x <= rx1 && y >= ry0 && y <= ry1) continue;
     buf[(y - oy) * pitch + (x - ox)] = buf[(clampi(y, ry0, ry1) - oy) * pitch + (clampi(x, rx0, rx1) - ox)];
@@ -411,182 +424,225 @@ __device__ __forceinline__ void load_pairs(const uint16_t* base, uint32_t (&P)[N
   }
 }
 
-template <int TW, int TH, bool HBD>
-__device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, int x0, int y0) {
+// Geometry of one tile: the windows, edge-descriptor rectangles and SAO
+// table origin that follow from the job and the tile origin.
+struct tile_geom {
+  int x0, y0, x1, y1, cx0, cy0, cx1, cy1;
+  bool chroma, dbk, dbk_c, in_grid;
+  lwin L, Cw;  // luma window (s.L), chroma window (Cb in s.Cc[0]; Cr alike in s.Cc[1])
+  mrect mlv, mlh, mcv, mch;
+  sao_tab tb;
+};
+
+template <int TW, int TH>
+__device__ __forceinline__ tile_geom tile_geometry(const dev_job& j, fused_lds<TW, TH>& s, const tile_ref& tr) {
   typedef fused_lds<TW, TH> LD;
-  constexpr int NT = 256;  // threads per workgroup
+  tile_geom g;
   const int W = j.w, H = j.h;
-  const int x1 = min(x0 + TW, W), y1 = min(y0 + TH, H);
-  const int maxv = (1 << j.bd) - 1;
-  const bool chroma = j.wc > 0;
-  const int cx0 = x0 >> 1, cy0 = y0 >> 1, cx1 = min(x1 >> 1, j.wc), cy1 = min(y1 >> 1, j.hc);
-  // ---- geometry of everything the tile reads
-  lwin L{s.L, LD::LW, max(0, x0 - 16), max(0, y0 - 16), 0, 0};
-  L.w = min(W, x1 + 16) - L.ox;
-  L.h = min(H, y1 + 16) - L.oy;
-  // chroma window (Cb; Cr has the same geometry in s.Cc[1])
-  lwin Cw{s.Cc[0], LD::CW, max(0, cx0 - 4), max(0, cy0 - 4), 0, 0};
-  Cw.w = min(j.wc, cx1 + 4) - Cw.ox;
-  Cw.h = min(j.hc, cy1 + 4) - Cw.oy;
-  const bool dbk = j.do_dbk, dbk_c = chroma && j.do_dbk && j.do_dbk_chroma;
-  mrect mlv{0, 0, 0, 0}, mlh{0, 0, 0, 0}, mcv{0, 0, 0, 0}, mch{0, 0, 0, 0};
-  if (dbk) {
+  g.x0 = tr.x0;
+  g.y0 = tr.y0;
+  g.x1 = min(g.x0 + TW, W);
+  g.y1 = min(g.y0 + TH, H);
+  g.chroma = j.wc > 0;
+  g.cx0 = g.x0 >> 1;
+  g.cy0 = g.y0 >> 1;
+  g.cx1 = min(g.x1 >> 1, j.wc);
+  g.cy1 = min(g.y1 >> 1, j.hc);
+  g.L = lwin{s.L, LD::LW, max(0, g.x0 - 16), max(0, g.y0 - 16), 0, 0};
+  g.L.w = min(W, g.x1 + 16) - g.L.ox;
+  g.L.h = min(H, g.y1 + 16) - g.L.oy;
+  g.Cw = lwin{s.Cc[0], LD::CW, max(0, g.cx0 - 4), max(0, g.cy0 - 4), 0, 0};
+  g.Cw.w = min(j.wc, g.cx1 + 4) - g.Cw.ox;
+  g.Cw.h = min(j.hc, g.cy1 + 4) - g.Cw.oy;
+  g.dbk = j.do_dbk;
+  g.dbk_c = g.chroma && j.do_dbk && j.do_dbk_chroma;
+  g.mlv = g.mlh = g.mcv = g.mch = mrect{0, 0, 0, 0};
+  if (g.dbk) {
     // vertical edges x in [x0-10, x1+10] over the window rows
-    mlv.x0 = max(0, cdiv_i(x0 - 10, 4));
-    mlv.nx = min(j.gw - 1, fdiv(x1 + 10, 4)) - mlv.x0 + 1;
-    mlv.y0 = L.oy / 4;
-    mlv.ny = min(j.gh - 1, (L.oy + L.h - 1) / 4) - mlv.y0 + 1;
+    g.mlv.x0 = max(0, cdiv_i(g.x0 - 10, 4));
+    g.mlv.nx = min(j.gw - 1, fdiv(g.x1 + 10, 4)) - g.mlv.x0 + 1;
+    g.mlv.y0 = g.L.oy / 4;
+    g.mlv.ny = min(j.gh - 1, (g.L.oy + g.L.h - 1) / 4) - g.mlv.y0 + 1;
     // horizontal edges y in [y0-10, y1+10] over columns [x0-4, x1+4)
-    mlh.y0 = max(0, cdiv_i(y0 - 10, 4));
-    mlh.ny = min(j.gh - 1, fdiv(y1 + 10, 4)) - mlh.y0 + 1;
-    mlh.x0 = max(0, fdiv(x0 - 4, 4));
-    mlh.nx = min(j.gw - 1, fdiv(min(W, x1 + 4) - 1, 4)) - mlh.x0 + 1;
+    g.mlh.y0 = max(0, cdiv_i(g.y0 - 10, 4));
+    g.mlh.ny = min(j.gh - 1, fdiv(g.y1 + 10, 4)) - g.mlh.y0 + 1;
+    g.mlh.x0 = max(0, fdiv(g.x0 - 4, 4));
+    g.mlh.nx = min(j.gw - 1, fdiv(min(W, g.x1 + 4) - 1, 4)) - g.mlh.x0 + 1;
   }
-  if (dbk_c) {
+  if (g.dbk_c) {
     // chroma vertical edges x in [cx0-5, cx1+5], horizontal y in [cy0-5, cy1+5]
-    mcv.x0 = max(0, cdiv_i(cx0 - 5, 8));
-    mcv.nx = min(j.cvw - 1, fdiv(cx1 + 5, 8)) - mcv.x0 + 1;
-    mcv.y0 = Cw.oy / 2;
-    mcv.ny = min(j.cvh - 1, (Cw.oy + Cw.h - 1) / 2) - mcv.y0 + 1;
-    mch.y0 = max(0, cdiv_i(cy0 - 5, 8));
-    mch.ny = min(j.chh - 1, fdiv(cy1 + 5, 8)) - mch.y0 + 1;
-    mch.x0 = max(0, fdiv(cx0 - 3, 2));
-    mch.nx = min(j.chw - 1, fdiv(min(j.wc, cx1 + 3) - 1, 2)) - mch.x0 + 1;
+    g.mcv.x0 = max(0, cdiv_i(g.cx0 - 5, 8));
+    g.mcv.nx = min(j.cvw - 1, fdiv(g.cx1 + 5, 8)) - g.mcv.x0 + 1;
+    g.mcv.y0 = g.Cw.oy / 2;
+    g.mcv.ny = min(j.cvh - 1, (g.Cw.oy + g.Cw.h - 1) / 2) - g.mcv.y0 + 1;
+    g.mch.y0 = max(0, cdiv_i(g.cy0 - 5, 8));
+    g.mch.ny = min(j.chh - 1, fdiv(g.cy1 + 5, 8)) - g.mch.y0 + 1;
+    g.mch.x0 = max(0, fdiv(g.cx0 - 3, 2));
+    g.mch.nx = min(j.chw - 1, fdiv(min(j.wc, g.cx1 + 3) - 1, 2)) - g.mch.x0 + 1;
   }
   // SAO parameter table: CTBs covering luma x in [x0-4, x1+3] (chroma reach
   // included) and likewise y
-  sao_tab tb{0, 0};
+  g.tb = sao_tab{0, 0};
   if (j.do_sao) {
-    tb.bx = max(0, (x0 - 4) >> j.sao_ctb_log2);
-    tb.by = max(0, (y0 - 4) >> j.sao_ctb_log2);
+    g.tb.bx = max(0, (g.x0 - 4) >> j.sao_ctb_log2);
+    g.tb.by = max(0, (g.y0 - 4) >> j.sao_ctb_log2);
   }
-  // ALF parameters of the tile's CTB
-  const int ctb = 1 << j.ctb_log2;
-  const int rx = x0 >> j.ctb_log2, ry = y0 >> j.ctb_log2;
-  const bool in_grid = j.do_alf && rx < j.wctb && ry < j.hctb;
-  const size_t ci = in_grid ? (size_t)ry * j.wctb + rx : 0;
-  const uint32_t cw = in_grid ? j.alf_ctb[ci] : 0xffffffffu;
-  const int bank_idx = (int)(cw & 0xff);
-  const uint64_t r = in_grid ? j.alf_rect[ci] : (((uint64_t)W << 32) | ((uint64_t)H << 48));
-  // ---- load: samples (8 luma / 4 chroma per access), edge descriptors, SAO
-  // and ALF parameters; every load of a thread is issued before the first is
-  // used.  Window origins are multiples of 16 (luma) / 4 (chroma) samples and
-  // plane pitches multiples of 64, so the vector accesses are aligned; they
-  // may read up to 7 samples past the window's last column, inside the pitch.
-  {
-    constexpr int NCH = LD::LW / 8, PER = (LD::LH * NCH + NT - 1) / NT;
-    constexpr int NC4 = LD::CW / 4, PERC = (2 * LD::CH * NC4 + NT - 1) / NT;
-    constexpr int PLV = (LD::MLV_W * LD::MLV_H + NT - 1) / NT, PLH = (LD::MLH_W * LD::MLH_H + NT - 1) / NT;
-    constexpr int PCV = (LD::MCV_W * LD::MCV_H + NT - 1) / NT, PCH = (LD::MCH_W * LD::MCH_H + NT - 1) / NT;
-    constexpr int PBK = (ALF_BANK_WORDS + NT - 1) / NT;
-    const int tid = (int)threadIdx.x;
+  const int rx = g.x0 >> j.ctb_log2, ry = g.y0 >> j.ctb_log2;
+  g.in_grid = j.do_alf && rx < j.wctb && ry < j.hctb;
+  return g;
+}
+
+// Everything one tile reads from HBM, held in registers between the issue of
+// the loads and their commit to LDS: samples (8 luma / 4 chroma per access),
+// edge descriptors, the ALF banks and the SAO cells.  Window origins are
+// multiples of 16 (luma) / 4 (chroma) samples and plane pitches multiples of
+// 64, so the vector accesses are aligned; they may read up to 7 samples past
+// the window's last column, inside the pitch.  Every element is written by
+// issue() (0 where nothing is loaded), so none stays live from one tile to
+// the next.
+template <int TW, int TH>
+struct tile_loads {
+  typedef fused_lds<TW, TH> LD;
+  static constexpr int NT = 256;
+  static constexpr int NCH = LD::LW / 8, PER = (LD::LH * NCH + NT - 1) / NT;
+  static constexpr int NC4 = LD::CW / 4, PERC = (LD::CH * NC4 + NT - 1) / NT;  // per chroma plane
+  static constexpr int PBK = (ALF_BANK_WORDS + NT - 1) / NT;
+  uint4 v[PER];
+  uint2 vc[2][PERC];
+  rect_stage<LD::MLV_W, LD::MLV_H> slv;
+  rect_stage<LD::MLH_W, LD::MLH_H> slh;
+  rect_stage<LD::MCV_W, LD::MCV_H> scv0, scv1;
+  rect_stage<LD::MCH_W, LD::MCH_H> sch0, sch1;
+  uint32_t bk[PBK];
+  uint64_t small;  // chroma ALF / CC-ALF bank words (threads 0..103)
+  uint64_t sv;     // SAO cell parameters (threads 128..163)
+
+  __device__ __forceinline__ void issue(const dev_job& j, const tile_geom& g, const tile_ref& tr) {
+    const int tid = ltid();
+    const lwin& L = g.L;
     const int nch = (L.w + 7) >> 3, total = L.h * NCH;
     const uint16_t* src = j.in.y + (size_t)L.oy * j.py + L.ox;
-    uint4 v[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
-      if (i < total && ch < nch) v[k] = *reinterpret_cast<const uint4*>(src + (size_t)yy * j.py + 8 * ch);
+      v[k] = (i < total && ch < nch) ? gld(reinterpret_cast<const uint4*>(src + (size_t)yy * j.py + 8 * ch))
+                                     : make_uint4(0, 0, 0, 0);
     }
-    // tiles larger than 64x64 commit the luma first, so that fewer loads
-    // (and registers) are in flight at once
-    constexpr bool SPLIT = TW * TH > 64 * 64;
-    const int lut_lds = j.bd <= 10;
-    auto commit_luma = [&]() {
+    const int nchc = (g.Cw.w + 3) >> 2, per_plane = g.Cw.h * NC4;
 #pragma unroll
-      for (int k = 0; k < PER; k++) {
-        const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
-        if (i < total && ch < nch) {
-          if (j.do_lmcs) {
-            uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const uint32_t a = w4[q] & 0xffffu, b = w4[q] >> 16;
-              const uint32_t la = (lut_lds && a < 1024) ? s.lut[a] : j.lmcs_lut[a];
-              const uint32_t lb = (lut_lds && b < 1024) ? s.lut[b] : j.lmcs_lut[b];
-              w4[q] = la | (lb << 16);
-            }
-            v[k] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
-          }
-          *reinterpret_cast<uint4*>(&s.L[yy * LD::LW + 8 * ch]) = v[k];
-        }
-      }
-    };
-    if (SPLIT) commit_luma();
-    const int nchc = (Cw.w + 3) >> 2, per_plane = Cw.h * NC4;
-    uint2 vc[PERC];
-    if (chroma) {
+    for (int c = 0; c < 2; c++) {
+      const uint16_t* csrc = (c ? j.in.cr : j.in.cb) + (size_t)g.Cw.oy * j.pc + g.Cw.ox;
 #pragma unroll
       for (int k = 0; k < PERC; k++) {
-        const int i = tid + k * NT, c = i >= per_plane ? 1 : 0, ii = i - c * per_plane;
-        const int yy = ii / NC4, ch = ii - yy * NC4;
-        const uint16_t* csrc = c ? j.in.cr : j.in.cb;
-        if (i < 2 * per_plane && ch < nchc)
-          vc[k] = *reinterpret_cast<const uint2*>(csrc + (size_t)(Cw.oy + yy) * j.pc + Cw.ox + 4 * ch);
+        const int i = tid + k * NT, yy = i / NC4, ch = i - yy * NC4;
+        vc[c][k] = (g.chroma && i < per_plane && ch < nchc)
+                       ? gld(reinterpret_cast<const uint2*>(csrc + (size_t)yy * j.pc + 4 * ch))
+                       : make_uint2(0, 0);
       }
     }
-    rect_stage<LD::MLV_W, LD::MLV_H> slv;
-    rect_stage<LD::MLH_W, LD::MLH_H> slh;
-    rect_stage<LD::MCV_W, LD::MCV_H> scv0, scv1;
-    rect_stage<LD::MCH_W, LD::MCH_H> sch0, sch1;
-    if (dbk) {
-      slv.issue(j.e_lv, j.gw, mlv);
-      slh.issue(j.e_lh, j.gw, mlh);
-    }
-    if (dbk_c) {
-      scv0.issue(j.e_cv[0], j.cvw, mcv);
-      scv1.issue(j.e_cv[1], j.cvw, mcv);
-      sch0.issue(j.e_ch[0], j.chw, mch);
-      sch1.issue(j.e_ch[1], j.chw, mch);
-    }
-    uint32_t bk[PBK];
+    slv.issue(g.dbk ? j.e_lv : nullptr, j.gw, g.mlv);
+    slh.issue(g.dbk ? j.e_lh : nullptr, j.gw, g.mlh);
+    scv0.issue(g.dbk_c ? j.e_cv[0] : nullptr, j.cvw, g.mcv);
+    scv1.issue(g.dbk_c ? j.e_cv[1] : nullptr, j.cvw, g.mcv);
+    sch0.issue(g.dbk_c ? j.e_ch[0] : nullptr, j.chw, g.mch);
+    sch1.issue(g.dbk_c ? j.e_ch[1] : nullptr, j.chw, g.mch);
+    const int bank_idx = (int)(tr.alf_cw & 0xff);
     const bool alf_on = bank_idx != (int)ALF_OFF8;
     const uint32_t* bsrc = j.alf_bank + (size_t)(alf_on ? bank_idx : 0) * ALF_BANK_WORDS;
 #pragma unroll
-    for (int k = 0; k < PBK; k++) bk[k] = (alf_on && tid + k * NT < ALF_BANK_WORDS) ? bsrc[tid + k * NT] : 0u;
-    // small tables: chroma ALF / CC-ALF banks (104 words), SAO cells (36)
-    uint64_t small = 0;
-    if (in_grid && tid < ALF_CBANK_WORDS) small = j.alf_cbank[tid];
-    else if (in_grid && tid < ALF_CBANK_WORDS + 2 * ALF_CCBANK_WORDS)
-      small = (uint32_t)j.alf_ccbank[tid - ALF_CBANK_WORDS];
+    for (int k = 0; k < PBK; k++) bk[k] = (alf_on && tid + k * NT < ALF_BANK_WORDS) ? gld(bsrc + tid + k * NT) : 0u;
+    small = 0;
+    if (g.in_grid && tid < ALF_CBANK_WORDS) small = gld(j.alf_cbank + tid);
+    else if (g.in_grid && tid < ALF_CBANK_WORDS + 2 * ALF_CCBANK_WORDS)
+      small = (uint32_t)gld(j.alf_ccbank + tid - ALF_CBANK_WORDS);
     // SAO cells: thread 128 + 4 * cell + k loads component k (k < 3) or the
     // rectangle (k = 3); off components and cells outside the grid read 0
-    uint64_t sv = 0;
-    const int scell = tid - 128, scx = tb.bx + (scell >> 2) % 3, scy = tb.by + (scell >> 2) / 3, sk = scell & 3;
-    const bool sao_cell = j.do_sao && scell >= 0 && scell < 36;
-    if (sao_cell && scx < j.swctb && scy < j.shctb) {
+    sv = 0;
+    const int scell = tid - 128, scx = g.tb.bx + (scell >> 2) % 3, scy = g.tb.by + (scell >> 2) / 3, sk = scell & 3;
+    if (j.do_sao && scell >= 0 && scell < 36 && scx < j.swctb && scy < j.shctb) {
       const size_t sci = (size_t)scy * j.swctb + scx;
       const bool on = sk == 3 || (sk == 0 ? j.sao_luma : j.sao_chroma);
-      if (on) sv = sk < 3 ? j.sao[sci * 3 + sk] : (j.sao_rect ? j.sao_rect[sci] : 0);
+      if (on) sv = sk < 3 ? gld(j.sao + sci * 3 + sk) : (j.sao_rect ? gld(j.sao_rect + sci) : 0);
     }
-    // ---- commit to LDS
-    if (!SPLIT) commit_luma();
-    if (chroma) {
+  }
+
+  // No next tile: every element defined all the same, so that the compiler
+  // sees the registers dead between commit() and the next issue().
+  __device__ __forceinline__ void clear() {
 #pragma unroll
-      for (int k = 0; k < PERC; k++) {
-        const int i = tid + k * NT, c = i >= per_plane ? 1 : 0, ii = i - c * per_plane;
-        const int yy = ii / NC4, ch = ii - yy * NC4;
-        if (i < 2 * per_plane && ch < nchc) *reinterpret_cast<uint2*>(&s.Cc[c][yy * LD::CW + 4 * ch]) = vc[k];
+    for (int k = 0; k < PER; k++) v[k] = make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int k = 0; k < PERC; k++) vc[c][k] = make_uint2(0, 0);
+    slv.issue(nullptr, 0, mrect{0, 0, 0, 0});
+    slh.issue(nullptr, 0, mrect{0, 0, 0, 0});
+    scv0.issue(nullptr, 0, mrect{0, 0, 0, 0});
+    scv1.issue(nullptr, 0, mrect{0, 0, 0, 0});
+    sch0.issue(nullptr, 0, mrect{0, 0, 0, 0});
+    sch1.issue(nullptr, 0, mrect{0, 0, 0, 0});
+#pragma unroll
+    for (int k = 0; k < PBK; k++) bk[k] = 0u;
+    small = sv = 0;
+  }
+
+  // Writes the loaded data to LDS (luma through the LMCS inverse table) and
+  // lists the nonzero edge descriptors.  The caller synchronises afterwards.
+  __device__ __forceinline__ void commit(const dev_job& j, const tile_geom& g, const tile_ref& tr,
+                                         fused_lds<TW, TH>& s) {
+    const int tid = ltid();
+    const int W = j.w, H = j.h;
+    const lwin& L = g.L;
+    const int nch = (L.w + 7) >> 3, total = L.h * NCH;
+    const int lut_lds = j.bd <= 10;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
+      if (i < total && ch < nch) {
+        uint4 w = v[k];
+        if (j.do_lmcs) {
+          uint32_t w4[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const uint32_t a = w4[q] & 0xffffu, b = w4[q] >> 16;
+            const uint32_t la = (lut_lds && a < 1024) ? s.lut[a] : gld(j.lmcs_lut + a);
+            const uint32_t lb = (lut_lds && b < 1024) ? s.lut[b] : gld(j.lmcs_lut + b);
+            w4[q] = la | (lb << 16);
+          }
+          w = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        }
+        *reinterpret_cast<uint4*>(&s.L[yy * LD::LW + 8 * ch]) = w;
       }
     }
-    if (dbk) {
-      slv.commit(s.m.lv, mlv, s.m.llv, &s.nlist[0], 0);
-      slh.commit(s.m.lh, mlh, s.m.llh, &s.nlist[1], 0);
+    if (g.chroma) {
+      const int nchc = (g.Cw.w + 3) >> 2, per_plane = g.Cw.h * NC4;
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int k = 0; k < PERC; k++) {
+          const int i = tid + k * NT, yy = i / NC4, ch = i - yy * NC4;
+          if (i < per_plane && ch < nchc) *reinterpret_cast<uint2*>(&s.Cc[c][yy * LD::CW + 4 * ch]) = vc[c][k];
+        }
     }
-    if (dbk_c) {
-      scv0.commit(s.m.cv[0], mcv, s.m.lcv, &s.nlist[2], 0);
-      scv1.commit(s.m.cv[1], mcv, s.m.lcv, &s.nlist[2], 1u << 15);
-      sch0.commit(s.m.ch[0], mch, s.m.lch, &s.nlist[3], 0);
-      sch1.commit(s.m.ch[1], mch, s.m.lch, &s.nlist[3], 1u << 15);
+    if (g.dbk) {
+      slv.commit(s.m.lv, g.mlv, s.m.llv, &s.nlist[0], 0);
+      slh.commit(s.m.lh, g.mlh, s.m.llh, &s.nlist[1], 0);
     }
-    if (alf_on) {
+    if (g.dbk_c) {
+      scv0.commit(s.m.cv[0], g.mcv, s.m.lcv, &s.nlist[2], 0);
+      scv1.commit(s.m.cv[1], g.mcv, s.m.lcv, &s.nlist[2], 1u << 15);
+      sch0.commit(s.m.ch[0], g.mch, s.m.lch, &s.nlist[3], 0);
+      sch1.commit(s.m.ch[1], g.mch, s.m.lch, &s.nlist[3], 1u << 15);
+    }
+    if ((tr.alf_cw & 0xff) != ALF_OFF8) {
 #pragma unroll
       for (int k = 0; k < PBK; k++)
         if (tid + k * NT < ALF_BANK_WORDS) s.bank[tid + k * NT] = bk[k];
     }
-    if (in_grid && tid < ALF_CBANK_WORDS) s.cbank[tid] = (uint32_t)small;
-    else if (in_grid && tid < ALF_CBANK_WORDS + 2 * ALF_CCBANK_WORDS) s.ccbank[tid - ALF_CBANK_WORDS] = (int32_t)small;
-    if (sao_cell) {
+    if (g.in_grid && tid < ALF_CBANK_WORDS) s.cbank[tid] = (uint32_t)small;
+    else if (g.in_grid && tid < ALF_CBANK_WORDS + 2 * ALF_CCBANK_WORDS) s.ccbank[tid - ALF_CBANK_WORDS] = (int32_t)small;
+    const int scell = tid - 128, sk = scell & 3;
+    if (j.do_sao && scell >= 0 && scell < 36) {
       if (sk < 3) {
         s.saop[scell >> 2][sk] = sv;
       } else {
@@ -606,6 +662,36 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       }
     }
   }
+};
+
+// Filters tile `tr` whose loads are in `ld`: commit, deblock, SAO, ALF.  When
+// `more` is set, the loads of the next tile `ntr` are issued into `ld` once
+// the luma ALF is done, so that they are in flight during the chroma ALF and
+// reach the registers by the next call.  (The next job is read from `jobs`
+// directly, which keeps its fields on scalar loads.)
+template <int TW, int TH, bool HBD>
+__device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& s, const tile_ref& tr,
+                                           tile_loads<TW, TH>& ld, const dev_job* __restrict__ jobs, bool more,
+                                           const tile_ref& ntr) {
+  typedef fused_lds<TW, TH> LD;
+  constexpr int NT = 256;  // threads per workgroup
+  const tile_geom g = tile_geometry<TW, TH>(j, s, tr);
+  const int x0 = g.x0, y0 = g.y0, x1 = g.x1, y1 = g.y1;
+  const int cx0 = g.cx0, cy0 = g.cy0, cx1 = g.cx1, cy1 = g.cy1;
+  const int W = j.w, H = j.h;
+  const int maxv = (1 << j.bd) - 1;
+  const bool chroma = g.chroma, dbk = g.dbk, dbk_c = g.dbk_c, in_grid = g.in_grid;
+  lwin L = g.L;
+  const lwin Cw = g.Cw;
+  const mrect mlv = g.mlv, mlh = g.mlh, mcv = g.mcv, mch = g.mch;
+  const sao_tab tb = g.tb;
+  // ALF parameters of the tile's CTB (from the tile record)
+  const int ctb = 1 << j.ctb_log2;
+  const int ry = y0 >> j.ctb_log2;
+  const uint32_t cw = tr.alf_cw;
+  const int bank_idx = (int)(cw & 0xff);
+  const uint64_t r = tr.alf_rect;
+  ld.commit(j, g, tr, s);
   __syncthreads();
   // ---- deblocking: all vertical edges, then all horizontal edges
   if (dbk) {
@@ -666,9 +752,11 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     const int amax = (1 << j.alf_bd) - 1;
     const int y_ctb = ry * ctb;
     const bool carve = !(y_ctb + ctb >= H) || (H - y_ctb) > ctb - 4;
-    uint16_t* dst = j.out.y;
+    // results go to the tile's output buffer in LDS (pitch TW, in the input
+    // window's space, dead since SAO) and leave in one pass at the end
+    uint16_t* const ob = s.L;
 #pragma unroll 1
-    for (int b = threadIdx.x; b < bw * bh; b += NT) {
+    for (int b = ltid(); b < bw * bh; b += NT) {
       const int x4 = x0 + 4 * (b % bw), y4 = y0 + 4 * (b / bw);
       uint32_t outp[4][2];
       if (bank_idx == (int)ALF_OFF8) {
@@ -717,20 +805,17 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
           else alf_pk_filter_rows<2, 2, 2, 8>(Q, f, amax, outp);
         }
       }
-      if (x4 + 4 <= x1) {
 #pragma unroll
-        for (int rr = 0; rr < 4; rr++)
-          if (y4 + rr < y1)
-            *reinterpret_cast<uint2*>(dst + (size_t)(y4 + rr) * j.py + x4) = make_uint2(outp[rr][0], outp[rr][1]);
-      } else {
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-          for (int c = 0; c < 4; c++)
-            if (y4 + rr < y1 && x4 + c < x1)
-              dst[(size_t)(y4 + rr) * j.py + x4 + c] = (uint16_t)(outp[rr][c >> 1] >> (16 * (c & 1)));
-      }
+      for (int rr = 0; rr < 4; rr++)
+        *reinterpret_cast<uint2*>(&ob[(y4 - y0 + rr) * TW + (x4 - x0)]) = make_uint2(outp[rr][0], outp[rr][1]);
     }
+  }
+  // the next tile's loads, in flight during the chroma ALF
+  if (more) {
+    const dev_job nj = jobs[ntr.job];  // one burst of scalar loads
+    ld.issue(nj, tile_geometry<TW, TH>(nj, s, ntr), ntr);
+  } else {
+    ld.clear();
   }
   // chroma ALF + CC-ALF: two horizontally adjacent samples per thread.
   // Sample x of Sc sits at an even storage column when x - cx0 is even, so
@@ -741,7 +826,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     const int ctbc = ctb >> 1;
     const int amax = (1 << j.alf_bd) - 1, half = 1 << (j.alf_bd - 1);
     const s16x2 zero2 = {0, 0}, vmax2 = {(short)amax, (short)amax};
-    for (int i = threadIdx.x; i < 2 * npair; i += NT) {
+    for (int i = ltid(); i < 2 * npair; i += NT) {
       const int c = i >= npair ? 1 : 0, k = i - c * npair;
       const int yy = k / pw2, x = cx0 + 2 * (k - yy * pw2), y = cy0 + yy;
       const uint16_t* sc = s.o.Sc[c];
@@ -816,12 +901,51 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
           outp = o2[0] | (o2[1] << 16);
         }
       }
-      uint16_t* dst = (c ? j.out.cr : j.out.cb) + (size_t)y * j.pc + x;
-      if (x + 1 < cx1) *reinterpret_cast<uint32_t*>(dst) = outp;
-      else dst[0] = (uint16_t)outp;
+      // output buffer [2][TH/2][TW/2] in the chroma input windows' space
+      uint16_t* const oc = &s.Cc[0][0];
+      *reinterpret_cast<uint32_t*>(&oc[(c * (TH / 2) + (y - cy0)) * (TW / 2) + (x - cx0)]) = outp;
     }
   }
-  if (threadIdx.x < 4) s.nlist[threadIdx.x] = 0;  // descriptor lists of the next tile
+  // ---- store the tile: 16-byte chunks of whole rows; issued after the next
+  // tile's loads, so that waiting for those loads does not wait for these
+  // stores (one in-order counter covers both)
+  __syncthreads();
+  {
+    const int tid = ltid();
+    auto put = [&](const uint16_t* src, uint16_t* dst, int xa, int xb) {
+      const uint4 v = *reinterpret_cast<const uint4*>(src);
+      if (xa + 8 <= xb) {
+        gst(reinterpret_cast<uint4*>(dst), v);
+      } else {
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (xa + k < xb) gst(dst + k, (uint16_t)(w[k >> 1] >> (16 * (k & 1))));
+      }
+    };
+    constexpr int LCH = TW / 8, NLC = (TH * LCH + NT - 1) / NT;
+#pragma unroll
+    for (int q = 0; q < NLC; q++) {
+      const int i = tid + q * NT, yy = i / LCH, xx = 8 * (i - yy * LCH);
+      if (i < TH * LCH && y0 + yy < y1 && x0 + xx < x1)
+        put(&s.L[yy * TW + xx], j.out.y + (size_t)(y0 + yy) * j.py + x0 + xx, x0 + xx, x1);
+    }
+    if (chroma) {
+      // both planes: 2 * (TH/2) rows; Cr lies at a fixed distance from Cb
+      // (one allocation), which keeps the plane choice arithmetic
+      constexpr int CCH = TW / 16, NCC = (TH * CCH + NT - 1) / NT;
+      const uint16_t* const oc = &s.Cc[0][0];
+      const ptrdiff_t dcr = j.out.cr - j.out.cb;
+#pragma unroll
+      for (int q = 0; q < NCC; q++) {
+        const int i = tid + q * NT, row = i / CCH, xx = 8 * (i - row * CCH);
+        const int c = row >= TH / 2 ? 1 : 0, yy = row - c * (TH / 2);
+        if (i < TH * CCH && cy0 + yy < cy1 && cx0 + xx < cx1)
+          put(&oc[row * (TW / 2) + xx], j.out.cb + c * dcr + (size_t)(cy0 + yy) * j.pc + cx0 + xx, cx0 + xx, cx1);
+      }
+    }
+  }
+  if (ltid() < 4) s.nlist[ltid()] = 0;  // descriptor lists of the next tile
   __syncthreads();  // LDS is reused by the next tile
 }
 
@@ -836,19 +960,31 @@ __global__ void __launch_bounds__(256, 2) k_fused(const dev_job* __restrict__ jo
   const uint32_t chunk = (n_tiles + 7) >> 3;
   const uint32_t begin = xcd * chunk, end = min(n_tiles, begin + chunk);
   int cur_job = -1;
-  if (threadIdx.x < 4) s.nlist[threadIdx.x] = 0;
+  if (ltid() < 4) s.nlist[ltid()] = 0;
   __syncthreads();
+  // Software pipeline: the loads of tile t+1 are issued while tile t is
+  // filtered (inside fused_tile), those of the first tile here.
+  tile_loads<TW, TH> ld;
+  tile_ref tr{};
+  if (begin + k < end) {
+    tr = tiles[begin + k];
+    const dev_job& j0 = jobs[tr.job];
+    ld.issue(j0, tile_geometry<TW, TH>(j0, s, tr), tr);
+  }
   for (uint32_t t = begin + k; t < end; t += per_xcd) {
-    const tile_ref tr = tiles[t];
+    const bool more = t + per_xcd < end;
+    tile_ref ntr{};
+    if (more) ntr = tiles[t + per_xcd];
     const dev_job& j = jobs[tr.job];
     if ((int)tr.job != cur_job) {
       // LMCS inverse table in LDS for bit depths <= 10
       if (j.do_lmcs && j.bd <= 10)
-        for (int i = threadIdx.x; i < 1024; i += blockDim.x) s.lut[i] = j.lmcs_lut[i];
+        for (int i = ltid(); i < 1024; i += blockDim.x) s.lut[i] = gld(j.lmcs_lut + i);
       cur_job = (int)tr.job;
       __syncthreads();
     }
-    fused_tile<TW, TH, HBD>(j, s, tr.x0, tr.y0);
+    fused_tile<TW, TH, HBD>(j, s, tr, ld, jobs, more, ntr);
+    tr = ntr;
   }
 }
 

@@ -134,4 +134,23 @@ struct vvclf_job {
   hipStream_t pool_stream = nullptr;  // set when allocated from the stream-ordered pool
   hipGraph_t graph = nullptr;         // the plain sequence, captured on first graph launch
   hipGraphExec_t exec = nullptr;
+  // host copies of the per-CTB ALF words and rectangles (tile records)
+  std::vector<uint32_t> h_alf_ctb;
+  std::vector<uint64_t> h_alf_rect;
 };
+
+namespace vvclf {
+// The fused kernel's record of the tile at (x, y) of job `j` (index `ji` in
+// the launch's job array).
+inline tile_ref make_tile(const vvclf_job* j, uint32_t ji, uint32_t x, uint32_t y) {
+  const dev_job& d = j->d;
+  tile_ref t{ji, (uint16_t)x, (uint16_t)y, 0xffffffffu, 0u, ((uint64_t)(uint32_t)d.w << 32) | ((uint64_t)(uint32_t)d.h << 48)};
+  const uint32_t rx = x >> d.ctb_log2, ry = y >> d.ctb_log2;
+  if (d.do_alf && rx < (uint32_t)d.wctb && ry < (uint32_t)d.hctb) {
+    const size_t ci = (size_t)ry * d.wctb + rx;
+    t.alf_cw = j->h_alf_ctb[ci];
+    t.alf_rect = j->h_alf_rect[ci];
+  }
+  return t;
+}
+}  // namespace vvclf

@@ -69,10 +69,16 @@ static constexpr int ALF_CBANK_WORDS = 8 * 6;
 // CC bank: [4 filters][7 taps] as i32 per component.
 static constexpr int ALF_CCBANK_WORDS = 4 * 7;
 
-// One output tile of the fused batched form: job index and tile origin.
+// One output tile of the fused batched form: job index, tile origin, and the
+// ALF parameters of the CTB holding the tile (its alf_ctb word, 0xffffffff
+// where ALF is off or outside the CTB grid, and its alf_rect), filled on the
+// host so that the kernel starts the tile without a dependent load.
 struct tile_ref {
   uint32_t job;
   uint16_t x0, y0;
+  uint32_t alf_cw;
+  uint32_t pad;
+  uint64_t alf_rect;
 };
 
 struct dev_planes {

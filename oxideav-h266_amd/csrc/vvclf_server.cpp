@@ -390,7 +390,7 @@ void launch_loop(vvclf_server* s) {
       for (uint32_t oi : order) {
         const vvclf_job* j = fl.reqs[oi]->job;
         for (uint32_t y = 0; y < j->h; y += th)
-          for (uint32_t x = 0; x < j->w; x += tw) sl.h_tiles[k++] = tile_ref{oi, (uint16_t)x, (uint16_t)y};
+          for (uint32_t x = 0; x < j->w; x += tw) sl.h_tiles[k++] = make_tile(j, oi, x, y);
       }
       e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, fl.reqs.size() * sizeof(dev_job), hipMemcpyHostToDevice, s->s_comp);
       if (e == hipSuccess)
