@@ -953,7 +953,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
 // XCD (workgroups are dispatched round-robin over the 8 XCDs), so spatially
 // adjacent tiles - which share halo samples - hit the same L2.
 template <int TW, int TH, bool HBD>
-__global__ void __launch_bounds__(256, 2) k_fused(const dev_job* __restrict__ jobs, const tile_ref* __restrict__ tiles,
+__global__ void __launch_bounds__(256, 3) k_fused(const dev_job* __restrict__ jobs, const tile_ref* __restrict__ tiles,
                                                uint32_t n_tiles) {
   __shared__ fused_lds<TW, TH> s;
   const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;
