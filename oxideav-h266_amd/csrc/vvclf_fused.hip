@@ -53,6 +53,7 @@ struct fused_lds {
   uint32_t cbank[ALF_CBANK_WORDS];
   int32_t ccbank[2 * ALF_CCBANK_WORDS];
   uint64_t saop[9][3];  // SAO parameters of the (up to) 3x3 CTBs around the tile
+  uint4 saorun[9][3];   // the same decoded for the run form (lds_sao)
   uint64_t saor[9];     // and their sample rectangles
   uint64_t safe[9][3];  // per CTB and component: [x0, x1) x [y0, y1) whose 8-neighbourhoods lie
                         // inside the picture and the CTB's rectangle (u16 fields)
@@ -289,71 +290,94 @@ struct sao_tab {
 // SAO of one component from window `src` into window `dst` (dst's extent is
 // the region produced, at most RW samples wide).  sub = 1 for chroma.  Work is
 // split into runs of 8 samples at x = 8k (one CTB per run, SAO CTBs being at
-// least 8 samples wide), walked flat so that the lanes stay busy.  Runs with
-// every offset in i8 use the table form (sao_eo8 / sao_bo8); edge-offset
-// samples whose 8-neighbourhood is not inside the picture and the CTB's
-// rectangle take the per-sample check.
-template <int RW>
-__device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)[3], const uint64_t* saor,
-                                        const uint64_t (*safe)[3], const sao_tab& tb, const lwin& src, lwin& dst,
-                                        int comp, int pw, int ph) {
-  const int sub = comp ? 1 : 0;
+// least 8 samples wide), walked flat so that the lanes stay busy.  A run reads
+// its CTB's decoded parameters (runs[cell][comp]: type | class or band << 8 |
+// fits << 16, then the i8 offset table) and all three sample rows up front,
+// so that the LDS reads overlap.  Runs with every offset in i8 use the table
+// form (sao_eo8 / sao_bo8); edge-offset samples whose 8-neighbourhood is not
+// inside the picture and the CTB's rectangle take the per-sample check.
+// SRC16: src rows at x8 are 16-byte aligned (else 8); DST8: dst rows at x8
+// are 8-byte aligned (else 4).  NPL = 2 walks both chroma planes in one loop
+// (component comp0 + plane; plane 1 at src.p + sd1 / dst.p + dd1, same
+// geometry).
+template <int RW, bool SRC16, bool DST8, int NPL>
+__device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)[3], const uint4 (*runs)[3],
+                                        const uint64_t* saor, const uint64_t (*safe)[3], const sao_tab& tb,
+                                        const lwin& src0, const lwin& dst0, int comp0, int sd1, int dd1, int pw,
+                                        int ph) {
+  const int sub = comp0 ? 1 : 0;
   const int log2c = j.sao_ctb_log2 - sub;
   const int maxv = (1 << j.sao_bd) - 1;
   const bool has_rect = j.sao_rect != nullptr;
-  const int ka = dst.ox >> 3, nk = ((dst.ox + dst.w - 1) >> 3) - ka + 1;
+  const int ka = dst0.ox >> 3, nk = ((dst0.ox + dst0.w - 1) >> 3) - ka + 1;
   constexpr int NK = (RW + 14) / 8;  // runs per row, at most
-  for (int i = ltid(); i < NK * dst.h; i += 256) {
+  const int per_plane = NK * dst0.h;
+  for (int i0 = ltid(); i0 < NPL * per_plane; i0 += 256) {
+    const int pl = (NPL > 1 && i0 >= per_plane) ? 1 : 0, i = i0 - pl * per_plane;
+    const int comp = comp0 + pl;
+    lwin src = src0, dst = dst0;
+    src.p += pl * sd1;
+    dst.p += pl * dd1;
     const int yy = i / NK, kk = i - yy * NK;
     if (kk >= nk) continue;
     const int y = dst.oy + yy, x8 = (ka + kk) * 8;
     const int xa = max(x8, dst.ox), xb = min(x8 + 8, dst.ox + dst.w);  // samples produced
     const int cell = ((y >> log2c) - tb.by) * 3 + ((x8 >> log2c) - tb.bx);
-    const uint64_t prm = saop[cell][comp];
-    const sao_run run = sao_run_of(prm);
-    // row yr at x8-1 .. x8+8 (direct reads when inside the window, else clamped)
-    const bool direct = x8 - 1 >= src.ox && x8 + 9 <= src.ox + src.w;
-    auto row = [&](int yr, int (&R)[10]) {
-      if (direct && yr >= src.oy && yr < src.oy + src.h) {
+    const uint4 rp = runs[cell][comp];
+    const uint64_t sr = safe[cell][comp];
+    // rows y-1, y, y+1 at x8-1 .. x8+8 (direct reads when inside the window,
+    // else clamped)
+    int A[10], P[10], B[10];
+    const bool direct = x8 - 1 >= src.ox && x8 + 9 <= src.ox + src.w && y - 1 >= src.oy && y + 2 <= src.oy + src.h;
+    if (direct) {
+      auto row = [&](int yr, int (&R)[10]) {
         const uint16_t* q = src.p + (yr - src.oy) * src.pitch + (x8 - src.ox);
-        const uint2 a = *reinterpret_cast<const uint2*>(q), b = *reinterpret_cast<const uint2*>(q + 4);
-        const uint32_t w[4] = {a.x, a.y, b.x, b.y};
+        uint32_t w[4];
+        if (SRC16) {
+          const uint4 v = *reinterpret_cast<const uint4*>(q);
+          w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+          const uint2 a = *reinterpret_cast<const uint2*>(q), b = *reinterpret_cast<const uint2*>(q + 4);
+          w[0] = a.x; w[1] = a.y; w[2] = b.x; w[3] = b.y;
+        }
 #pragma unroll
         for (int k = 0; k < 8; k++) R[k + 1] = (int)((w[k >> 1] >> (16 * (k & 1))) & 0xffffu);
         R[0] = q[-1];
         R[9] = q[8];
-      } else {
+      };
+      row(y - 1, A);
+      row(y, P);
+      row(y + 1, B);
+    } else {
 #pragma unroll
-        for (int k = 0; k < 10; k++) R[k] = src.rd(x8 - 1 + k, yr);
+      for (int k = 0; k < 10; k++) {
+        A[k] = src.rd(x8 - 1 + k, y - 1);
+        P[k] = src.rd(x8 - 1 + k, y);
+        B[k] = src.rd(x8 - 1 + k, y + 1);
       }
-    };
-    int P[10], o[8];
-    row(y, P);
+    }
+    const int type = (int)(rp.x & 3), aux = (int)((rp.x >> 8) & 31);
+    const bool fits = (rp.x >> 16) & 1;
+    const uint64_t tab = (uint64_t)rp.y | ((uint64_t)rp.z << 32);
+    int o[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) o[k] = P[k + 1];
-    if (run.type == 1) {
-      if (run.fits) {
-        sao_bo8(P, run.aux, run.tab, j.sao_bd, maxv, o);
+    if (type == 1) {
+      if (fits) {
+        sao_bo8(P, aux, tab, j.sao_bd, maxv, o);
       } else {
+        const uint64_t prm = saop[cell][comp];
 #pragma unroll
         for (int k = 0; k < 8; k++) o[k] = sao_band_sample(P[k + 1], prm, j.sao_bd, maxv);
       }
-    } else if (run.type == 2) {
-      const int cls = run.aux & 3;
-      int A[10], B[10];
-      if (cls != 0) {
-        row(y - 1, A);
-        row(y + 1, B);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 10; k++) A[k] = B[k] = P[k];
-      }
-      const uint64_t sr = safe[cell][comp];
+    } else if (type == 2) {
+      const int cls = aux & 3;
       const bool inner = xa >= (int)(sr & 0xffff) && xb <= (int)((sr >> 16) & 0xffff) &&
                          y >= (int)((sr >> 32) & 0xffff) && y < (int)(sr >> 48);
-      if (inner && run.fits) {
-        sao_eo8_any(cls, A, P, B, run.tab, maxv, o);
+      if (inner && fits) {
+        sao_eo8_any(cls, A, P, B, tab, maxv, o);
       } else {
+        const uint64_t prm = saop[cell][comp];
         int h0, v0, h1, v1;
         eo_offsets(cls, h0, v0, h1, v1);
         int cx0 = 0, cy0 = 0, cx1 = pw, cy1 = ph;
@@ -384,9 +408,23 @@ __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)
         }
       }
     }
+    if (xa == x8 && xb == x8 + 8) {
+      uint16_t* d = dst.p + (y - dst.oy) * dst.pitch + (x8 - dst.ox);
+      uint32_t w[4];
 #pragma unroll
-    for (int k = 0; k < 8; k++)
-      if (x8 + k >= xa && x8 + k < xb) dst.wr(x8 + k, y, o[k]);
+      for (int k = 0; k < 4; k++) w[k] = (uint32_t)o[2 * k] | ((uint32_t)o[2 * k + 1] << 16);
+      if (DST8) {
+        reinterpret_cast<uint2*>(d)[0] = make_uint2(w[0], w[1]);
+        reinterpret_cast<uint2*>(d)[1] = make_uint2(w[2], w[3]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) reinterpret_cast<uint32_t*>(d)[k] = w[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (x8 + k >= xa && x8 + k < xb) dst.wr(x8 + k, y, o[k]);
+    }
   }
 }
 
@@ -645,6 +683,9 @@ struct tile_loads {
     if (j.do_sao && scell >= 0 && scell < 36) {
       if (sk < 3) {
         s.saop[scell >> 2][sk] = sv;
+        const sao_run rn = sao_run_of(sv);
+        s.saorun[scell >> 2][sk] = make_uint4((uint32_t)rn.type | ((uint32_t)rn.aux << 8) | ((uint32_t)rn.fits << 16),
+                                              (uint32_t)rn.tab, (uint32_t)(rn.tab >> 32), 0u);
       } else {
         s.saor[scell >> 2] = sv;
         // rectangles inside which a sample's whole 8-neighbourhood lies in
@@ -715,18 +756,25 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   S.w = min(W, x1 + 3) - S.ox;
   S.h = min(H, y1 + 3) - S.oy;
   S.p = s.o.S + (S.oy - soy) * LD::SW + (S.ox - sox);
-  if (j.do_sao) lds_sao<TW + 6>(j, s.saop, s.saor, s.safe, tb, L, S, 0, W, H);
+  if (j.do_sao) lds_sao<TW + 6, true, true, 1>(j, s.saop, s.saorun, s.saor, s.safe, tb, L, S, 0, 0, 0, W, H);
   else lds_copy(L, S);
   if (chroma) {
-    for (int c = 0; c < 2; c++) {
-      lwin Sc{nullptr, LD::SCW, max(0, cx0 - 2), max(0, cy0 - 2), 0, 0};
-      Sc.w = min(j.wc, cx1 + 2) - Sc.ox;
-      Sc.h = min(j.hc, cy1 + 2) - Sc.oy;
-      Sc.p = s.o.Sc[c] + (Sc.oy - scoy) * LD::SCW + (Sc.ox - scox);
-      lwin Cs = Cw;
-      Cs.p = s.Cc[c];
-      if (j.do_sao) lds_sao<TW / 2 + 4>(j, s.saop, s.saor, s.safe, tb, Cs, Sc, c + 1, j.wc, j.hc);
-      else lds_copy(Cs, Sc);
+    lwin Sc{nullptr, LD::SCW, max(0, cx0 - 2), max(0, cy0 - 2), 0, 0};
+    Sc.w = min(j.wc, cx1 + 2) - Sc.ox;
+    Sc.h = min(j.hc, cy1 + 2) - Sc.oy;
+    Sc.p = s.o.Sc[0] + (Sc.oy - scoy) * LD::SCW + (Sc.ox - scox);
+    lwin Cs = Cw;
+    Cs.p = s.Cc[0];
+    constexpr int SD1 = LD::CH * LD::CW, DD1 = LD::SCH * LD::SCW;  // Cr after Cb
+    if (j.do_sao) {
+      lds_sao<TW / 2 + 4, false, false, 2>(j, s.saop, s.saorun, s.saor, s.safe, tb, Cs, Sc, 1, SD1, DD1, j.wc, j.hc);
+    } else {
+      for (int c = 0; c < 2; c++) {
+        lwin a = Cs, b = Sc;
+        a.p += c * SD1;
+        b.p += c * DD1;
+        lds_copy(a, b);
+      }
     }
   }
   // ---- ALF: this tile lies inside one CTB
