@@ -14,7 +14,10 @@ The same JSON line also carries:
 - "served": continuous batching of Poisson arrivals (120 pictures/s over 32
   streams of 832x480 / 1920x1080 / 3840x2160) with p50 / p99 latency;
 - "roofline" of the dominant plain kernel and of the fused kernel, from HIP
-  events recorded on the launch stream;
+  events recorded on the launch stream, with the HBM traffic and the vector
+  ALU instruction count per dispatch from the rocprofv3 summaries in
+  profiles/ ("valu": issue rate against 256 CUs x 2.4 GHz x one wave64
+  instruction per CU-cycle, the bound the integer filters actually meet);
 - "cpu_baseline": the CPU oracle timed on host cores.
 """
 import argparse
@@ -124,21 +127,44 @@ def batched_run(vv, ctx, jobs, reps, warmup=2):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_instr.json")
+# Issue ceiling of vector ALU work: 256 CUs x 2.4 GHz x one wave64 VALU
+# instruction per CU per cycle (four SIMD16 units, four cycles per wave64 op).
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
 
 
-def traffic_for(kernel):
-    """HBM bytes per dispatch of `kernel` on this workload (FETCH_SIZE + WRITE_SIZE, calibrated), from the
-    rocprofv3 summary in profiles/, or None when not profiled."""
+def profile_entry(path, kernel):
+    """The entry of a per-kernel rocprofv3 summary in profiles/ whose name
+    starts with `kernel` (template arguments may follow), or None."""
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(path) as fh:
             d = json.load(fh)
     except (OSError, ValueError):
         return None
     for k, v in d.items():
         name = k.replace("void ", "")
         if name.startswith(kernel) and (len(name) == len(kernel) or name[len(kernel)] in "<,"):
-            return int(round(v["read_bytes"] + v["write_bytes"]))
+            return v
     return None
+
+
+def traffic_for(kernel):
+    """HBM bytes per dispatch of `kernel` on this workload (FETCH_SIZE + WRITE_SIZE, calibrated), from the
+    rocprofv3 summary in profiles/, or None when not profiled."""
+    v = profile_entry(TRAFFIC_FILE, kernel)
+    return int(round(v["read_bytes"] + v["write_bytes"])) if v else None
+
+
+def valu_for(kernel, avg_launch_ms):
+    """The kernel's vector-ALU issue rate against the VALU ceiling: SQ_INSTS_VALU
+    per dispatch (rocprofv3 summary in profiles/) over the live launch time."""
+    v = profile_entry(VALU_FILE, kernel)
+    if not v or not avg_launch_ms:
+        return None
+    rate = v["valu_wave_instr"] / (avg_launch_ms / 1e3)
+    return {"wave_instr_per_launch": int(v["valu_wave_instr"]), "achieved_G_per_s": round(rate / 1e9, 1),
+            "peak_G_per_s": round(VALU_PEAK_WAVE_INSTR_PER_S / 1e9, 1),
+            "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
 
 
 def served_run(vv, ctx, seconds, rate, n_streams=32, seed=7):
@@ -249,7 +275,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_for(KERNEL_NAMES[dom]),
                      "algorithmic_bytes_per_launch": dom_bytes,
-                     "avg_launch_ms": round(stage_ms[dom], 5)},
+                     "avg_launch_ms": round(stage_ms[dom], 5),
+                     "valu": valu_for(KERNEL_NAMES[dom], stage_ms[dom])},
     }
     if not args.no_batched:
         log("batched form, same pictures")
@@ -260,7 +287,9 @@ def main():
                           "frac": round(bt["hbm_algorithmic_GBs"] / HBM_PEAK_GBS, 4),
                           "traffic": traffic_for("k_fused<%d, %d" % tuple(int(v) for v in bt["tile"].split("x"))),
                           "algorithmic_bytes_per_launch": CHAIN_BYTES_PER_PX * px,
-                          "avg_launch_ms": bt["ms_per_launch"]}
+                          "avg_launch_ms": bt["ms_per_launch"],
+                          "valu": valu_for("k_fused<%d, %d" % tuple(int(v) for v in bt["tile"].split("x")),
+                                           bt["ms_per_launch"])}
         out["batched"] = {"same_8x4k": bt}
     for j in jobs:
         j.close()
