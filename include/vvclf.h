@@ -342,10 +342,25 @@ void* vvclf_ctx_stream(vvclf_ctx* ctx);
 int vvclf_ctx_synchronize(vvclf_ctx* ctx);
 
 /* Job: one picture plus its side information, resident in HBM.  The host
- * derives the compact per-edge and per-CTB tables once here; input samples
- * are copied from frame->pic.  The input planes stay untouched by runs, so a
- * job can be run repeatedly with identical results. */
+ * derives the compact per-edge and per-CTB tables once here (the CU walk runs
+ * on several host threads for large pictures); input samples are copied from
+ * frame->pic.  The input planes stay untouched by runs, so a job can be run
+ * repeatedly with identical results. */
 int vvclf_job_create(vvclf_ctx* ctx, const vvclf_frame* frame, vvclf_job** out);
+/* As vvclf_job_create.  VVCLF_JOB_GPU_DERIVE asks for the deblocking
+ * descriptors to be derived on the GPU from the uploaded CU records and
+ * motion grid (the walk of deblock_cu_dir, src/deblock.rs:707-983, one thread
+ * per record); that path is switched off in this build and the call returns
+ * VVCLF_ERR_UNSUPPORTED.  No counterpart in the reference. */
+#define VVCLF_JOB_GPU_DERIVE 1u
+int vvclf_job_create_flags(vvclf_ctx* ctx, const vvclf_frame* frame, uint32_t flags, vvclf_job** out);
+/* Copies one of a job's deblocking tables to host memory: which 0..5 the
+ * descriptor maps (luma V, luma H, Cb V, Cr V, Cb H, Cr H; one u32 per
+ * segment), 6..9 the work lists (luma V, luma H, chroma V, chroma H).  *size
+ * receives the table's size in bytes (0 when the job does not deblock that
+ * plane); nothing is copied when cap is smaller.  For tests and tools; no
+ * counterpart in the reference. */
+int vvclf_job_deblock_table(const vvclf_job* job, uint32_t which, void* out, size_t cap, size_t* size);
 int vvclf_job_destroy(vvclf_job* job);
 /* Re-upload input samples (same geometry as at creation). */
 int vvclf_job_set_input(vvclf_job* job, const vvclf_picture* pic);

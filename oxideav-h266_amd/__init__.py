@@ -217,6 +217,8 @@ SIGNATURES = {
     "vvclf_ctx_stream": (VP, [VP]),
     "vvclf_ctx_synchronize": (C.c_int, [VP]),
     "vvclf_job_create": (C.c_int, [VP, P(Frame), P(VP)]),
+    "vvclf_job_create_flags": (C.c_int, [VP, P(Frame), C.c_uint32, P(VP)]),
+    "vvclf_job_deblock_table": (C.c_int, [VP, C.c_uint32, VP, C.c_size_t, P(C.c_size_t)]),
     "vvclf_job_destroy": (C.c_int, [VP]),
     "vvclf_job_set_input": (C.c_int, [VP, P(Picture)]),
     "vvclf_job_run": (C.c_int, [VP, VP]),
@@ -390,9 +392,22 @@ class Context:
 class Job:
     """One picture resident in HBM (vvclf_job_*)."""
 
-    def __init__(self, ctx, frame_ptr):
+    GPU_DERIVE = 1  # VVCLF_JOB_GPU_DERIVE
+
+    def __init__(self, ctx, frame_ptr, flags=0):
         self.h = VP()
-        check(lib.vvclf_job_create(ctx.h, frame_ptr, C.byref(self.h)))
+        check(lib.vvclf_job_create_flags(ctx.h, frame_ptr, flags, C.byref(self.h)))
+
+    def deblock_table(self, which):
+        """One deblocking table (0..5 descriptor maps, 6..9 work lists) as a
+        numpy uint32 array."""
+        import numpy as np
+        size = C.c_size_t()
+        check(lib.vvclf_job_deblock_table(self.h, which, None, 0, C.byref(size)))
+        out = np.zeros(size.value // 4, dtype=np.uint32)
+        if size.value:
+            check(lib.vvclf_job_deblock_table(self.h, which, out.ctypes.data_as(VP), size.value, C.byref(size)))
+        return out
 
     def run(self, stream=None):
         check(lib.vvclf_job_run(self.h, stream))

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ARCH = os.environ.get("VVCLF_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["vvclf_plain.hip", "vvclf_fused.hip", "vvclf_host.cpp", "vvclf_api.cpp", "vvclf_batch.cpp",
+HIP_SOURCES = ["vvclf_plain.hip", "vvclf_fused.hip", "vvclf_derive.hip", "vvclf_host.cpp", "vvclf_api.cpp", "vvclf_batch.cpp",
                "vvclf_server.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
              # packed FP32 instructions are disabled for all device code

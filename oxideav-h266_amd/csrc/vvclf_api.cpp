@@ -6,6 +6,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vvclf.h"
@@ -65,6 +66,13 @@ vvclf_ctx* default_ctx() {
 
 namespace vvclf {
 
+// Host threads for the deblocking-descriptor walk of calls that handle one
+// picture at a time (the serving prep threads use one each).
+static int drop_in_threads() {
+  static const int n = (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+  return n;
+}
+
 int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvclf_job** out,
                       uint64_t* dry_stats, const job_opts* opts, host_tables* keep) {
   const resident_tables* res = opts ? opts->res : nullptr;
@@ -101,11 +109,24 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
     build_lmcs_lut(f->lmcs, t.lmcs_lut);
   }
   bool do_dbk = (stages & STAGE_DBK) && !f->deblock.disabled;
+  const bool gpu_dbk = do_dbk && !dry_stats && opts && opts->gpu_derive;
   if (do_dbk) {
     if (f->deblock.bit_depth < 8 || f->deblock.bit_depth > 16)
       return fail(VVCLF_ERR_INVALID, "deblock: bit_depth %u", f->deblock.bit_depth);
     if (f->n_cus && !f->cus) return fail(VVCLF_ERR_INVALID, "deblock: NULL cus");
-    if ((rc = derive_deblock_maps(*f, W, H, Wc, Hc, t))) return rc;
+    if (gpu_dbk) {  // sizes only; the descriptors are derived on the GPU after the upload
+      t.gw = (W + 3) / 4 + 2;
+      t.gh = (H + 3) / 4 + 2;
+      t.has_chroma_dbk = f->chroma_format_idc != 0 && Wc > 0;
+      if (t.has_chroma_dbk) {
+        t.cvw = (Wc + 7) / 8 + 1;
+        t.cvh = (Hc + 1) / 2 + 1;
+        t.chw = (Wc + 1) / 2 + 1;
+        t.chh = (Hc + 7) / 8 + 1;
+      }
+    } else if ((rc = derive_deblock_maps(*f, W, H, Wc, Hc, t, opts ? opts->host_threads : 1))) {
+      return rc;
+    }
   }
   if ((rc = derive_sao_alf(*f, W, H, t, stages))) return rc;
   if (dry_stats) {
@@ -200,15 +221,44 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   size_t o_sao = sb.add(t.sao), o_saor = sb.add(t.sao_rect);
   size_t o_actb = sb.add(t.alf_ctb), o_arect = sb.add(t.alf_rect), o_bank = sb.add(t.alf_bank);
   size_t o_cbank = sb.add(t.alf_cbank), o_ccbank = sb.add(t.alf_ccbank);
-  j->bytes_side = sb.blob.size();
+  const size_t host_side = sb.blob.size();
+  // GPU derivation: maps and lists live after the uploaded tables
+  size_t tail = (host_side + 255) / 256 * 256;
+  auto reserve = [&](size_t n) {
+    const size_t off = tail;
+    tail = (tail + std::max<size_t>(n, 4) + 255) / 256 * 256;
+    return off;
+  };
+  size_t maps_begin = tail, maps_end = tail;
+  if (gpu_dbk) {
+    const size_t lm = (size_t)t.gw * t.gh * 4, cvm = (size_t)t.cvw * t.cvh * 4, chm = (size_t)t.chw * t.chh * 4;
+    o_lv = reserve(lm);
+    o_lh = reserve(lm);
+    if (t.has_chroma_dbk) {
+      o_cv0 = reserve(cvm);
+      o_cv1 = reserve(cvm);
+      o_ch0 = reserve(chm);
+      o_ch1 = reserve(chm);
+    }
+    maps_end = tail;
+    o_llv = reserve(lm);
+    o_llh = reserve(lm);
+    if (t.has_chroma_dbk) {
+      o_lcv = reserve(2 * cvm);
+      o_lch = reserve(2 * chm);
+    }
+  }
+  j->bytes_side = gpu_dbk ? tail : host_side;
   e = ps ? hipMallocAsync(&j->dev_side, j->bytes_side, ps) : hipMalloc(&j->dev_side, j->bytes_side);
   if (e != hipSuccess) {
     j->dev_side = nullptr;
     vvclf_job_destroy(j);
     return fail(VVCLF_ERR_NOMEM, "hipMalloc side tables: %s", hipGetErrorString(e));
   }
-  e = ps ? hipMemcpyAsync(j->dev_side, sb.blob.data(), j->bytes_side, hipMemcpyHostToDevice, ps)
-         : hipMemcpy(j->dev_side, sb.blob.data(), j->bytes_side, hipMemcpyHostToDevice);
+  e = ps ? hipMemcpyAsync(j->dev_side, sb.blob.data(), host_side, hipMemcpyHostToDevice, ps)
+         : hipMemcpy(j->dev_side, sb.blob.data(), host_side, hipMemcpyHostToDevice);
+  if (e == hipSuccess && gpu_dbk)
+    e = hipMemsetAsync((uint8_t*)j->dev_side + maps_begin, 0, maps_end - maps_begin, ps);
   if (e == hipSuccess && ps) e = hipStreamSynchronize(ps);
   if (e != hipSuccess) {
     vvclf_job_destroy(j);
@@ -236,6 +286,35 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   d.n_lh = (int32_t)t.l_lh.size();
   d.n_cv = (int32_t)t.l_cv.size();
   d.n_ch = (int32_t)t.l_ch.size();
+  if (gpu_dbk) {
+    gpu_maps gm;
+    gm.e_lv = (uint32_t*)(sd + o_lv);
+    gm.e_lh = (uint32_t*)(sd + o_lh);
+    gm.e_cv[0] = (uint32_t*)(sd + o_cv0);
+    gm.e_cv[1] = (uint32_t*)(sd + o_cv1);
+    gm.e_ch[0] = (uint32_t*)(sd + o_ch0);
+    gm.e_ch[1] = (uint32_t*)(sd + o_ch1);
+    gm.gw = t.gw;
+    gm.gh = t.gh;
+    gm.cvw = t.cvw;
+    gm.cvh = t.cvh;
+    gm.chw = t.chw;
+    gm.chh = t.chh;
+    gm.l_lv = (uint32_t*)(sd + o_llv);
+    gm.l_lh = (uint32_t*)(sd + o_llh);
+    gm.l_cv = (uint32_t*)(sd + o_lcv);
+    gm.l_ch = (uint32_t*)(sd + o_lch);
+    gm.chroma = t.has_chroma_dbk;
+    uint32_t n[4];
+    if ((rc = derive_deblock_gpu(*f, W, H, Wc, Hc, gm, ps, n))) {
+      vvclf_job_destroy(j);
+      return rc;
+    }
+    d.n_lv = (int32_t)n[0];
+    d.n_lh = (int32_t)n[1];
+    d.n_cv = (int32_t)n[2];
+    d.n_ch = (int32_t)n[3];
+  }
   d.sao = (const uint64_t*)(sd + o_sao);
   d.sao_rect = t.sao_rect.empty() ? nullptr : (const uint64_t*)(sd + o_saor);
   d.alf_ctb = (const uint32_t*)(sd + o_actb);
@@ -384,12 +463,49 @@ int vvclf_ctx_synchronize(vvclf_ctx* ctx) {
 }
 
 int vvclf_job_create(vvclf_ctx* ctx, const vvclf_frame* frame, vvclf_job** out) {
-  return job_create_stages(ctx, frame, STAGE_ALL, out, nullptr);
+  return vvclf_job_create_flags(ctx, frame, 0, out);
+}
+
+int vvclf_job_create_flags(vvclf_ctx* ctx, const vvclf_frame* frame, uint32_t flags, vvclf_job** out) {
+  clear_error();
+  if (flags & ~VVCLF_JOB_GPU_DERIVE) return fail(VVCLF_ERR_INVALID, "job_create_flags: unknown flags %#x", flags);
+  if (flags & VVCLF_JOB_GPU_DERIVE)
+    return fail(VVCLF_ERR_UNSUPPORTED, "GPU derivation of the deblocking descriptors is switched off");
+  job_opts o;
+  o.host_threads = drop_in_threads();
+  return job_create_stages(ctx, frame, STAGE_ALL, out, nullptr, &o);
+}
+
+int vvclf_job_deblock_table(const vvclf_job* job, uint32_t which, void* out, size_t cap, size_t* size) {
+  clear_error();
+  if (!job || !size || which > 9) return fail(VVCLF_ERR_INVALID, "job_deblock_table: bad argument");
+  const dev_job& d = job->d;
+  const uint32_t* src = nullptr;
+  size_t n = 0;
+  if (d.do_dbk) {
+    switch (which) {
+      case 0: src = d.e_lv; n = (size_t)d.gw * d.gh; break;
+      case 1: src = d.e_lh; n = (size_t)d.gw * d.gh; break;
+      case 2: case 3: src = d.e_cv[which - 2]; n = d.do_dbk_chroma ? (size_t)d.cvw * d.cvh : 0; break;
+      case 4: case 5: src = d.e_ch[which - 4]; n = d.do_dbk_chroma ? (size_t)d.chw * d.chh : 0; break;
+      case 6: src = d.l_lv; n = (size_t)d.n_lv; break;
+      case 7: src = d.l_lh; n = (size_t)d.n_lh; break;
+      case 8: src = d.l_cv; n = (size_t)d.n_cv; break;
+      default: src = d.l_ch; n = (size_t)d.n_ch; break;
+    }
+  }
+  *size = n * 4;
+  if (!n || !out || cap < n * 4) return VVCLF_OK;
+  hipSetDevice(job->ctx->device);
+  hipError_t e = hipMemcpy(out, src, n * 4, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? VVCLF_OK : hip_fail(e, "job_deblock_table");
 }
 
 int vvclf_frame_check(const vvclf_frame* frame, uint64_t stats[8]) {
   uint64_t local[8];
-  return job_create_stages(nullptr, frame, STAGE_ALL, nullptr, stats ? stats : local);
+  job_opts o;
+  o.host_threads = drop_in_threads();
+  return job_create_stages(nullptr, frame, STAGE_ALL, nullptr, stats ? stats : local, &o);
 }
 
 int vvclf_job_destroy(vvclf_job* j) {
@@ -580,6 +696,7 @@ static int run_sync(vvclf_frame* f, uint32_t stages) {
   job_opts o;
   o.stream = ctx->stream;
   o.stage = &stage;
+  o.host_threads = drop_in_threads();
   vvclf_job* j = nullptr;
   int rc = job_create_stages(ctx, f, stages, &j, nullptr, &o);
   if (rc) return rc;

@@ -77,7 +77,7 @@ int vvclf_batch_create(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, 
   int occ = fused_max_blocks_per_cu((int)t.tile_w, (int)t.tile_h, hbd);
   if (occ <= 0) return fail(VVCLF_ERR_HIP, "batch: fused kernel does not fit on a CU");
   if (t.wgs_per_cu == 0 || (int)t.wgs_per_cu > occ) t.wgs_per_cu = (uint32_t)occ;
-  t.threads = 256;
+  t.threads = (uint32_t)fused_threads((int)t.tile_w, (int)t.tile_h);
   // tile list: largest pictures first, raster order within a picture
   std::vector<uint32_t> order(n_jobs);
   for (uint32_t i = 0; i < n_jobs; i++) order[i] = i;
@@ -209,7 +209,7 @@ int vvclf_autotune(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, vvcl
     if (!shape_fits(js, sh[0], sh[1])) continue;
     int occ = fused_max_blocks_per_cu(sh[0], sh[1], hbd);
     for (int w = 1; w <= occ; w++) {
-      vvclf_tune t{(uint32_t)sh[0], (uint32_t)sh[1], 256, (uint32_t)w};
+      vvclf_tune t{(uint32_t)sh[0], (uint32_t)sh[1], (uint32_t)fused_threads(sh[0], sh[1]), (uint32_t)w};
       vvclf_batch* b = nullptr;
       if (vvclf_batch_create(ctx, jobs, n_jobs, &t, &b) != VVCLF_OK) continue;
       float ms = 0.f;

@@ -772,13 +772,84 @@ __device__ __forceinline__ int alf_lds_block_class(const uint16_t* base, int vb,
   return carve ? alf_lds_class<2, true, PITCH>(base, bd) : alf_lds_class<2, false, PITCH>(base, bd);
 }
 
+// Virtual-boundary variant of the block at yb = y4 - yCtb: 0 none, 1 the block
+// above the boundary row, 2 the block below it.
+__device__ __forceinline__ int alf_vb_kind(int yb, int ctb) { return yb == ctb - 8 ? 1 : (yb == ctb - 4 ? 2 : 0); }
+
+// N rows of 12 samples (6 aligned pairs) starting at `base` (8-byte aligned).
+template <int PITCH, int N>
+__device__ __forceinline__ void load_pairs(const uint16_t* base, uint32_t (&P)[N][6]) {
+#pragma unroll
+  for (int rr = 0; rr < N; rr++) {
+    const uint2* q = reinterpret_cast<const uint2*>(base + rr * PITCH);
+    const uint2 a = q[0], b = q[1], c = q[2];
+    P[rr][0] = a.x; P[rr][1] = a.y; P[rr][2] = b.x; P[rr][3] = b.y; P[rr][4] = c.x; P[rr][5] = c.y;
+  }
+}
+
+// ALF luma of the 4x4 block at (x4, y4) from an LDS buffer S (pitch SW,
+// origin (sox, soy)) holding the SAO output of [x4-4, x4+8) x [y4-3, y4+7)
+// with every position outside the CTB's fetch rectangle already replaced by
+// the clamped sample.  outp[r] receives row y4+r as two u16 pairs.  The
+// classification streams three window rows at a time and the filter runs in
+// two halves of two output rows, each reloading its 8 window rows: the
+// compiler barriers keep the loads apart so that fewer registers are live.
+// alf_on = false copies the block.  (src/alf.rs:822-1010, :1052-1180.)
+template <int SW, bool HBD>
+__device__ __forceinline__ void alf_luma_block_lds(const uint16_t* S, int sox, int soy, int x4, int y4,
+                                                   const uint32_t* bank, bool alf_on, int y_ctb, int ctb, bool carve,
+                                                   int alf_bd, uint32_t (&outp)[4][2]) {
+  const int amax = (1 << alf_bd) - 1;
+  if (!alf_on) {
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const uint2 q = *reinterpret_cast<const uint2*>(&S[(y4 + rr - soy) * SW + (x4 - sox)]);
+      outp[rr][0] = q.x;
+      outp[rr][1] = q.y;
+    }
+  } else if (HBD) {
+    int Wn[10][10];
+#pragma unroll
+    for (int rr = 0; rr < 10; rr++)
+#pragma unroll
+      for (int cc = 0; cc < 10; cc++) Wn[rr][cc] = S[(y4 - 3 + rr - soy) * SW + (x4 - 3 + cc - sox)];
+    int outv[4][4];
+    alf_block(Wn, bank, y4 - y_ctb, ctb, carve, alf_bd, amax, outv);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      outp[rr][0] = (uint32_t)outv[rr][0] | ((uint32_t)outv[rr][1] << 16);
+      outp[rr][1] = (uint32_t)outv[rr][2] | ((uint32_t)outv[rr][3] << 16);
+    }
+  } else {
+    const uint16_t* base = &S[(y4 - 3 - soy) * SW + (x4 - 4 - sox)];
+    const int vb = alf_vb_kind(y4 - y_ctb, ctb);
+    const int cls = alf_lds_block_class<SW>(base, vb, carve, alf_bd);
+    const uint32_t* f = bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
+    __asm__ volatile("" ::: "memory");
+    {
+      uint32_t Q[8][6];
+      load_pairs<SW>(base, Q);
+      if (vb == 0) alf_pk_filter_rows<0, 0, 2, 8>(Q, f, amax, outp);
+      else if (vb == 1) alf_pk_filter_rows<1, 0, 2, 8>(Q, f, amax, outp);
+      else alf_pk_filter_rows<2, 0, 2, 8>(Q, f, amax, outp);
+    }
+    __asm__ volatile("" ::: "memory");
+    {
+      uint32_t Q[8][6];
+      load_pairs<SW>(base + 2 * SW, Q);
+      if (vb == 0) alf_pk_filter_rows<0, 2, 2, 8>(Q, f, amax, outp);
+      else if (vb == 1) alf_pk_filter_rows<1, 2, 2, 8>(Q, f, amax, outp);
+      else alf_pk_filter_rows<2, 2, 2, 8>(Q, f, amax, outp);
+    }
+  }
+}
+
 // Class of a block at yb = y4 - yCtb (virtual-boundary variant picked here).
 __device__ __forceinline__ int alf_pk_block_class(const uint32_t (&P)[10][6], int vb, bool carve, int bd) {
   if (vb == 0) return alf_pk_class<0, false>(P, bd);
   if (vb == 1) return carve ? alf_pk_class<1, true>(P, bd) : alf_pk_class<1, false>(P, bd);
   return carve ? alf_pk_class<2, true>(P, bd) : alf_pk_class<2, false>(P, bd);
 }
-__device__ __forceinline__ int alf_vb_kind(int yb, int ctb) { return yb == ctb - 8 ? 1 : (yb == ctb - 4 ? 2 : 0); }
 
 // Classification + filter of one block from the packed window; `bank` holds
 // the CTB's 25 x 4 x 12 tap words, yb = y4 - yCtb.

@@ -23,6 +23,10 @@ namespace vvclf {
 
 template <int TW, int TH>
 struct fused_lds {
+  // threads per workgroup (four waves, one per SIMD; a workgroup of six
+  // waves would place two on some SIMDs, which the register budget of a
+  // second workgroup then cannot match)
+  static constexpr int NT = 256;
   static constexpr int LW = TW + 32, LH = TH + 32;          // luma input region
   static constexpr int CW = TW / 2 + 8, CH = TH / 2 + 8;    // chroma input region
   static constexpr int SW = TW + 8, SH = TH + 6;            // SAO'd luma, origin (x0-4, y0-3)
@@ -97,15 +101,15 @@ struct mrect {
 // words, LDS row pitch MW); issue() for every rectangle first, then
 // commit(), so that all loads of a tile are in flight together.  Element i
 // covers LDS row i / MW, column i % MW (compile-time divisor).
-template <int MW, int MH>
+template <int MW, int MH, int NT>
 struct rect_stage {
-  static constexpr int PER = (MW * MH + 255) / 256;
+  static constexpr int PER = (MW * MH + NT - 1) / NT;
   uint32_t v[PER];
   // map == nullptr: nothing to load (every element 0)
   __device__ __forceinline__ void issue(const uint32_t* map, int mw, const mrect& r) {
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = ltid() + k * 256, yy = i / MW, xx = i - yy * MW;
+      const int i = ltid() + k * NT, yy = i / MW, xx = i - yy * MW;
       v[k] = (map && yy < r.ny && xx < r.nx) ? gld(&map[(size_t)(r.y0 + yy) * mw + r.x0 + xx]) : 0u;
     }
   }
@@ -114,7 +118,7 @@ struct rect_stage {
                                          uint32_t tag) const {
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = ltid() + k * 256, yy = i / MW, xx = i - yy * MW;
+      const int i = ltid() + k * NT, yy = i / MW, xx = i - yy * MW;
       const bool in = yy < r.ny && xx < r.nx;
       if (in) dst[i] = v[k];
       list_append(in && v[k] != 0, tag | (uint32_t)(yy << 6) | (uint32_t)xx, list, cnt);
@@ -300,7 +304,7 @@ struct sao_tab {
 // are 8-byte aligned (else 4).  NPL = 2 walks both chroma planes in one loop
 // (component comp0 + plane; plane 1 at src.p + sd1 / dst.p + dd1, same
 // geometry).
-template <int RW, bool SRC16, bool DST8, int NPL>
+template <int RW, bool SRC16, bool DST8, int NPL, int NT>
 __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)[3], const uint4 (*runs)[3],
                                         const uint64_t* saor, const uint64_t (*safe)[3], const sao_tab& tb,
                                         const lwin& src0, const lwin& dst0, int comp0, int sd1, int dd1, int pw,
@@ -312,7 +316,7 @@ __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)
   const int ka = dst0.ox >> 3, nk = ((dst0.ox + dst0.w - 1) >> 3) - ka + 1;
   constexpr int NK = (RW + 14) / 8;  // runs per row, at most
   const int per_plane = NK * dst0.h;
-  for (int i0 = ltid(); i0 < NPL * per_plane; i0 += 256) {
+  for (int i0 = ltid(); i0 < NPL * per_plane; i0 += NT) {
     const int pl = (NPL > 1 && i0 >= per_plane) ? 1 : 0, i = i0 - pl * per_plane;
     const int comp = comp0 + pl;
     lwin src = src0, dst = dst0;
@@ -451,17 +455,6 @@ __device__ __forceinline__ void pad_to_rect(uint16_t* buf, int pitch, int ox, in
   }
 }
 
-// N rows of 12 samples (6 aligned pairs) starting at `base` (8-byte aligned).
-template <int PITCH, int N>
-__device__ __forceinline__ void load_pairs(const uint16_t* base, uint32_t (&P)[N][6]) {
-#pragma unroll
-  for (int rr = 0; rr < N; rr++) {
-    const uint2* q = reinterpret_cast<const uint2*>(base + rr * PITCH);
-    const uint2 a = q[0], b = q[1], c = q[2];
-    P[rr][0] = a.x; P[rr][1] = a.y; P[rr][2] = b.x; P[rr][3] = b.y; P[rr][4] = c.x; P[rr][5] = c.y;
-  }
-}
-
 // Geometry of one tile: the windows, edge-descriptor rectangles and SAO
 // table origin that follow from the job and the tile origin.
 struct tile_geom {
@@ -541,16 +534,16 @@ __device__ __forceinline__ tile_geom tile_geometry(const dev_job& j, fused_lds<T
 template <int TW, int TH>
 struct tile_loads {
   typedef fused_lds<TW, TH> LD;
-  static constexpr int NT = 256;
+  static constexpr int NT = LD::NT;
   static constexpr int NCH = LD::LW / 8, PER = (LD::LH * NCH + NT - 1) / NT;
   static constexpr int NC4 = LD::CW / 4, PERC = (LD::CH * NC4 + NT - 1) / NT;  // per chroma plane
   static constexpr int PBK = (ALF_BANK_WORDS + NT - 1) / NT;
   uint4 v[PER];
   uint2 vc[2][PERC];
-  rect_stage<LD::MLV_W, LD::MLV_H> slv;
-  rect_stage<LD::MLH_W, LD::MLH_H> slh;
-  rect_stage<LD::MCV_W, LD::MCV_H> scv0, scv1;
-  rect_stage<LD::MCH_W, LD::MCH_H> sch0, sch1;
+  rect_stage<LD::MLV_W, LD::MLV_H, NT> slv;
+  rect_stage<LD::MLH_W, LD::MLH_H, NT> slh;
+  rect_stage<LD::MCV_W, LD::MCV_H, NT> scv0, scv1;
+  rect_stage<LD::MCH_W, LD::MCH_H, NT> sch0, sch1;
   uint32_t bk[PBK];
   uint64_t small;  // chroma ALF / CC-ALF bank words (threads 0..103)
   uint64_t sv;     // SAO cell parameters (threads 128..163)
@@ -715,7 +708,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
                                            tile_loads<TW, TH>& ld, const dev_job* __restrict__ jobs, bool more,
                                            const tile_ref& ntr) {
   typedef fused_lds<TW, TH> LD;
-  constexpr int NT = 256;  // threads per workgroup
+  constexpr int NT = LD::NT;  // threads per workgroup
   const tile_geom g = tile_geometry<TW, TH>(j, s, tr);
   const int x0 = g.x0, y0 = g.y0, x1 = g.x1, y1 = g.y1;
   const int cx0 = g.cx0, cy0 = g.cy0, cx1 = g.cx1, cy1 = g.cy1;
@@ -756,7 +749,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   S.w = min(W, x1 + 3) - S.ox;
   S.h = min(H, y1 + 3) - S.oy;
   S.p = s.o.S + (S.oy - soy) * LD::SW + (S.ox - sox);
-  if (j.do_sao) lds_sao<TW + 6, true, true, 1>(j, s.saop, s.saorun, s.saor, s.safe, tb, L, S, 0, 0, 0, W, H);
+  if (j.do_sao) lds_sao<TW + 6, true, true, 1, NT>(j, s.saop, s.saorun, s.saor, s.safe, tb, L, S, 0, 0, 0, W, H);
   else lds_copy(L, S);
   if (chroma) {
     lwin Sc{nullptr, LD::SCW, max(0, cx0 - 2), max(0, cy0 - 2), 0, 0};
@@ -767,7 +760,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     Cs.p = s.Cc[0];
     constexpr int SD1 = LD::CH * LD::CW, DD1 = LD::SCH * LD::SCW;  // Cr after Cb
     if (j.do_sao) {
-      lds_sao<TW / 2 + 4, false, false, 2>(j, s.saop, s.saorun, s.saor, s.safe, tb, Cs, Sc, 1, SD1, DD1, j.wc, j.hc);
+      lds_sao<TW / 2 + 4, false, false, 2, NT>(j, s.saop, s.saorun, s.saor, s.safe, tb, Cs, Sc, 1, SD1, DD1, j.wc, j.hc);
     } else {
       for (int c = 0; c < 2; c++) {
         lwin a = Cs, b = Sc;
@@ -807,52 +800,8 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     for (int b = ltid(); b < bw * bh; b += NT) {
       const int x4 = x0 + 4 * (b % bw), y4 = y0 + 4 * (b / bw);
       uint32_t outp[4][2];
-      if (bank_idx == (int)ALF_OFF8) {
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-          const uint2 q = *reinterpret_cast<const uint2*>(&s.o.S[(y4 + rr - soy) * LD::SW + (x4 - sox)]);
-          outp[rr][0] = q.x;
-          outp[rr][1] = q.y;
-        }
-      } else if (HBD) {
-        int Wn[10][10];
-#pragma unroll
-        for (int rr = 0; rr < 10; rr++)
-#pragma unroll
-          for (int cc = 0; cc < 10; cc++) Wn[rr][cc] = Sl(x4 - 3 + cc, y4 - 3 + rr);
-        int outv[4][4];
-        alf_block(Wn, s.bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outv);
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-          outp[rr][0] = (uint32_t)outv[rr][0] | ((uint32_t)outv[rr][1] << 16);
-          outp[rr][1] = (uint32_t)outv[rr][2] | ((uint32_t)outv[rr][3] << 16);
-        }
-      } else {
-        // classification over the 10-row neighbourhood, then the filter in
-        // two halves of two output rows, each reloading its 8 window rows:
-        // the compiler barriers keep the loads apart so that fewer
-        // registers are live at once
-        const uint16_t* base = &s.o.S[(y4 - 3 - soy) * LD::SW + (x4 - 4 - sox)];
-        const int vb = alf_vb_kind(y4 - y_ctb, ctb);
-        const int cls = alf_lds_block_class<LD::SW>(base, vb, carve, j.alf_bd);
-        const uint32_t* f = s.bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
-        __asm__ volatile("" ::: "memory");
-        {
-          uint32_t Q[8][6];
-          load_pairs<LD::SW>(base, Q);
-          if (vb == 0) alf_pk_filter_rows<0, 0, 2, 8>(Q, f, amax, outp);
-          else if (vb == 1) alf_pk_filter_rows<1, 0, 2, 8>(Q, f, amax, outp);
-          else alf_pk_filter_rows<2, 0, 2, 8>(Q, f, amax, outp);
-        }
-        __asm__ volatile("" ::: "memory");
-        {
-          uint32_t Q[8][6];
-          load_pairs<LD::SW>(base + 2 * LD::SW, Q);
-          if (vb == 0) alf_pk_filter_rows<0, 2, 2, 8>(Q, f, amax, outp);
-          else if (vb == 1) alf_pk_filter_rows<1, 2, 2, 8>(Q, f, amax, outp);
-          else alf_pk_filter_rows<2, 2, 2, 8>(Q, f, amax, outp);
-        }
-      }
+      alf_luma_block_lds<LD::SW, HBD>(s.o.S, sox, soy, x4, y4, s.bank, bank_idx != (int)ALF_OFF8, y_ctb, ctb, carve,
+                                      j.alf_bd, outp);
 #pragma unroll
       for (int rr = 0; rr < 4; rr++)
         *reinterpret_cast<uint2*>(&ob[(y4 - y0 + rr) * TW + (x4 - x0)]) = make_uint2(outp[rr][0], outp[rr][1]);
@@ -1001,7 +950,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
 // XCD (workgroups are dispatched round-robin over the 8 XCDs), so spatially
 // adjacent tiles - which share halo samples - hit the same L2.
 template <int TW, int TH, bool HBD>
-__global__ void __launch_bounds__(256, 3) k_fused(const dev_job* __restrict__ jobs, const tile_ref* __restrict__ tiles,
+__global__ void __launch_bounds__((fused_lds<TW, TH>::NT), 3) k_fused(const dev_job* __restrict__ jobs, const tile_ref* __restrict__ tiles,
                                                uint32_t n_tiles) {
   __shared__ fused_lds<TW, TH> s;
   const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;
@@ -1049,17 +998,21 @@ namespace {
 template <int TW, int TH>
 int occupancy_of(bool hbd) {
   int n = 0;
-  hipError_t e = hbd ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<TW, TH, true>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<TW, TH, false>, 256, 0);
+  constexpr int NT = fused_lds<TW, TH>::NT;
+  hipError_t e = hbd ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<TW, TH, true>, NT, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<TW, TH, false>, NT, 0);
   return e == hipSuccess ? n : 0;
 }
 template <int TW, int TH>
 void launch_shape(bool hbd, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles, uint32_t grid,
                   hipStream_t s) {
-  if (hbd) hipLaunchKernelGGL((k_fused<TW, TH, true>), dim3(grid), dim3(256), 0, s, jobs, tiles, n_tiles);
-  else hipLaunchKernelGGL((k_fused<TW, TH, false>), dim3(grid), dim3(256), 0, s, jobs, tiles, n_tiles);
+  constexpr int NT = fused_lds<TW, TH>::NT;
+  if (hbd) hipLaunchKernelGGL((k_fused<TW, TH, true>), dim3(grid), dim3(NT), 0, s, jobs, tiles, n_tiles);
+  else hipLaunchKernelGGL((k_fused<TW, TH, false>), dim3(grid), dim3(NT), 0, s, jobs, tiles, n_tiles);
 }
 }  // namespace
+
+int fused_threads(int tw, int th) { return tw * th > 64 * 64 ? fused_lds<128, 64>::NT : fused_lds<64, 64>::NT; }
 
 int fused_max_blocks_per_cu(int tw, int th, bool hbd) {
   if (tw == 64 && th == 64) return occupancy_of<64, 64>(hbd);

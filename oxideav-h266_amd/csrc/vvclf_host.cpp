@@ -4,15 +4,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vvclf.h"
 #include "vvclf_host_internal.h"
 #include "vvclf_internal.h"
+#include "vvclf_walk.h"
 
 namespace vvclf {
 
@@ -58,327 +62,63 @@ struct cu_grid {
         for (uint32_t cx = x0; cx < x1; cx++) cells[(size_t)cy * w + cx] = (int32_t)i;
     }
   }
-  int32_t at(int32_t x, int32_t y) const {
-    if (x < 0 || y < 0) return -1;
-    uint32_t cx = (uint32_t)x / 4, cy = (uint32_t)y / 4;
-    if (cx >= w || cy >= h) return -1;
-    return cells[(size_t)cy * w + cx];
-  }
+  walk::grid_view view() const { return walk::grid_view{cells.data(), w, h}; }
 };
 
-inline uint32_t clampu_rel(int32_t v, uint32_t hi_excl) {
-  if (v < 0) return 0;
-  if ((uint32_t)v >= hi_excl) return hi_excl - 1;
-  return (uint32_t)v;
-}
-
-// CU-relative offset along the split / edge-normal axis.
-inline uint32_t rel_along(const vvclf_deblock_cu& c, int32_t lx, int32_t ly, bool vert) {
-  return vert ? clampu_rel(lx - (int32_t)c.x, c.w) : clampu_rel(ly - (int32_t)c.y, c.h);
-}
-
-inline uint32_t tb_idx(const vvclf_deblock_cu& c, uint32_t rel) {
-  uint32_t idx = 0;
-  for (uint32_t i = 0; i < c.tb_n_bounds && i < 3; i++)
-    if (rel >= c.tb_bounds[i]) idx = i + 1;
-  return idx;
-}
-
-inline uint32_t tb_len_split(const vvclf_deblock_cu& c, uint32_t rel, uint32_t dim) {
-  uint32_t s = 0, e = dim;
-  for (uint32_t i = 0; i < c.tb_n_bounds && i < 3; i++) {
-    if (rel < c.tb_bounds[i]) {
-      e = c.tb_bounds[i];
-      break;
-    }
-    s = c.tb_bounds[i];
-  }
-  return e - s;
-}
-
-// DeblockCu::tu_coded_at, src/deblock.rs:168-201
-bool tu_coded(const vvclf_deblock_cu& c, uint32_t comp, int32_t lx, int32_t ly) {
-  if (c.has_tu64) {
-    uint32_t tx = std::min(clampu_rel(lx - (int32_t)c.x, c.w) / 64, 1u);
-    uint32_t ty = std::min(clampu_rel(ly - (int32_t)c.y, c.h) / 64, 1u);
-    return comp == 0 ? c.tu64_y[ty][tx] : (comp == 1 ? c.tu64_cb[ty][tx] : c.tu64_cr[ty][tx]);
-  }
-  if (c.has_tb_split && (comp == 0 || !c.tb_luma_only)) {
-    uint32_t i = tb_idx(c, rel_along(c, lx, ly, c.tb_vertical != 0));
-    return comp == 0 ? c.tb_y_coded[i] : (comp == 1 ? c.tb_cb_coded[i] : c.tb_cr_coded[i]);
-  }
-  return comp == 0 ? c.tu_y_coded : (comp == 1 ? c.tu_cb_coded : c.tu_cr_coded);
-}
-
-// luma_tb_len / chroma_tb_len, src/deblock.rs:207-248
-uint32_t tb_len(const vvclf_deblock_cu& c, int32_t lx, int32_t ly, bool vert, bool chroma) {
-  uint32_t dim = vert ? c.w : c.h;
-  uint32_t rel = rel_along(c, lx, ly, vert);
-  if (c.has_tu64 && dim > 64) return std::min(dim - (rel / 64) * 64, 64u);
-  if (c.has_tb_split && (c.tb_vertical != 0) == vert && (!chroma || !c.tb_luma_only)) return tb_len_split(c, rel, dim);
-  return dim;
-}
-
-// is_luma_tb_boundary / is_chroma_tb_boundary, src/deblock.rs:252-291
-bool tb_boundary(const vvclf_deblock_cu& c, uint32_t off, bool vert, bool chroma) {
-  if (off == 0) return false;
-  uint32_t dim = vert ? c.w : c.h;
-  if (off >= dim) return false;
-  if (c.has_tu64 && dim > 64) return off % 64 == 0;
-  if (c.has_tb_split && (c.tb_vertical != 0) == vert && (!chroma || !c.tb_luma_only)) {
-    for (uint32_t i = 0; i < c.tb_n_bounds && i < 3; i++)
-      if (c.tb_bounds[i] == off) return true;
-  }
-  return false;
-}
-
-vvclf_mv_cell mv_at(const vvclf_mv_grid* g, int32_t x, int32_t y) {
-  vvclf_mv_cell z;
-  memset(&z, 0, sizeof(z));
-  if (!g || x < 0 || y < 0) return z;
-  uint32_t cx = (uint32_t)x / 4, cy = (uint32_t)y / 4;
-  if (cx >= g->cells_w || cy >= g->cells_h) return z;
-  return g->cells[(size_t)cy * g->cells_w + cx];
-}
-
-inline bool mv_far(const int32_t a[2], const int32_t b[2]) { return std::abs(a[0] - b[0]) >= 8 || std::abs(a[1] - b[1]) >= 8; }
-
-// bs_motion_rule, src/deblock.rs:992-1045
-bool motion_differs(const vvclf_mv_cell& p, const vvclf_mv_cell& q) {
-  int np = (p.pred_l0 != 0) + (p.pred_l1 != 0), nq = (q.pred_l0 != 0) + (q.pred_l1 != 0);
-  if (np != nq) return true;
-  if (np == 0) return false;
-  if (np == 1) {
-    int32_t pp = p.pred_l0 ? p.poc_l0 : p.poc_l1, qp = q.pred_l0 ? q.poc_l0 : q.poc_l1;
-    if (pp != qp) return true;
-    return mv_far(p.pred_l0 ? p.mv_l0 : p.mv_l1, q.pred_l0 ? q.mv_l0 : q.mv_l1);
-  }
-  bool same = p.poc_l0 == q.poc_l0 && p.poc_l1 == q.poc_l1;
-  bool swapped = p.poc_l0 == q.poc_l1 && p.poc_l1 == q.poc_l0;
-  if (!same && !swapped) return true;
-  if (p.poc_l0 != p.poc_l1) {
-    if (same) return mv_far(p.mv_l0, q.mv_l0) || mv_far(p.mv_l1, q.mv_l1);
-    return mv_far(p.mv_l0, q.mv_l1) || mv_far(p.mv_l1, q.mv_l0);
-  }
-  return (mv_far(p.mv_l0, q.mv_l0) || mv_far(p.mv_l1, q.mv_l1)) &&
-         (mv_far(p.mv_l0, q.mv_l1) || mv_far(p.mv_l1, q.mv_l0));
-}
-
-// derive_bs, src/deblock.rs:1055-1119
-int boundary_strength(uint32_t comp, const vvclf_deblock_cu& p, const vvclf_deblock_cu& q, const vvclf_mv_grid* g,
-                      int32_t px, int32_t py, int32_t qx, int32_t qy, bool is_tb, bool idc2) {
-  if (comp == 0 && p.bdpcm_luma && q.bdpcm_luma) return 0;
-  if (comp != 0 && p.bdpcm_chroma && q.bdpcm_chroma) return 0;
-  if (p.intra || q.intra || p.ciip || q.ciip) return 2;
-  if (is_tb && (tu_coded(p, comp, px, py) || tu_coded(q, comp, qx, qy))) return 1;
-  if (comp != 0 || !idc2) return 0;
-  if ((p.ibc != 0) != (q.ibc != 0)) return 1;
-  if (p.ibc && q.ibc) {
-    if (!g) return 0;
-    vvclf_mv_cell a = mv_at(g, px, py), b = mv_at(g, qx, qy);
-    return mv_far(a.mv_l0, b.mv_l0) ? 1 : 0;
-  }
-  if (g && motion_differs(mv_at(g, px, py), mv_at(g, qx, qy))) return 1;
-  return 0;
-}
-
-int32_t tc_value(int32_t qt, uint32_t bd) {
-  static const int16_t TC[66] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,   0,   0,   0,   0,   0,   0,   0,
-                                 0,  3,  4,  4,  4,  4,  5,  5,  5,  5,   7,   7,   8,   9,   10,  10,  11,
-                                 13, 14, 15, 17, 19, 21, 24, 25, 29, 33,  36,  41,  45,  51,  57,  64,  71,
-                                 80, 89, 100, 112, 125, 141, 157, 177, 198, 222, 250, 280, 314, 352, 395};
-  int32_t t = TC[qt];
-  return bd < 10 ? (t + (1 << (9 - bd))) >> (10 - bd) : t * (1 << (bd - 10));
-}
-
-struct plane_params {
-  uint32_t comp, sub_w, sub_h;
-  int32_t beta_off, tc_off, qp_off;
-  uint32_t bd, ctb;
-};
-
-int32_t chroma_qp(const plane_params& pp, const vvclf_deblock_cu& c) {
-  if (c.qp_c[0] == INT32_MIN) return std::clamp(c.qp_y + pp.qp_off, 0, 63);
-  if (c.joint_cbcr2) return c.qp_c[2];
-  return c.qp_c[std::min(pp.comp - 1, 1u)];
-}
-
+// One direction's descriptors of one plane (claim marks stripped by
+// finish_map).
 struct edge_map {
   std::vector<uint32_t> e;
-  std::vector<uint8_t> seen;
   uint32_t w = 0, h = 0;
   void init(uint32_t w_, uint32_t h_) {
     w = w_;
     h = h_;
     e.assign((size_t)w * h, 0);
-    seen.assign((size_t)w * h, 0);
   }
+  walk::map_view view() { return walk::map_view{e.data(), w, h}; }
 };
 
-// deblock_cu_dir, src/deblock.rs:707-983, emitting descriptors.  Returns
-// false (with g_err set) when two records claim one segment (the records do
-// not tile the picture, the precondition of src/deblock.rs:463-465).
-bool walk_cu(const plane_params& pp, const vvclf_deblock_cu* cus, const cu_grid& grid, const vvclf_mv_grid* mv,
-             uint32_t idx, bool vert, const uint32_t* nf, uint32_t n_nf, uint32_t pic_w, uint32_t pic_h,
-             edge_map& m, uint64_t* hist) {
-  const vvclf_deblock_cu& cu = cus[idx];
-  const uint32_t comp = pp.comp;
-  if (comp != 0) {
-    uint32_t lead_c = vert ? cu.x / pp.sub_w : cu.y / pp.sub_h;
-    if (lead_c % 8 != 0) return true;
+// Strips the claim marks of a map, lists its nonzero descriptors as
+// tag | row << 16 | column (row-major), and for a luma map (luma_dir 1
+// vertical, 2 horizontal) checks every pair of consecutive same-direction
+// edges on a line with luma_pair_safe.  Returns false if a pair interacts.
+bool finish_map(std::vector<uint32_t>& e, uint32_t w, uint32_t h, uint32_t tag, int luma_dir,
+                std::vector<uint32_t>& out) {
+  out.clear();
+  bool safe = true;
+  std::vector<int32_t> prev_y;
+  std::vector<uint32_t> prev_e;
+  if (luma_dir == 2) {
+    prev_y.assign(w, -1);
+    prev_e.assign(w, 0);
   }
-  uint32_t n_cb = vert ? cu.w : cu.h, n_other = vert ? cu.h : cu.w;
-  uint32_t lead = vert ? cu.x : cu.y;
-  bool filter_edge = lead != 0 && std::find(nf, nf + n_nf, lead) == nf + n_nf;
-  uint32_t grid_step = comp == 0 ? 4u : 8u * (vert ? pp.sub_w : pp.sub_h);
-  uint32_t seg_step = comp == 0 ? 4u : 2u * (vert ? pp.sub_h : pp.sub_w);
-  uint32_t nsb = 1;
-  if (comp == 0 && cu.has_num_sb) nsb = std::max(vert ? cu.num_sb_x : cu.num_sb_y, 1u);
-  uint32_t sb = std::max(8u, n_cb / nsb);
-  uint32_t n8 = n_cb / 8;
-  uint32_t last_sb = std::min(std::max(n8, 1u) - 1, nsb - 1);
-  uint32_t ctb_h_c = pp.ctb / std::max(pp.sub_h, 1u);
-  for (uint32_t off = 0; off < n_cb; off += grid_step) {
-    bool is_tb = off == 0 ? filter_edge : tb_boundary(cu, off, vert, comp != 0);
-    bool is_sb = comp == 0 && off % sb == 0 && off / sb <= last_sb && (off > 0 || filter_edge);
-    if (!(is_tb || is_sb)) continue;
-    for (uint32_t k = 0; k < n_other; k += seg_step) {
-      int32_t qx = vert ? (int32_t)(cu.x + off) : (int32_t)(cu.x + k);
-      int32_t qy = vert ? (int32_t)(cu.y + k) : (int32_t)(cu.y + off);
-      int32_t px = vert ? qx - 1 : qx, py = vert ? qy : qy - 1;
-      int32_t pi = off == 0 ? grid.at(px, py) : (int32_t)idx;
-      if (pi < 0) continue;
-      if (off == 0 && (uint32_t)pi == idx) continue;
-      // map slot (segment index along the edge, edge index along the normal)
-      uint32_t sx, sy;
-      if (comp == 0) {
-        sx = (uint32_t)qx / 4;
-        sy = (uint32_t)qy / 4;
-        if ((uint32_t)qx >= pic_w + 8 || (uint32_t)qy >= pic_h + 8) continue;
-      } else {
-        int32_t ex = qx / (int32_t)pp.sub_w, ey = qy / (int32_t)pp.sub_h;
-        uint32_t cw = (pic_w + pp.sub_w - 1) / pp.sub_w, chh = (pic_h + pp.sub_h - 1) / pp.sub_h;
-        (void)cw;
-        (void)chh;
-        sx = vert ? (uint32_t)ex / 8 : (uint32_t)ex / 2;
-        sy = vert ? (uint32_t)ey / 2 : (uint32_t)ey / 8;
-      }
-      if (sx >= m.w || sy >= m.h) continue;  // entirely outside the picture
-      size_t slot = (size_t)sy * m.w + sx;
-      if (m.seen[slot]) {
-        fail(VVCLF_ERR_INVALID, "deblock: CU records overlap at (%d,%d): they must tile the picture", qx, qy);
-        return false;
-      }
-      m.seen[slot] = 1;
-      const vvclf_deblock_cu& p = cus[pi];
-      int bs = boundary_strength(comp, p, cu, mv, px, py, qx, qy, is_tb, is_sb);
-      if (hist && comp == 0) hist[bs]++;
-      if (bs == 0) continue;
-      uint32_t mflp, mflq;
-      bool ctb_row = false;
-      int32_t qpv;
-      if (comp == 0) {
-        mflp = mflq = 0;
-        if (is_tb) {
-          uint32_t wq = tb_len(cu, qx, qy, vert, false), wp = tb_len(p, px, py, vert, false);
-          if (wp <= 4 || wq <= 4) {
-            mflp = mflq = 1;
-          } else {
-            mflp = wp >= 32 ? 7 : 3;
-            mflq = wq >= 32 ? 7 : 3;
-          }
-        }
-        if (is_sb) {
-          if (off == 0) {
-            if (nsb > 1) mflq = std::min(mflq, 5u);
-            if (p.has_num_sb) mflp = std::min(mflp, 5u);
-          } else {
-            auto tb_at = [&](int64_t z) -> bool {
-              if (z < 0 || (uint64_t)z >= n_cb) return false;
-              if (z == 0) return filter_edge;
-              return tb_boundary(cu, (uint32_t)z, vert, false);
-            };
-            if (is_tb) {
-              mflp = std::min(mflp, 5u);
-              mflq = std::min(mflq, 5u);
-            } else if (tb_at((int64_t)off - 4) || tb_at((int64_t)off + 4)) {
-              mflp = mflq = 1;
-            } else if (off / sb == 1 || off / sb == std::max(n8, 1u) - 1 || tb_at((int64_t)off - sb) ||
-                       tb_at((int64_t)off + sb)) {
-              mflp = mflq = 2;
-            } else {
-              mflp = mflq = 3;
-            }
-          }
-        }
-        ctb_row = !vert && off == 0 && pp.ctb > 0 && cu.y % pp.ctb == 0;
-        qpv = (p.qp_y + cu.qp_y + 1) >> 1;
-      } else {
-        uint32_t sub_axis = vert ? pp.sub_w : pp.sub_h;
-        uint32_t wq = tb_len(cu, qx, qy, vert, true) / std::max(sub_axis, 1u);
-        uint32_t wp = tb_len(p, px, py, vert, true) / std::max(sub_axis, 1u);
-        if (wp >= 8 && wq >= 8) {
-          bool row = !vert && off == 0 && ctb_h_c > 0 && (cu.y / pp.sub_h) % ctb_h_c == 0;
-          mflp = row ? 1 : 3;
-          mflq = 3;
-        } else {
-          mflp = mflq = 1;
-        }
-        if (mflp == 1 && mflq == 1 && bs != 2) continue;  // src/deblock.rs:1823, :1888
-        qpv = (chroma_qp(pp, p) + chroma_qp(pp, cu) + 1) >> 1;
-      }
-      int32_t qb = std::clamp(qpv + pp.beta_off * 2, 0, 63);
-      int32_t qt = std::clamp(qpv + 2 * (bs - 1) + pp.tc_off * 2, 0, 65);
-      if (tc_value(qt, pp.bd) == 0) continue;  // tC == 0: no-op (src/deblock.rs:1429, :1815)
-      if (hist && comp != 0) hist[3]++;
-      m.e[slot] = pack_edge((uint32_t)bs, mflp, mflq, p.plt != 0, cu.plt != 0, ctb_row, (uint32_t)qb, (uint32_t)qt);
-    }
-  }
-  return true;
-}
-
-// Conservative same-direction interaction test for the concurrent pass: for
-// two consecutive edges on one line, the samples one may write must not reach
-// the samples the other uses.  w/r follow the filter extents of
-// src/deblock.rs:1224-1776 for the given maxFilterLength pair.
-inline uint32_t luma_writes(uint32_t m, uint32_t o) {
-  if (m > 3) return m;
-  if (o > 3 || (m > 2 && o > 2)) return 3;
-  return (m > 1 && o > 1) ? 2 : 1;
-}
-inline uint32_t luma_reads(uint32_t m, uint32_t o) {
-  if (m > 3) return m + 1;
-  if (o > 3 || (m > 2 && o > 2)) return 4;
-  return 3;
-}
-
-bool luma_map_concurrent_safe(const edge_map& m, bool vert) {
-  // vert: consecutive edges along x within a map row; else along y within a column.
-  uint32_t lines = vert ? m.h : m.w, n = vert ? m.w : m.h;
-  for (uint32_t l = 0; l < lines; l++) {
-    int32_t prev = -1;
+  for (uint32_t y = 0; y < h; y++) {
+    uint32_t* row = e.data() + (size_t)y * w;
+    int32_t px = -1;
     uint32_t pe = 0;
-    for (uint32_t i = 0; i < n; i++) {
-      uint32_t e = vert ? m.e[(size_t)l * m.w + i] : m.e[(size_t)i * m.w + l];
-      if (!e) continue;
-      if (prev >= 0) {
-        uint32_t d = 4 * (i - (uint32_t)prev);
-        uint32_t a_p = (pe >> E_MFLP_SHIFT) & 7, a_q = (pe >> E_MFLQ_SHIFT) & 7;
-        uint32_t b_p = (e >> E_MFLP_SHIFT) & 7, b_q = (e >> E_MFLQ_SHIFT) & 7;
-        if (!vert && (e & E_CTBROW_BIT)) b_p = std::min(b_p, 3u);
-        if (luma_writes(a_q, a_p) + luma_reads(b_p, b_q) > d || luma_reads(a_q, a_p) + luma_writes(b_p, b_q) > d)
-          return false;
+    for (uint32_t x = 0; x < w; x++) {
+      const uint32_t v = row[x] & ~walk::SLOT_CLAIMED;
+      row[x] = v;
+      if (!v) continue;
+      out.push_back(tag | (y << 16) | x);
+      if (luma_dir == 1) {
+        if (px >= 0 && !walk::luma_pair_safe(pe, v, 4 * (x - (uint32_t)px), true)) safe = false;
+        px = (int32_t)x;
+        pe = v;
+      } else if (luma_dir == 2) {
+        if (prev_y[x] >= 0 && !walk::luma_pair_safe(prev_e[x], v, 4 * (y - (uint32_t)prev_y[x]), false)) safe = false;
+        prev_y[x] = (int32_t)y;
+        prev_e[x] = v;
       }
-      prev = (int32_t)i;
-      pe = e;
     }
   }
-  return true;
+  return safe;
 }
 
 }  // namespace
 
-int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc, uint32_t Hc, host_tables& t) {
+int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc, uint32_t Hc, host_tables& t,
+                        int threads) {
   const vvclf_deblock_params& prm = f.deblock;
   uint32_t ctb = 1u << prm.ctb_log2_size_y;
   cu_grid grid;
@@ -387,30 +127,31 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
     const vvclf_deblock_cu& c = f.cus[i];
     if ((c.x | c.y | c.w | c.h) & 3) return fail(VVCLF_ERR_UNSUPPORTED, "deblock: CU %u not on the 4x4 grid", i);
   }
-  uint64_t hist[4] = {0, 0, 0, 0};
-  edge_map lv, lh;
+  // the six passes of src/deblock.rs:519-604 (luma V/H, Cb V/H, Cr V/H), each
+  // into its own map
+  struct pass {
+    walk::plane_params pp;
+    const vvclf_deblock_cu* cus;
+    uint32_t n;
+    const cu_grid* grid;
+    const vvclf_mv_grid* mv;
+    bool vert;
+    edge_map* m;
+  };
+  std::vector<pass> passes;
+  edge_map lv, lh, cv[2], ch[2];
   t.gw = (W + 3) / 4 + 2;
   t.gh = (H + 3) / 4 + 2;
   lv.init(t.gw, t.gh);
   lh.init(t.gw, t.gh);
-  plane_params pl = {0, 1, 1, prm.luma_beta_offset_div2, prm.luma_tc_offset_div2, 0, prm.bit_depth, ctb};
-  for (uint32_t i = 0; i < f.n_cus; i++)
-    if (!walk_cu(pl, f.cus, grid, f.mv_grid, i, true, f.no_filter_cols, f.n_no_filter_cols, W, H, lv, hist))
-      return VVCLF_ERR_INVALID;
-  for (uint32_t i = 0; i < f.n_cus; i++)
-    if (!walk_cu(pl, f.cus, grid, f.mv_grid, i, false, f.no_filter_rows, f.n_no_filter_rows, W, H, lh, hist))
-      return VVCLF_ERR_INVALID;
-  if (!luma_map_concurrent_safe(lv, true) || !luma_map_concurrent_safe(lh, false))
-    return fail(VVCLF_ERR_UNSUPPORTED,
-                "deblock: same-direction luma edges interact (sub-block edge spacing); the concurrent pass "
-                "would not reproduce the sequential result");
-  t.e_lv = std::move(lv.e);
-  t.e_lh = std::move(lh.e);
+  const walk::plane_params pl = {0, 1, 1, prm.luma_beta_offset_div2, prm.luma_tc_offset_div2, 0, prm.bit_depth, ctb};
+  passes.push_back({pl, f.cus, f.n_cus, &grid, f.mv_grid, true, &lv});
+  passes.push_back({pl, f.cus, f.n_cus, &grid, f.mv_grid, false, &lh});
   t.has_chroma_dbk = false;
+  cu_grid cgrid;
   if (f.chroma_format_idc != 0 && Wc > 0) {
     const vvclf_deblock_cu* ccus = f.has_chroma_cus ? f.chroma_cus : f.cus;
     uint32_t nc = f.has_chroma_cus ? f.n_chroma_cus : f.n_cus;
-    cu_grid cgrid;
     const cu_grid* cg = &grid;
     if (f.has_chroma_cus) {
       cgrid.build(W, H, ccus, nc);
@@ -425,44 +166,97 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
     t.chw = (Wc + 1) / 2 + 1;
     t.chh = (Hc + 7) / 8 + 1;
     for (int comp = 1; comp <= 2; comp++) {
-      plane_params pc = {(uint32_t)comp,
-                         2,
-                         2,
-                         comp == 1 ? prm.cb_beta_offset_div2 : prm.cr_beta_offset_div2,
-                         comp == 1 ? prm.cb_tc_offset_div2 : prm.cr_tc_offset_div2,
-                         comp == 1 ? prm.chroma_qp_offset_cb : prm.chroma_qp_offset_cr,
-                         prm.bit_depth,
-                         ctb};
-      edge_map cv, ch;
-      cv.init(t.cvw, t.cvh);
-      ch.init(t.chw, t.chh);
-      for (uint32_t i = 0; i < nc; i++)
-        if (!walk_cu(pc, ccus, *cg, nullptr, i, true, f.no_filter_cols, f.n_no_filter_cols, W, H, cv, hist))
-          return VVCLF_ERR_INVALID;
-      for (uint32_t i = 0; i < nc; i++)
-        if (!walk_cu(pc, ccus, *cg, nullptr, i, false, f.no_filter_rows, f.n_no_filter_rows, W, H, ch, hist))
-          return VVCLF_ERR_INVALID;
-      t.e_cv[comp - 1] = std::move(cv.e);
-      t.e_ch[comp - 1] = std::move(ch.e);
+      const walk::plane_params pc = {(uint32_t)comp,
+                               2,
+                               2,
+                               comp == 1 ? prm.cb_beta_offset_div2 : prm.cr_beta_offset_div2,
+                               comp == 1 ? prm.cb_tc_offset_div2 : prm.cr_tc_offset_div2,
+                               comp == 1 ? prm.chroma_qp_offset_cb : prm.chroma_qp_offset_cr,
+                               prm.bit_depth,
+                               ctb};
+      cv[comp - 1].init(t.cvw, t.cvh);
+      ch[comp - 1].init(t.chw, t.chh);
+      passes.push_back({pc, ccus, nc, cg, nullptr, true, &cv[comp - 1]});
+      passes.push_back({pc, ccus, nc, cg, nullptr, false, &ch[comp - 1]});
     }
     t.has_chroma_dbk = true;
   }
-  // compact lists of the descriptors that filter something
-  auto list_nonzero = [](const std::vector<uint32_t>& e, uint32_t w, uint32_t tag, std::vector<uint32_t>& out) {
-    for (size_t i = 0; i < e.size(); i++)
-      if (e[i]) out.push_back(tag | ((uint32_t)(i / w) << 16) | (uint32_t)(i % w));
+  // Every thread walks its share of the records through all passes.
+  const int nt = f.n_cus >= 4096 ? std::max(1, std::min(threads, 16)) : 1;
+  std::vector<std::array<uint64_t, 4>> hists((size_t)nt, std::array<uint64_t, 4>{0, 0, 0, 0});
+  std::atomic<bool> bad{false};
+  int32_t bad_xy[2] = {0, 0};
+  std::mutex bad_mu;
+  auto work = [&](int ti) {
+    int32_t xy[2];
+    for (const pass& ps : passes) {
+      const uint32_t b = (uint32_t)((uint64_t)ps.n * ti / nt), e = (uint32_t)((uint64_t)ps.n * (ti + 1) / nt);
+      const uint32_t* nf = ps.vert ? f.no_filter_cols : f.no_filter_rows;
+      const uint32_t n_nf = ps.vert ? f.n_no_filter_cols : f.n_no_filter_rows;
+      for (uint32_t i = b; i < e; i++) {
+        if (!walk::walk_cu(ps.pp, ps.cus, ps.grid->view(), ps.mv, i, ps.vert, nf, n_nf, W, H, ps.m->view(),
+                           hists[ti].data(), xy)) {
+          std::lock_guard<std::mutex> g(bad_mu);
+          if (!bad.exchange(true)) {
+            bad_xy[0] = xy[0];
+            bad_xy[1] = xy[1];
+          }
+          return;
+        }
+        if (bad.load(std::memory_order_relaxed)) return;
+      }
+    }
   };
-  t.l_lv.clear();
-  t.l_lh.clear();
+  std::vector<std::thread> pool;
+  for (int ti = 1; ti < nt; ti++) pool.emplace_back(work, ti);
+  work(0);
+  for (auto& th : pool) th.join();
+  if (bad)
+    return fail(VVCLF_ERR_INVALID, "deblock: CU records overlap at (%d,%d): they must tile the picture", bad_xy[0],
+                bad_xy[1]);
+  uint64_t hist[4] = {0, 0, 0, 0};
+  for (const auto& h : hists)
+    for (int i = 0; i < 4; i++) hist[i] += h[i];
+  // finish the maps (one thread each): claim marks off, work lists, and the
+  // check that lets a direction pass run concurrently in place
+  std::vector<uint32_t> lists[6];
+  bool ok[6] = {true, true, true, true, true, true};
+  {
+    std::vector<std::thread> fin;
+    auto go = [&](int k, edge_map& m, uint32_t tag, int dir) {
+      auto fn = [&, k, tag, dir]() { ok[k] = finish_map(m.e, m.w, m.h, tag, dir, lists[k]); };
+      if (nt > 1) fin.emplace_back(fn);
+      else fn();
+    };
+    go(0, lv, 0, 1);
+    go(1, lh, 0, 2);
+    if (t.has_chroma_dbk)
+      for (int c = 0; c < 2; c++) {
+        go(2 + c, cv[c], (uint32_t)c << 31, 0);
+        go(4 + c, ch[c], (uint32_t)c << 31, 0);
+      }
+    for (auto& th : fin) th.join();
+  }
+  if (!ok[0] || !ok[1])
+    return fail(VVCLF_ERR_UNSUPPORTED,
+                "deblock: same-direction luma edges interact (sub-block edge spacing); the concurrent pass "
+                "would not reproduce the sequential result");
+  t.e_lv = std::move(lv.e);
+  t.e_lh = std::move(lh.e);
+  t.l_lv = std::move(lists[0]);
+  t.l_lh = std::move(lists[1]);
   t.l_cv.clear();
   t.l_ch.clear();
-  list_nonzero(t.e_lv, t.gw, 0, t.l_lv);
-  list_nonzero(t.e_lh, t.gw, 0, t.l_lh);
-  if (t.has_chroma_dbk)
+  if (t.has_chroma_dbk) {
     for (int c = 0; c < 2; c++) {
-      list_nonzero(t.e_cv[c], t.cvw, (uint32_t)c << 31, t.l_cv);
-      list_nonzero(t.e_ch[c], t.chw, (uint32_t)c << 31, t.l_ch);
+      t.e_cv[c] = std::move(cv[c].e);
+      t.e_ch[c] = std::move(ch[c].e);
     }
+    t.l_cv = std::move(lists[2]);
+    t.l_cv.insert(t.l_cv.end(), lists[3].begin(), lists[3].end());
+    t.l_ch = std::move(lists[4]);
+    t.l_ch.insert(t.l_ch.end(), lists[5].begin(), lists[5].end());
+  }
   for (int i = 0; i < 3; i++) t.bs_hist[i] = hist[i];
   t.chroma_filtered = hist[3];
   return VVCLF_OK;

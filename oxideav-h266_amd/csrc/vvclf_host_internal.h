@@ -42,7 +42,9 @@ int hip_fail(hipError_t e, const char* what);
 const char* last_error();
 void clear_error();
 
-int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc, uint32_t Hc, host_tables& t);
+// threads: host threads walking the CU records (pictures of >= 4096 records)
+int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc, uint32_t Hc, host_tables& t,
+                        int threads = 1);
 int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t, uint32_t stages);
 void build_lmcs_lut(const vvclf_lmcs_binding& b, std::vector<uint16_t>& lut);
 void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t);
@@ -69,7 +71,26 @@ struct job_opts {
   hipStream_t stream = nullptr;  // stream-ordered allocation and copies (memory pool)
   const resident_tables* res = nullptr;
   pinned_buf* stage = nullptr;   // upload through this pinned buffer
+  int host_threads = 1;          // threads deriving the deblocking descriptors (host walk)
+  // derive the deblocking descriptors on the GPU (derive_deblock_gpu); no
+  // caller sets it: the path is switched off until a device memory fault it
+  // raised is explained
+  bool gpu_derive = false;
 };
+
+// Device destinations of the GPU deblocking derivation, inside a job's side
+// allocation: maps zero-filled, lists with room for every slot of their maps.
+struct gpu_maps {
+  uint32_t *e_lv, *e_lh, *e_cv[2], *e_ch[2];
+  uint32_t gw, gh, cvw, cvh, chw, chh;
+  uint32_t *l_lv, *l_lh, *l_cv, *l_ch;
+  bool chroma;
+};
+// The descriptors and work lists of derive_deblock_maps, computed on the GPU
+// on stream s (the call waits for it); n_out receives the list lengths
+// (luma V, luma H, chroma V, chroma H).
+int derive_deblock_gpu(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc, uint32_t Hc, const gpu_maps& o,
+                       hipStream_t s, uint32_t n_out[4]);
 
 // Host planes <-> a job's contiguous sample set (luma, Cb, Cr at the device
 // pitches) through a pinned buffer: CPU row copies plus one linear DMA.
@@ -88,6 +109,7 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
 hipError_t launch_fused(int tw, int th, bool hbd, const dev_job* jobs, const tile_ref* tiles, uint32_t n_tiles,
                         uint32_t grid, hipStream_t s);
 int fused_max_blocks_per_cu(int tw, int th, bool hbd);
+int fused_threads(int tw, int th);  // threads per workgroup of a tile shape
 inline bool job_needs_hbd(const dev_job& d) { return d.do_alf && d.alf_bd > 12; }
 // A fused tile lies inside one ALF CTB (one filter bank per tile) and spans at
 // most 3x3 SAO CTBs with its SAO reach (the kernel stages those parameters).

@@ -3,6 +3,7 @@
 // (`dev_job`) passed by value; all pointers are device pointers into HBM.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace vvclf {
@@ -32,8 +33,8 @@ enum : uint32_t {
   E_QT_SHIFT = 17,
 };
 
-static inline uint32_t pack_edge(uint32_t bs, uint32_t mflp, uint32_t mflq, bool pltp, bool pltq, bool ctbrow,
-                                 uint32_t qb, uint32_t qt) {
+__host__ __device__ inline uint32_t pack_edge(uint32_t bs, uint32_t mflp, uint32_t mflq, bool pltp, bool pltq,
+                                              bool ctbrow, uint32_t qb, uint32_t qt) {
   return (bs << E_BS_SHIFT) | (mflp << E_MFLP_SHIFT) | (mflq << E_MFLQ_SHIFT) | (pltp ? E_PLTP_BIT : 0u) |
          (pltq ? E_PLTQ_BIT : 0u) | (ctbrow ? E_CTBROW_BIT : 0u) | (qb << E_QB_SHIFT) | (qt << E_QT_SHIFT);
 }

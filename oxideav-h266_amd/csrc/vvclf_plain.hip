@@ -12,6 +12,7 @@
 // all segments of a pass run concurrently with the sequential result.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "vvclf_filters.h"
@@ -71,8 +72,8 @@ __global__ void __launch_bounds__(256) k_prep(dev_job j) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const uint32_t a = w[i] & 0xffffu, b = w[i] >> 16;
-        const uint32_t lo = (lut_lds && a < 1024) ? lut[a] : j.lmcs_lut[a];
-        const uint32_t hi = (lut_lds && b < 1024) ? lut[b] : j.lmcs_lut[b];
+        const uint32_t lo = (lut_lds && a < 1024) ? lut[a] : gld(j.lmcs_lut + a);
+        const uint32_t hi = (lut_lds && b < 1024) ? lut[b] : gld(j.lmcs_lut + b);
         w[i] = lo | (hi << 16);
       }
       v[r] = make_uint4(w[0], w[1], w[2], w[3]);
