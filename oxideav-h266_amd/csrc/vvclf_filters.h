@@ -366,25 +366,32 @@ __device__ __forceinline__ int sao_band_sample(int p, uint64_t prm, int bd, int 
 }
 
 // ---------------------------------------------------------------------------
-// SAO over runs of 8 samples sharing one CTB (src/sao.rs:504-626).  The four
-// offsets of a CTB component are folded into a table of i8 in a 64-bit word:
-// edge offset T[e] for the raw index e = 2 + sign(p-n0) + sign(p-n1)
-// (remapped 0,1,2,3,4 -> offsets 1,2,-,3,4, index 2 adds nothing), band
-// offset T[k] for k = band - band_position in 0..3.  Valid when every offset
-// fits in i8 (10-bit: |offset| <= 31; 12-bit: <= 124); otherwise the
-// per-sample path above is used.
+// SAO over runs of 8 samples sharing one CTB (src/sao.rs:504-626), two
+// samples per 32-bit word (low half first).  The four offsets of a CTB
+// component are folded into a table of 8 bytes, each offset T biased to
+// T + 128 and unused entries 128 (offset 0): edge offset byte e for the raw
+// index e = 2 + sign(p-n0) + sign(p-n1) (remapped 0,1,2,3,4 -> offsets
+// 1,2,-,3,4, index 2 adds nothing), band offset byte k for k = band -
+// band_position in 0..3 (k >= 4: byte 4).  One v_perm_b32 looks the byte up
+// for both samples of a word.  Valid when every offset fits in i8 (10-bit:
+// |offset| <= 31; 12-bit: <= 124) and the SAO bit depth is at most 15;
+// otherwise the per-sample path above is used.
+//
+// The packed steps saturate at 0 and 65535 where the integer ones do not:
+// sign(p - n) + 1 is min(sat(sat(p + 1) - n), 2) and the output
+// min(sat(sat(p + T') - 128), maxv).  They differ from the integer result
+// only for p = 65535 or p + T' > 65535, i.e. p > 65280, whose output is maxv
+// either way when maxv <= 32767.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int med3i(int a, int b, int c) {
-  int r;
-  __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-  return r;
-}
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 struct sao_run {
-  int type;      // 0 off, 1 band, 2 edge
-  int aux;       // edge class, or band position
-  uint64_t tab;  // i8 offsets (see above)
-  bool fits;     // all offsets fit in i8
+  int type;       // 0 off, 1 band, 2 edge
+  int aux;        // edge class, or band position
+  uint64_t btab;  // biased offset bytes (see above)
+  bool fits;      // all offsets fit in i8
 };
 
 __device__ __forceinline__ sao_run sao_run_of(uint64_t prm) {
@@ -393,46 +400,109 @@ __device__ __forceinline__ sao_run sao_run_of(uint64_t prm) {
   r.aux = (int)((prm >> 2) & 31);
   const int o1 = sao_offset(1, prm), o2 = sao_offset(2, prm), o3 = sao_offset(3, prm), o4 = sao_offset(4, prm);
   r.fits = max(max(abs(o1), abs(o2)), max(abs(o3), abs(o4))) <= 127;
-  auto b = [](int v, int k) -> uint64_t { return (uint64_t)(uint8_t)(int8_t)v << (8 * k); };
-  r.tab = r.type == 2 ? (b(o1, 0) | b(o2, 1) | b(o3, 3) | b(o4, 4)) : (b(o1, 0) | b(o2, 1) | b(o3, 2) | b(o4, 3));
+  auto b = [](int v, int k) -> uint64_t { return (uint64_t)(uint8_t)(v + 128) << (8 * k); };
+  const uint64_t unused = 0x8080808080808080ull;
+  r.btab = r.type == 2 ? ((unused & ~0xffff00ffffull) | b(o1, 0) | b(o2, 1) | b(o3, 3) | b(o4, 4))
+                       : ((unused & ~0xffffffffull) | b(o1, 0) | b(o2, 1) | b(o3, 2) | b(o4, 3));
   return r;
 }
 
-// Edge offset of samples x8..x8+7 of row y: A, P, B hold rows y-1, y, y+1 at
-// x8-1 .. x8+8 (index i+1 is sample x8+i).  Neighbours by class (Table 44).
+// Eight samples x8..x8+7 of a row, packed, with the samples x8-1 (l) and
+// x8+8 (r) in the low 16 bits of their words.
+struct sao_row8 {
+  uint32_t w[4];
+  uint32_t l, r;
+};
+
+// Samples 2k-1, 2k and 2k+1, 2k+2 of a run as packed words.
+__device__ __forceinline__ uint32_t pk_prev(const sao_row8& R, int k) {
+  return k == 0 ? ((R.w[0] << 16) | R.l) : __builtin_amdgcn_alignbit(R.w[k], R.w[k - 1], 16);
+}
+__device__ __forceinline__ uint32_t pk_next(const sao_row8& R, int k) {
+  return __builtin_amdgcn_alignbit(k == 3 ? R.r : R.w[k + 1], R.w[k], 16);
+}
+
+// Both samples of p plus their biased table bytes idx (per half, <= 7),
+// clipped to [0, maxv]; maxv2 holds maxv in both halves.
+__device__ __forceinline__ uint32_t sao_pk_add(uint32_t p, uint32_t idx, uint64_t btab, uint32_t maxv2) {
+  const uint32_t t = __builtin_amdgcn_perm((uint32_t)(btab >> 32), (uint32_t)btab, idx | 0x0c000c00u);
+  const u16x2 x = __builtin_elementwise_sub_sat(__builtin_elementwise_add_sat(as_u16x2(p), as_u16x2(t)), (u16x2)(128));
+  return as_u32(__builtin_elementwise_min(x, as_u16x2(maxv2)));
+}
+
+// Edge offset of a run: A, P, B are rows y-1, y, y+1.  Neighbours by class
+// (Table 44).
 template <int CLS>
-__device__ __forceinline__ void sao_eo8(const int (&A)[10], const int (&P)[10], const int (&B)[10], uint64_t tab,
-                                        int maxv, int (&out)[8]) {
+__device__ __forceinline__ void sao_eo8(const sao_row8& A, const sao_row8& P, const sao_row8& B, uint64_t btab,
+                                        uint32_t maxv2, uint32_t (&out)[4]) {
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const int p = P[i + 1];
-    const int n0 = CLS == 0 ? P[i] : (CLS == 1 ? A[i + 1] : (CLS == 2 ? A[i] : A[i + 2]));
-    const int n1 = CLS == 0 ? P[i + 2] : (CLS == 1 ? B[i + 1] : (CLS == 2 ? B[i + 2] : B[i]));
-    const int e = 2 + med3i(p - n0, -1, 1) + med3i(p - n1, -1, 1);
-    const int off = (int)(int8_t)(uint8_t)(tab >> (8 * e));
-    out[i] = min(max(p + off, 0), maxv);
+  for (int k = 0; k < 4; k++) {
+    const uint32_t p = P.w[k];
+    const uint32_t n0 = CLS == 0 ? pk_prev(P, k) : (CLS == 1 ? A.w[k] : (CLS == 2 ? pk_prev(A, k) : pk_next(A, k)));
+    const uint32_t n1 = CLS == 0 ? pk_next(P, k) : (CLS == 1 ? B.w[k] : (CLS == 2 ? pk_next(B, k) : pk_prev(B, k)));
+    const u16x2 pp1 = __builtin_elementwise_add_sat(as_u16x2(p), (u16x2)(1));
+    const u16x2 c0 = __builtin_elementwise_min(__builtin_elementwise_sub_sat(pp1, as_u16x2(n0)), (u16x2)(2));
+    const u16x2 c1 = __builtin_elementwise_min(__builtin_elementwise_sub_sat(pp1, as_u16x2(n1)), (u16x2)(2));
+    out[k] = sao_pk_add(p, as_u32(c0) + as_u32(c1), btab, maxv2);
   }
 }
 
-__device__ __forceinline__ void sao_eo8_any(int cls, const int (&A)[10], const int (&P)[10], const int (&B)[10],
-                                            uint64_t tab, int maxv, int (&out)[8]) {
+__device__ __forceinline__ void sao_eo8_any(int cls, const sao_row8& A, const sao_row8& P, const sao_row8& B,
+                                            uint64_t btab, uint32_t maxv2, uint32_t (&out)[4]) {
   switch (cls & 3) {
-    case 0: sao_eo8<0>(A, P, B, tab, maxv, out); break;
-    case 1: sao_eo8<1>(A, P, B, tab, maxv, out); break;
-    case 2: sao_eo8<2>(A, P, B, tab, maxv, out); break;
-    default: sao_eo8<3>(A, P, B, tab, maxv, out); break;
+    case 0: sao_eo8<0>(A, P, B, btab, maxv2, out); break;
+    case 1: sao_eo8<1>(A, P, B, btab, maxv2, out); break;
+    case 2: sao_eo8<2>(A, P, B, btab, maxv2, out); break;
+    default: sao_eo8<3>(A, P, B, btab, maxv2, out); break;
   }
 }
 
-// Band offset of 8 samples P[1..8].
-__device__ __forceinline__ void sao_bo8(const int (&P)[10], int pos, uint64_t tab, int bd, int maxv, int (&out)[8]) {
-  const int sh = bd >= 5 ? bd - 5 : 0;
+// Band offset of a run.
+__device__ __forceinline__ void sao_bo8(const sao_row8& P, int pos, uint64_t btab, int bd, uint32_t maxv2,
+                                        uint32_t (&out)[4]) {
+  const unsigned short sh = (unsigned short)(bd >= 5 ? bd - 5 : 0);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const u16x2 band = as_u16x2(P.w[k]) >> (u16x2)(sh);
+    const u16x2 kk = __builtin_elementwise_min((band - (u16x2)((unsigned short)pos)) & (u16x2)(31), (u16x2)(4));
+    out[k] = sao_pk_add(P.w[k], as_u32(kk), btab, maxv2);
+  }
+}
+
+// Sample i (-1..8) of a run as an integer.
+__device__ __forceinline__ int sao_at(const sao_row8& R, int i) {
+  return i < 0 ? (int)R.l : (i > 7 ? (int)R.r : (int)((R.w[i >> 1] >> (16 * (i & 1))) & 0xffffu));
+}
+
+// Sample i of o replaced by v.
+__device__ __forceinline__ void sao_put(uint32_t (&o)[4], int i, int v) {
+  o[i >> 1] = (i & 1) ? ((o[i >> 1] & 0xffffu) | ((uint32_t)v << 16)) : ((o[i >> 1] & 0xffff0000u) | (uint32_t)v);
+}
+
+// Edge offset of a run sample by sample, for any class (dx0, dy0, dx1, dy1:
+// Table 44 neighbour offsets) and any offsets (prm), each neighbour checked
+// against [cx0, cx1) x [cy0, cy1): a sample with a neighbour outside is only
+// clipped.  x8, y: position of the run's first sample.
+__device__ __forceinline__ void sao_eo8_checked(int dx0, int dy0, int dx1, int dy1, const sao_row8& A,
+                                                const sao_row8& P, const sao_row8& B, int x8, int y, int cx0,
+                                                int cy0, int cx1, int cy1, uint64_t prm, int maxv, uint32_t (&o)[4]) {
+  uint32_t n0w[4], n1w[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const sao_row8& R0 = dy0 == 0 ? P : A;
+    const sao_row8& R1 = dy1 == 0 ? P : B;
+    n0w[k] = dx0 < 0 ? pk_prev(R0, k) : (dx0 > 0 ? pk_next(R0, k) : R0.w[k]);
+    n1w[k] = dx1 < 0 ? pk_prev(R1, k) : (dx1 > 0 ? pk_next(R1, k) : R1.w[k]);
+  }
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    const int p = P[i + 1];
-    const int k = ((p >> sh) - pos) & 31;
-    const int off = k < 4 ? (int)(int8_t)(uint8_t)(tab >> (8 * k)) : 0;
-    out[i] = min(max(p + off, 0), maxv);
+    const int sh = 16 * (i & 1);
+    const int pv = (int)((P.w[i >> 1] >> sh) & 0xffffu), n0 = (int)((n0w[i >> 1] >> sh) & 0xffffu),
+              n1 = (int)((n1w[i >> 1] >> sh) & 0xffffu);
+    const int x = x8 + i, nx0 = x + dx0, ny0 = y + dy0, nx1 = x + dx1, ny1 = y + dy1;
+    const bool outside = nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
+                         nx1 >= cx1 || ny1 >= cy1;
+    sao_put(o, i, outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, n0, n1, prm, maxv));
   }
 }
 

@@ -292,142 +292,162 @@ struct sao_tab {
 };
 
 // SAO of one component from window `src` into window `dst` (dst's extent is
-// the region produced, at most RW samples wide).  sub = 1 for chroma.  Work is
-// split into runs of 8 samples at x = 8k (one CTB per run, SAO CTBs being at
-// least 8 samples wide), walked flat so that the lanes stay busy.  A run reads
-// its CTB's decoded parameters (runs[cell][comp]: type | class or band << 8 |
-// fits << 16, then the i8 offset table) and all three sample rows up front,
-// so that the LDS reads overlap.  Runs with every offset in i8 use the table
-// form (sao_eo8 / sao_bo8); edge-offset samples whose 8-neighbourhood is not
-// inside the picture and the CTB's rectangle take the per-sample check.
-// SRC16: src rows at x8 are 16-byte aligned (else 8); DST8: dst rows at x8
-// are 8-byte aligned (else 4).  NPL = 2 walks both chroma planes in one loop
-// (component comp0 + plane; plane 1 at src.p + sd1 / dst.p + dd1, same
+// the region produced, at most RW x RH samples).  sub = 1 for chroma.  Work is
+// split into strips of 8 samples at x = 8k by R rows (one CTB column per
+// strip, SAO CTBs being at least 8 samples wide; at most two CTB rows, their
+// height being at least 8 > R), one strip per thread (R is chosen so that
+// one pass covers the region).  A strip reads the
+// decoded parameters of its CTBs (runs[cell][comp]: type | class or band << 8
+// | fits << 16, then the biased offset bytes) and its R + 2 sample rows up
+// front, so that the LDS reads overlap and each row is read once.  Rows whose
+// CTB has every offset in i8 use the packed table form (sao_eo8 / sao_bo8);
+// the other rows, and edge-offset rows whose 8-neighbourhoods are not all
+// inside the picture and the CTB's rectangle, are done afterwards sample by
+// sample.  SRC16: src rows at x8 are 16-byte aligned (else 8); DST8: dst rows
+// at x8 are 8-byte aligned (else 4).  NPL = 2 walks both chroma planes in one
+// loop (component comp0 + plane; plane 1 at src.p + sd1 / dst.p + dd1, same
 // geometry).
-template <int RW, bool SRC16, bool DST8, int NPL, int NT>
+// The least strip height (at most 7) at which the strips of a RW x RH
+// region, NPL planes, take one pass of NT threads.
+constexpr int sao_strip_rows(int RW, int RH, int NPL, int NT) {
+  int r = 1;
+  while (r < 7 && NPL * ((RW + 14) / 8) * ((RH + r - 1) / r) > NT) r++;
+  return r;
+}
+
+template <int RW, int RH, bool SRC16, bool DST8, int NPL, int NT, int R = sao_strip_rows(RW, RH, NPL, NT)>
 __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)[3], const uint4 (*runs)[3],
                                         const uint64_t* saor, const uint64_t (*safe)[3], const sao_tab& tb,
                                         const lwin& src0, const lwin& dst0, int comp0, int sd1, int dd1, int pw,
                                         int ph) {
+  static_assert(R + 1 <= 8, "a strip spans at most two CTB rows");
   const int sub = comp0 ? 1 : 0;
   const int log2c = j.sao_ctb_log2 - sub;
   const int maxv = (1 << j.sao_bd) - 1;
+  const uint32_t maxv2 = (uint32_t)maxv * 0x10001u;
   const bool has_rect = j.sao_rect != nullptr;
   const int ka = dst0.ox >> 3, nk = ((dst0.ox + dst0.w - 1) >> 3) - ka + 1;
-  constexpr int NK = (RW + 14) / 8;  // runs per row, at most
-  const int per_plane = NK * dst0.h;
+  constexpr int NK = (RW + 14) / 8;      // strip columns, at most
+  constexpr int NS = (RH + R - 1) / R;  // strips per column, at most
+  constexpr int per_plane = NK * NS;
   for (int i0 = ltid(); i0 < NPL * per_plane; i0 += NT) {
     const int pl = (NPL > 1 && i0 >= per_plane) ? 1 : 0, i = i0 - pl * per_plane;
     const int comp = comp0 + pl;
     lwin src = src0, dst = dst0;
     src.p += pl * sd1;
     dst.p += pl * dd1;
-    const int yy = i / NK, kk = i - yy * NK;
-    if (kk >= nk) continue;
-    const int y = dst.oy + yy, x8 = (ka + kk) * 8;
+    // column-major: a wave's strips share few CTB columns, and so mostly one
+    // SAO type and class
+    const int kk = i / NS, st = i - kk * NS;
+    const int ys = dst.oy + st * R;
+    if (kk >= nk || ys >= dst.oy + dst.h) continue;
+    const int nr = min(R, dst.oy + dst.h - ys);
+    const int x8 = (ka + kk) * 8;
     const int xa = max(x8, dst.ox), xb = min(x8 + 8, dst.ox + dst.w);  // samples produced
-    const int cell = ((y >> log2c) - tb.by) * 3 + ((x8 >> log2c) - tb.bx);
-    const uint4 rp = runs[cell][comp];
-    const uint64_t sr = safe[cell][comp];
-    // rows y-1, y, y+1 at x8-1 .. x8+8 (direct reads when inside the window,
-    // else clamped)
-    int A[10], P[10], B[10];
-    const bool direct = x8 - 1 >= src.ox && x8 + 9 <= src.ox + src.w && y - 1 >= src.oy && y + 2 <= src.oy + src.h;
+    // the CTBs of the strip's first and last rows
+    const int cxr = (x8 >> log2c) - tb.bx;
+    const int cell0 = ((ys >> log2c) - tb.by) * 3 + cxr, cell1 = (((ys + nr - 1) >> log2c) - tb.by) * 3 + cxr;
+    const int ysplit = ((ys >> log2c) + 1) << log2c;  // first row in cell1
+    const uint4 rp0 = runs[cell0][comp], rp1 = runs[cell1][comp];
+    const uint64_t sr0 = safe[cell0][comp], sr1 = safe[cell1][comp];
+    // rows ys-1 .. ys+R at x8-1 .. x8+8 (direct reads when inside the
+    // window, else clamped)
+    sao_row8 Rw[R + 2];
+    const bool direct = x8 - 1 >= src.ox && x8 + 9 <= src.ox + src.w && ys - 1 >= src.oy && ys + R + 1 <= src.oy + src.h;
     if (direct) {
-      auto row = [&](int yr, int (&R)[10]) {
-        const uint16_t* q = src.p + (yr - src.oy) * src.pitch + (x8 - src.ox);
-        uint32_t w[4];
+      const uint16_t* q = src.p + (ys - 1 - src.oy) * src.pitch + (x8 - src.ox);
+#pragma unroll
+      for (int r = 0; r < R + 2; r++, q += src.pitch) {
         if (SRC16) {
           const uint4 v = *reinterpret_cast<const uint4*>(q);
-          w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+          Rw[r].w[0] = v.x; Rw[r].w[1] = v.y; Rw[r].w[2] = v.z; Rw[r].w[3] = v.w;
         } else {
           const uint2 a = *reinterpret_cast<const uint2*>(q), b = *reinterpret_cast<const uint2*>(q + 4);
-          w[0] = a.x; w[1] = a.y; w[2] = b.x; w[3] = b.y;
+          Rw[r].w[0] = a.x; Rw[r].w[1] = a.y; Rw[r].w[2] = b.x; Rw[r].w[3] = b.y;
         }
-#pragma unroll
-        for (int k = 0; k < 8; k++) R[k + 1] = (int)((w[k >> 1] >> (16 * (k & 1))) & 0xffffu);
-        R[0] = q[-1];
-        R[9] = q[8];
-      };
-      row(y - 1, A);
-      row(y, P);
-      row(y + 1, B);
+        Rw[r].l = q[-1];
+        Rw[r].r = q[8];
+      }
     } else {
 #pragma unroll
-      for (int k = 0; k < 10; k++) {
-        A[k] = src.rd(x8 - 1 + k, y - 1);
-        P[k] = src.rd(x8 - 1 + k, y);
-        B[k] = src.rd(x8 - 1 + k, y + 1);
+      for (int r = 0; r < R + 2; r++) {
+        const int yr = ys - 1 + r;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          Rw[r].w[k] = (uint32_t)src.rd(x8 + 2 * k, yr) | ((uint32_t)src.rd(x8 + 2 * k + 1, yr) << 16);
+        Rw[r].l = src.rd(x8 - 1, yr);
+        Rw[r].r = src.rd(x8 + 8, yr);
       }
     }
-    const int type = (int)(rp.x & 3), aux = (int)((rp.x >> 8) & 31);
-    const bool fits = (rp.x >> 16) & 1;
-    const uint64_t tab = (uint64_t)rp.y | ((uint64_t)rp.z << 32);
-    int o[8];
+    uint32_t later = 0;  // rows left to the per-sample path
 #pragma unroll
-    for (int k = 0; k < 8; k++) o[k] = P[k + 1];
-    if (type == 1) {
-      if (fits) {
-        sao_bo8(P, aux, tab, j.sao_bd, maxv, o);
-      } else {
-        const uint64_t prm = saop[cell][comp];
-#pragma unroll
-        for (int k = 0; k < 8; k++) o[k] = sao_band_sample(P[k + 1], prm, j.sao_bd, maxv);
-      }
-    } else if (type == 2) {
-      const int cls = aux & 3;
-      const bool inner = xa >= (int)(sr & 0xffff) && xb <= (int)((sr >> 16) & 0xffff) &&
-                         y >= (int)((sr >> 32) & 0xffff) && y < (int)(sr >> 48);
-      if (inner && fits) {
-        sao_eo8_any(cls, A, P, B, tab, maxv, o);
-      } else {
-        const uint64_t prm = saop[cell][comp];
-        int h0, v0, h1, v1;
-        eo_offsets(cls, h0, v0, h1, v1);
-        int cx0 = 0, cy0 = 0, cx1 = pw, cy1 = ph;
-        if (has_rect) {
-          const uint64_t r = saor[cell];
-          cx0 = (int)(r & 0xffff); cy0 = (int)((r >> 16) & 0xffff); cx1 = (int)((r >> 32) & 0xffff); cy1 = (int)(r >> 48);
-          if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
-          cx0 = max(cx0, 0); cy0 = max(cy0, 0); cx1 = min(cx1, pw); cy1 = min(cy1, ph);
+    for (int r = 0; r < R; r++) {
+      if (r < nr) {
+        const int y = ys + r;
+        const bool second = y >= ysplit;
+        const uint4 rp = second ? rp1 : rp0;
+        const uint64_t sr = second ? sr1 : sr0;
+        const int type = (int)(rp.x & 3), aux = (int)((rp.x >> 8) & 31);
+        const bool packed = ((rp.x >> 16) & 1) && j.sao_bd <= 15;
+        const uint64_t btab = (uint64_t)rp.y | ((uint64_t)rp.z << 32);
+        const sao_row8& P = Rw[r + 1];
+        uint32_t o[4] = {P.w[0], P.w[1], P.w[2], P.w[3]};
+        bool done = true;
+        if (type == 1) {
+          if (packed) sao_bo8(P, aux, btab, j.sao_bd, maxv2, o);
+          else done = false;
+        } else if (type == 2) {
+          const bool inner = xa >= (int)(sr & 0xffff) && xb <= (int)((sr >> 16) & 0xffff) &&
+                             y >= (int)((sr >> 32) & 0xffff) && y < (int)(sr >> 48);
+          if (inner && packed) sao_eo8_any(aux & 3, Rw[r], P, Rw[r + 2], btab, maxv2, o);
+          else done = false;
         }
-        auto slow = [&](auto cls_tag) {
-          constexpr int CLS = decltype(cls_tag)::value;
+        if (!done) {
+          later |= 1u << r;
+        } else if (xa == x8 && xb == x8 + 8) {
+          uint16_t* d = dst.p + (y - dst.oy) * dst.pitch + (x8 - dst.ox);
+          if (DST8) {
+            reinterpret_cast<uint2*>(d)[0] = make_uint2(o[0], o[1]);
+            reinterpret_cast<uint2*>(d)[1] = make_uint2(o[2], o[3]);
+          } else {
 #pragma unroll
-          for (int k = 0; k < 8; k++) {
-            const int pv = P[k + 1];
-            const int n0 = CLS == 0 ? P[k] : (CLS == 1 ? A[k + 1] : (CLS == 2 ? A[k] : A[k + 2]));
-            const int n1 = CLS == 0 ? P[k + 2] : (CLS == 1 ? B[k + 1] : (CLS == 2 ? B[k + 2] : B[k]));
-            const int x = x8 + k, nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
-            const bool outside = nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
-                                 nx1 >= cx1 || ny1 >= cy1;
-            o[k] = outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, n0, n1, prm, maxv);
+            for (int k = 0; k < 4; k++) reinterpret_cast<uint32_t*>(d)[k] = o[k];
           }
-        };
-        switch (cls) {
-          case 0: slow(std::integral_constant<int, 0>{}); break;
-          case 1: slow(std::integral_constant<int, 1>{}); break;
-          case 2: slow(std::integral_constant<int, 2>{}); break;
-          default: slow(std::integral_constant<int, 3>{}); break;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            if (x8 + k >= xa && x8 + k < xb) dst.wr(x8 + k, y, (o[k >> 1] >> (16 * (k & 1))) & 0xffffu);
         }
       }
     }
-    if (xa == x8 && xb == x8 + 8) {
-      uint16_t* d = dst.p + (y - dst.oy) * dst.pitch + (x8 - dst.ox);
-      uint32_t w[4];
-#pragma unroll
-      for (int k = 0; k < 4; k++) w[k] = (uint32_t)o[2 * k] | ((uint32_t)o[2 * k + 1] << 16);
-      if (DST8) {
-        reinterpret_cast<uint2*>(d)[0] = make_uint2(w[0], w[1]);
-        reinterpret_cast<uint2*>(d)[1] = make_uint2(w[2], w[3]);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) reinterpret_cast<uint32_t*>(d)[k] = w[k];
+    // per sample, with the picture / rectangle check of each neighbour
+    while (later) {
+      const int r = __builtin_ctz(later);
+      later &= later - 1;
+      const int y = ys + r;
+      const int cell = y >= ysplit ? cell1 : cell0;
+      const uint64_t prm = saop[cell][comp];
+      const int type = (int)(prm & 3);
+      if (type == 1) {
+        for (int x = xa; x < xb; x++) dst.wr(x, y, sao_band_sample(src.rd(x, y), prm, j.sao_bd, maxv));
+        continue;
       }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (x8 + k >= xa && x8 + k < xb) dst.wr(x8 + k, y, o[k]);
+      int h0, v0, h1, v1;
+      eo_offsets((int)((prm >> 2) & 3), h0, v0, h1, v1);
+      int cx0 = 0, cy0 = 0, cx1 = pw, cy1 = ph;
+      if (has_rect) {
+        const uint64_t rr = saor[cell];
+        cx0 = (int)(rr & 0xffff); cy0 = (int)((rr >> 16) & 0xffff); cx1 = (int)((rr >> 32) & 0xffff); cy1 = (int)(rr >> 48);
+        if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
+        cx0 = max(cx0, 0); cy0 = max(cy0, 0); cx1 = min(cx1, pw); cy1 = min(cy1, ph);
+      }
+      for (int x = xa; x < xb; x++) {
+        const int pv = src.rd(x, y);
+        const int nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
+        const bool outside = nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
+                             nx1 >= cx1 || ny1 >= cy1;
+        dst.wr(x, y, outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, src.rd(nx0, ny0), src.rd(nx1, ny1), prm, maxv));
+      }
     }
   }
 }
@@ -678,7 +698,7 @@ struct tile_loads {
         s.saop[scell >> 2][sk] = sv;
         const sao_run rn = sao_run_of(sv);
         s.saorun[scell >> 2][sk] = make_uint4((uint32_t)rn.type | ((uint32_t)rn.aux << 8) | ((uint32_t)rn.fits << 16),
-                                              (uint32_t)rn.tab, (uint32_t)(rn.tab >> 32), 0u);
+                                              (uint32_t)rn.btab, (uint32_t)(rn.btab >> 32), 0u);
       } else {
         s.saor[scell >> 2] = sv;
         // rectangles inside which a sample's whole 8-neighbourhood lies in
@@ -749,7 +769,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   S.w = min(W, x1 + 3) - S.ox;
   S.h = min(H, y1 + 3) - S.oy;
   S.p = s.o.S + (S.oy - soy) * LD::SW + (S.ox - sox);
-  if (j.do_sao) lds_sao<TW + 6, true, true, 1, NT>(j, s.saop, s.saorun, s.saor, s.safe, tb, L, S, 0, 0, 0, W, H);
+  if (j.do_sao) lds_sao<TW + 6, TH + 6, true, true, 1, NT>(j, s.saop, s.saorun, s.saor, s.safe, tb, L, S, 0, 0, 0, W, H);
   else lds_copy(L, S);
   if (chroma) {
     lwin Sc{nullptr, LD::SCW, max(0, cx0 - 2), max(0, cy0 - 2), 0, 0};
@@ -760,7 +780,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     Cs.p = s.Cc[0];
     constexpr int SD1 = LD::CH * LD::CW, DD1 = LD::SCH * LD::SCW;  // Cr after Cb
     if (j.do_sao) {
-      lds_sao<TW / 2 + 4, false, false, 2, NT>(j, s.saop, s.saorun, s.saor, s.safe, tb, Cs, Sc, 1, SD1, DD1, j.wc, j.hc);
+      lds_sao<TW / 2 + 4, TH / 2 + 4, false, false, 2, NT>(j, s.saop, s.saorun, s.saor, s.safe, tb, Cs, Sc, 1, SD1, DD1, j.wc, j.hc);
     } else {
       for (int c = 0; c < 2; c++) {
         lwin a = Cs, b = Sc;

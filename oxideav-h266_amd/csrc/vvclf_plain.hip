@@ -195,32 +195,43 @@ __global__ void __launch_bounds__(256) k_dbk(dev_job j, int luma_blocks) {
 }
 
 // ---------------------------------------------------------------------------
-// SAO: A -> B, 4 samples per thread.  blockIdx.z = component.
+// SAO: A -> B, a strip of 8 samples by SAO_ROWS rows per thread, inside one
+// CTB (SAO CTBs are at least 16 luma / 8 chroma samples wide and high; strips
+// are aligned to 8 samples and SAO_ROWS rows).  blockIdx.z = component.  The
+// strip's SAO_ROWS + 2 rows are read once, all loads issued together, and
+// kept in registers; rows whose CTB has every offset in i8 use the packed
+// table form, the other rows (and edge-offset rows whose neighbourhoods leave
+// the picture or the CTB's rectangle) go sample by sample.
 // ---------------------------------------------------------------------------
+constexpr int SAO_ROWS = 4;
+
 __global__ void __launch_bounds__(256) k_sao(dev_job j) {
   const int comp = blockIdx.z;
   plane_view src = pick(j.a, j, comp), dst = pick(j.b, j, comp);
-  const int y = blockIdx.y;
+  const int ys = blockIdx.y * SAO_ROWS;
   const int x0 = (blockIdx.x * 256 + threadIdx.x) * 8;
-  if (y >= src.h || x0 >= src.w) return;  // lanes leave in x order: a lane's right neighbour lane is
-                                          // present whenever x0 + 8 is inside the picture
+  if (ys >= src.h || x0 >= src.w) return;  // lanes leave in x order: a lane's right neighbour lane is
+                                           // present whenever x0 + 8 is inside the picture
+  const int nr = min(SAO_ROWS, src.h - ys);
   const int sub = comp ? 1 : 0;
   const int log2c = j.sao_ctb_log2 - sub;
   const bool active = comp == 0 ? j.sao_luma : j.sao_chroma;
   const int maxv = (1 << j.sao_bd) - 1;
-  uint16_t* drow = dst.p + (size_t)y * dst.pitch;
-  // Rows y-1, y, y+1 (clamped) as 8-sample reads, all issued with the
-  // parameter read; rows are padded to the pitch (a multiple of 64), so the
-  // reads stay inside the plane past its width.  Edge-offset classes 1..3
-  // use the outer rows; adjacent rows' workgroups share them in L2.
-  const uint16_t* ra = src.p + (size_t)max(y - 1, 0) * src.pitch;
-  const uint16_t* rp = src.p + (size_t)y * src.pitch;
-  const uint16_t* rb = src.p + (size_t)min(y + 1, src.h - 1) * src.pitch;
-  const uint4 qa = *reinterpret_cast<const uint4*>(ra + x0);
-  const uint4 qp = *reinterpret_cast<const uint4*>(rp + x0);
-  const uint4 qb = *reinterpret_cast<const uint4*>(rb + x0);
-  // SAO CTBs are at least 8 samples wide, so the chunk lies in one CTB
-  const int rx = x0 >> log2c, ry = y >> log2c;
+  // Rows ys-1 .. ys+SAO_ROWS (clamped) as 8-sample reads, and per row the
+  // sample beyond the wave's end (x0-1 on lane 0, x0+8 elsewhere, clamped),
+  // all issued with the parameter read; rows are padded to the pitch (a
+  // multiple of 64), so the reads stay inside the plane past its width.
+  const int lane = threadIdx.x & 63;
+  const int ex = lane == 0 ? max(x0 - 1, 0) : min(x0 + 8, src.w - 1);
+  uint4 q[SAO_ROWS + 2];
+  uint32_t e[SAO_ROWS + 2];
+#pragma unroll
+  for (int r = 0; r < SAO_ROWS + 2; r++) {
+    const uint16_t* row = src.p + (size_t)clampi(ys - 1 + r, 0, src.h - 1) * src.pitch;
+    q[r] = *reinterpret_cast<const uint4*>(row + x0);
+    e[r] = row[ex];
+  }
+  const int rx = x0 >> log2c, ry = ys >> log2c;
   uint64_t prm = 0;
   size_t ci = 0;
   if (active && rx < j.swctb && ry < j.shctb) {
@@ -228,80 +239,66 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
     prm = j.sao[ci * 3 + comp];
   }
   // samples x0-1 and x0+8 of each row: from the neighbouring lanes (which
-  // hold the adjacent chunks), read directly at the wave's ends
-  const int lane = threadIdx.x & 63;
-  auto unpack = [&](const uint4& q, const uint16_t* r, int (&R)[10]) {
-    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+  // hold the adjacent chunks), at the wave's ends from e[]
+  sao_row8 Rw[SAO_ROWS + 2];
 #pragma unroll
-    for (int i = 0; i < 8; i++) R[i + 1] = (int)((w[i >> 1] >> (16 * (i & 1))) & 0xffffu);
-    int left = __shfl_up(R[8], 1), right = __shfl_down(R[1], 1);
-    if (lane == 0) left = x0 > 0 ? r[x0 - 1] : R[1];
-    if (lane == 63 || x0 + 8 >= src.w) right = x0 + 8 < src.w ? r[x0 + 8] : R[8];
-    R[0] = x0 > 0 ? left : R[1];
-    R[9] = right;
-  };
-  int A[10], P[10], B[10];
-  unpack(qa, ra, A);
-  unpack(qp, rp, P);
-  unpack(qb, rb, B);
+  for (int r = 0; r < SAO_ROWS + 2; r++) {
+    const uint4& v = q[r];
+    Rw[r].w[0] = v.x; Rw[r].w[1] = v.y; Rw[r].w[2] = v.z; Rw[r].w[3] = v.w;
+    const uint32_t up = (uint32_t)__shfl_up((int)(v.w >> 16), 1), down = (uint32_t)__shfl_down((int)(v.x & 0xffffu), 1);
+    Rw[r].l = lane == 0 ? (x0 > 0 ? e[r] : (v.x & 0xffffu)) : up;
+    Rw[r].r = x0 + 8 >= src.w ? (v.w >> 16) : (lane == 63 ? e[r] : down);
+  }
   const sao_run run = sao_run_of(prm);
-  int o[8];
+  const bool packed = run.fits && j.sao_bd <= 15;
+  const uint32_t maxv2 = (uint32_t)maxv * 0x10001u;
+  const int cls = run.aux & 3;
+  // the strip's neighbourhoods inside the picture and the CTB rectangle?
+  int cx0 = 0, cy0 = 0, cx1 = src.w, cy1 = src.h;
+  if (run.type == 2 && j.sao_rect) {
+    const uint64_t r = j.sao_rect[ci];
+    cx0 = (int)(r & 0xffff); cy0 = (int)((r >> 16) & 0xffff); cx1 = (int)((r >> 32) & 0xffff); cy1 = (int)(r >> 48);
+    if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
+    cx0 = max(cx0, 0); cy0 = max(cy0, 0); cx1 = min(cx1, src.w); cy1 = min(cy1, src.h);
+  }
+  const bool xin = x0 - 1 >= cx0 && x0 + 8 < cx1;
+  // f(r, o) filters row ys + r into o
+  auto each_row = [&](auto&& f) {
 #pragma unroll
-  for (int i = 0; i < 8; i++) o[i] = P[i + 1];
-  if (run.type == 1) {
-    if (run.fits) {
-      sao_bo8(P, run.aux, run.tab, j.sao_bd, maxv, o);
-    } else {
-#pragma unroll
-      for (int i = 0; i < 8; i++) o[i] = sao_band_sample(P[i + 1], prm, j.sao_bd, maxv);
-    }
-  } else if (run.type == 2) {
-    const int cls = run.aux & 3;
-    int h0, v0, h1, v1;
-    eo_offsets(cls, h0, v0, h1, v1);
-    // the chunk's neighbourhood inside the picture and the CTB rectangle?
-    int cx0 = 0, cy0 = 0, cx1 = src.w, cy1 = src.h;
-    if (j.sao_rect) {
-      const uint64_t r = j.sao_rect[ci];
-      cx0 = (int)(r & 0xffff); cy0 = (int)((r >> 16) & 0xffff); cx1 = (int)((r >> 32) & 0xffff); cy1 = (int)(r >> 48);
-      if (sub) { cx0 >>= 1; cy0 >>= 1; cx1 = (cx1 + 1) >> 1; cy1 = (cy1 + 1) >> 1; }
-      cx0 = max(cx0, 0); cy0 = max(cy0, 0); cx1 = min(cx1, src.w); cy1 = min(cy1, src.h);
-    }
-    const bool inner = x0 - 1 >= cx0 && x0 + 8 < cx1 && y - 1 >= cy0 && y + 1 < cy1;
-    if (inner && run.fits) {
-      sao_eo8_any(cls, A, P, B, run.tab, maxv, o);
-    } else {
-      // per sample, with the picture / rectangle check of each neighbour
-      auto slow = [&](auto cls_tag) {
-        constexpr int CLS = decltype(cls_tag)::value;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-          const int pv = P[i + 1];
-          const int n0 = CLS == 0 ? P[i] : (CLS == 1 ? A[i + 1] : (CLS == 2 ? A[i] : A[i + 2]));
-          const int n1 = CLS == 0 ? P[i + 2] : (CLS == 1 ? B[i + 1] : (CLS == 2 ? B[i + 2] : B[i]));
-          const int x = x0 + i, nx0 = x + h0, ny0 = y + v0, nx1 = x + h1, ny1 = y + v1;
-          const bool outside = nx0 < cx0 || ny0 < cy0 || nx1 < cx0 || ny1 < cy0 || nx0 >= cx1 || ny0 >= cy1 ||
-                               nx1 >= cx1 || ny1 >= cy1;
-          o[i] = outside ? clampi(pv, 0, maxv) : sao_edge_sample(pv, n0, n1, prm, maxv);
-        }
-      };
-      switch (cls) {
-        case 0: slow(std::integral_constant<int, 0>{}); break;
-        case 1: slow(std::integral_constant<int, 1>{}); break;
-        case 2: slow(std::integral_constant<int, 2>{}); break;
-        default: slow(std::integral_constant<int, 3>{}); break;
+    for (int r = 0; r < SAO_ROWS; r++) {
+      if (r >= nr) break;
+      uint32_t o[4] = {Rw[r + 1].w[0], Rw[r + 1].w[1], Rw[r + 1].w[2], Rw[r + 1].w[3]};
+      f(r, o);
+      uint16_t* drow = dst.p + (size_t)(ys + r) * dst.pitch;
+      if (x0 + 8 <= src.w) {
+        *reinterpret_cast<uint4*>(drow + x0) = make_uint4(o[0], o[1], o[2], o[3]);
+      } else {
+        for (int i = 0; i < 8; i++)
+          if (x0 + i < src.w) drow[x0 + i] = (uint16_t)(o[i >> 1] >> (16 * (i & 1)));
       }
     }
-  }
-  if (x0 + 8 <= src.w) {
-    uint32_t ow[4];
+  };
+  if (run.type == 1) {
+    each_row([&](int r, uint32_t (&o)[4]) {
+      if (packed) {
+        sao_bo8(Rw[r + 1], run.aux, run.btab, j.sao_bd, maxv2, o);
+      } else {
 #pragma unroll
-    for (int q = 0; q < 4; q++) ow[q] = (uint32_t)o[2 * q] | ((uint32_t)o[2 * q + 1] << 16);
-    *reinterpret_cast<uint4*>(drow + x0) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
-  } else {
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-      if (x0 + i < src.w) drow[x0 + i] = (uint16_t)o[i];
+        for (int i = 0; i < 8; i++) sao_put(o, i, sao_band_sample(sao_at(Rw[r + 1], i), prm, j.sao_bd, maxv));
+      }
+    });
+  } else if (run.type == 2) {
+    int dx0, dy0, dx1, dy1;
+    eo_offsets(cls, dx0, dy0, dx1, dy1);
+    each_row([&](int r, uint32_t (&o)[4]) {
+      const int y = ys + r;
+      if (packed && xin && y - 1 >= cy0 && y + 1 < cy1)
+        sao_eo8_any(cls, Rw[r], Rw[r + 1], Rw[r + 2], run.btab, maxv2, o);
+      else  // per sample, with the picture / rectangle check of each neighbour
+        sao_eo8_checked(dx0, dy0, dx1, dy1, Rw[r], Rw[r + 1], Rw[r + 2], x0, y, cx0, cy0, cx1, cy1, prm, maxv, o);
+    });
+  } else {  // off: copy
+    each_row([&](int, uint32_t (&)[4]) {});
   }
 }
 
@@ -544,7 +541,7 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
     if (lb + cb) hipLaunchKernelGGL(k_dbk<false>, dim3(lb + cb), dim3(256), 0, s, j, (int)lb);
   }
   mark(3);
-  if (j.do_sao) hipLaunchKernelGGL(k_sao, dim3(cdiv(j.w, 2048), j.h, ncomp), dim3(256), 0, s, j);
+  if (j.do_sao) hipLaunchKernelGGL(k_sao, dim3(cdiv(j.w, 2048), cdiv(j.h, SAO_ROWS), ncomp), dim3(256), 0, s, j);
   mark(4);
   if (j.do_alf) {
     int tl = j.ctb_log2 < 6 ? j.ctb_log2 : 6;
