@@ -44,15 +44,17 @@ __device__ __forceinline__ int rdc(const plane_view& v, int x, int y) {
 // ---------------------------------------------------------------------------
 // prep: 8 samples per thread, rows of every plane (blockIdx.z = component).
 // ---------------------------------------------------------------------------
-// A workgroup covers 2048 samples of PREP_ROWS rows; for luma with LMCS at
-// <= 10 bits the first 1024 entries of the inverse table are staged in LDS.
-constexpr int PREP_ROWS = 8;
+// A workgroup covers 2048 samples of the rows blockIdx.y + k * gridDim.y (at
+// most PREP_GRID_Y row workgroups per plane, so that for luma with LMCS at
+// <= 10 bits the first 1024 entries of the inverse table are staged in LDS
+// once per workgroup, not once per row).
+constexpr int PREP_GRID_Y = 512;
+
 __global__ void __launch_bounds__(256) k_prep(dev_job j) {
   __shared__ uint16_t lut[1024];
   const int comp = blockIdx.z;
   plane_view src = pick(j.in, j, comp), dst = pick(j.a, j, comp);
   const int x = (blockIdx.x * 256 + threadIdx.x) * 8;
-  const int y0 = blockIdx.y * PREP_ROWS;
   const bool map = comp == 0 && j.do_lmcs;
   const bool lut_lds = map && j.bd <= 10;
   if (lut_lds) {
@@ -60,15 +62,10 @@ __global__ void __launch_bounds__(256) k_prep(dev_job j) {
     __syncthreads();
   }
   if (x >= src.w) return;
-  uint4 v[PREP_ROWS];
-#pragma unroll
-  for (int r = 0; r < PREP_ROWS; r++)
-    if (y0 + r < src.h) v[r] = *reinterpret_cast<const uint4*>(src.p + (size_t)(y0 + r) * src.pitch + x);
-#pragma unroll
-  for (int r = 0; r < PREP_ROWS; r++) {
-    if (y0 + r >= src.h) break;
+  for (int y = blockIdx.y; y < src.h; y += gridDim.y) {
+    uint4 v = *reinterpret_cast<const uint4*>(src.p + (size_t)y * src.pitch + x);
     if (map) {
-      uint32_t w[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
+      uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const uint32_t a = w[i] & 0xffffu, b = w[i] >> 16;
@@ -76,9 +73,9 @@ __global__ void __launch_bounds__(256) k_prep(dev_job j) {
         const uint32_t hi = (lut_lds && b < 1024) ? lut[b] : gld(j.lmcs_lut + b);
         w[i] = lo | (hi << 16);
       }
-      v[r] = make_uint4(w[0], w[1], w[2], w[3]);
+      v = make_uint4(w[0], w[1], w[2], w[3]);
     }
-    *reinterpret_cast<uint4*>(dst.p + (size_t)(y0 + r) * dst.pitch + x) = v[r];
+    *reinterpret_cast<uint4*>(dst.p + (size_t)y * dst.pitch + x) = v;
   }
 }
 
@@ -203,7 +200,7 @@ __global__ void __launch_bounds__(256) k_dbk(dev_job j, int luma_blocks) {
 // table form, the other rows (and edge-offset rows whose neighbourhoods leave
 // the picture or the CTB's rectangle) go sample by sample.
 // ---------------------------------------------------------------------------
-constexpr int SAO_ROWS = 4;
+constexpr int SAO_ROWS = 1;
 
 __global__ void __launch_bounds__(256) k_sao(dev_job j) {
   const int comp = blockIdx.z;
@@ -526,7 +523,7 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
   };
   mark(0);
   // prep: in -> A (rows of 2048 samples per block)
-  hipLaunchKernelGGL(k_prep, dim3(cdiv(j.w, 2048), cdiv(j.h, PREP_ROWS), ncomp), dim3(256), 0, s, j);
+  hipLaunchKernelGGL(k_prep, dim3(cdiv(j.w, 2048), std::min(j.h, PREP_GRID_Y), ncomp), dim3(256), 0, s, j);
   mark(1);
   // deblocking: all vertical edges (luma and chroma in one launch), then all
   // horizontal edges
