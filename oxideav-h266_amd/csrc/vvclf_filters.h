@@ -297,6 +297,12 @@ __device__ __forceinline__ void luma_filter_line(const int* p, const int* q, con
   }
 }
 
+// Two samples (low and high half of v, each below 1024) through a 1024-entry
+// table.
+__device__ __forceinline__ uint32_t lut_pair(const uint16_t* lut, uint32_t v) {
+  return (uint32_t)lut[v & 0x3ffu] | ((uint32_t)lut[(v >> 16) & 0x3ffu] << 16);
+}
+
 // ---------------------------------------------------------------------------
 // Chroma deblocking, one line: c[0..7] = p3 p2 p1 p0 q0 q1 q2 q3.
 // ---------------------------------------------------------------------------

@@ -645,24 +645,36 @@ struct tile_loads {
     const int W = j.w, H = j.h;
     const lwin& L = g.L;
     const int nch = (L.w + 7) >> 3, total = L.h * NCH;
-    const int lut_lds = j.bd <= 10;
+    // LMCS: samples below 1024 through the table staged in LDS, with no branch
+    // per sample; chunks holding a larger sample (bit depth above 10, or
+    // samples beyond the bit depth) are rewritten from the global table.
+    const bool lut_lds = j.bd <= 10;
+    bool far = false;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
       if (i < total && ch < nch) {
         uint4 w = v[k];
         if (j.do_lmcs) {
-          uint32_t w4[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const uint32_t a = w4[q] & 0xffffu, b = w4[q] >> 16;
-            const uint32_t la = (lut_lds && a < 1024) ? s.lut[a] : gld(j.lmcs_lut + a);
-            const uint32_t lb = (lut_lds && b < 1024) ? s.lut[b] : gld(j.lmcs_lut + b);
-            w4[q] = la | (lb << 16);
-          }
-          w = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+          far |= !lut_lds || ((w.x | w.y | w.z | w.w) & 0xfc00fc00u) != 0;
+          if (lut_lds)
+            w = make_uint4(lut_pair(s.lut, w.x), lut_pair(s.lut, w.y), lut_pair(s.lut, w.z), lut_pair(s.lut, w.w));
         }
         *reinterpret_cast<uint4*>(&s.L[yy * LD::LW + 8 * ch]) = w;
+      }
+    }
+    if (far) {
+#pragma unroll
+      for (int k = 0; k < PER; k++) {
+        const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
+        const uint4 w = v[k];
+        if (i < total && ch < nch && (!lut_lds || ((w.x | w.y | w.z | w.w) & 0xfc00fc00u) != 0)) {
+          uint32_t w4[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            w4[q] = (uint32_t)gld(j.lmcs_lut + (w4[q] & 0xffffu)) | ((uint32_t)gld(j.lmcs_lut + (w4[q] >> 16)) << 16);
+          *reinterpret_cast<uint4*>(&s.L[yy * LD::LW + 8 * ch]) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        }
       }
     }
     if (g.chroma) {

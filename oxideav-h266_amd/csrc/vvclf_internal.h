@@ -110,7 +110,7 @@ struct dev_job {
   dev_planes a;    // work buffer A (LMCS + deblock in place)
   dev_planes b;    // work buffer B (SAO output)
   dev_planes out;  // final output
-  const uint16_t* lmcs_lut;  // 1 << bd entries
+  const uint16_t* lmcs_lut;  // 65536 entries: every u16 sample value
   // deblock edge maps
   int32_t gw, gh;            // luma 4x4 grid dims (ceil(w/4), ceil(h/4))
   const uint32_t* e_lv;      // [gh][gw] vertical edge at x=4*gx, rows 4*gy..+3

@@ -65,15 +65,17 @@ __global__ void __launch_bounds__(256) k_prep(dev_job j) {
   for (int y = blockIdx.y; y < src.h; y += gridDim.y) {
     uint4 v = *reinterpret_cast<const uint4*>(src.p + (size_t)y * src.pitch + x);
     if (map) {
-      uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      // samples below 1024 through the LDS table with no branch per sample;
+      // chunks with a larger sample through the global table
+      if (lut_lds && ((v.x | v.y | v.z | v.w) & 0xfc00fc00u) == 0) {
+        v = make_uint4(lut_pair(lut, v.x), lut_pair(lut, v.y), lut_pair(lut, v.z), lut_pair(lut, v.w));
+      } else {
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const uint32_t a = w[i] & 0xffffu, b = w[i] >> 16;
-        const uint32_t lo = (lut_lds && a < 1024) ? lut[a] : gld(j.lmcs_lut + a);
-        const uint32_t hi = (lut_lds && b < 1024) ? lut[b] : gld(j.lmcs_lut + b);
-        w[i] = lo | (hi << 16);
+        for (int i = 0; i < 4; i++)
+          w[i] = (uint32_t)gld(j.lmcs_lut + (w[i] & 0xffffu)) | ((uint32_t)gld(j.lmcs_lut + (w[i] >> 16)) << 16);
+        v = make_uint4(w[0], w[1], w[2], w[3]);
       }
-      v = make_uint4(w[0], w[1], w[2], w[3]);
     }
     *reinterpret_cast<uint4*>(dst.p + (size_t)y * dst.pitch + x) = v;
   }
