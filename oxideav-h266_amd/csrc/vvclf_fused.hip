@@ -78,19 +78,6 @@ __device__ __forceinline__ int ltid() {
   return t;
 }
 
-// Appends `val` for every lane with `pred` to an LDS list (one LDS atomic
-// per wave).  Called by all active lanes of the wave.
-__device__ __forceinline__ void list_append(bool pred, uint32_t val, uint16_t* list, uint32_t* cnt) {
-  const uint64_t m = __ballot(pred);
-  if (m == 0) return;
-  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t base = 0;
-  if ((int)(ltid() & 63) == leader) base = atomicAdd(cnt, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  if (pred) list[base + below] = (uint16_t)val;
-}
-
 // A rectangle of an edge-descriptor map (map units), staged in LDS with
 // origin (x0, y0).
 struct mrect {
@@ -113,15 +100,33 @@ struct rect_stage {
       v[k] = (map && yy < r.ny && xx < r.nx) ? gld(&map[(size_t)(r.y0 + yy) * mw + r.x0 + xx]) : 0u;
     }
   }
-  // also lists the nonzero words as tag | (row << 6) | column
+  // also lists the nonzero words as tag | (row << 6) | column, reserving the
+  // wave's list entries with one LDS atomic.  Called by all lanes of the
+  // wave.
   __device__ __forceinline__ void commit(uint32_t* dst, const mrect& r, uint16_t* list, uint32_t* cnt,
                                          uint32_t tag) const {
+    uint64_t m[PER];
+    uint32_t tot = 0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       const int i = ltid() + k * NT, yy = i / MW, xx = i - yy * MW;
       const bool in = yy < r.ny && xx < r.nx;
       if (in) dst[i] = v[k];
-      list_append(in && v[k] != 0, tag | (uint32_t)(yy << 6) | (uint32_t)xx, list, cnt);
+      m[k] = __ballot(in && v[k] != 0);
+      tot += (uint32_t)__popcll(m[k]);
+    }
+    if (tot == 0) return;
+    uint32_t base = 0;
+    if ((ltid() & 63) == 0) base = atomicAdd(cnt, tot);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[k] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[k], 0u));
+      if ((m[k] >> (ltid() & 63)) & 1) {
+        const int i = ltid() + k * NT, yy = i / MW, xx = i - yy * MW;
+        list[base + below] = (uint16_t)(tag | (uint32_t)(yy << 6) | (uint32_t)xx);
+      }
+      base += (uint32_t)__popcll(m[k]);
     }
   }
 };
