@@ -46,22 +46,34 @@ __device__ __forceinline__ void gst(uint4* p, uint4 v) {
   *(VVCLF_GLOBAL vu32x4*)p = vu32x4{v.x, v.y, v.z, v.w};
 }
 
-// Table 43 (spec data): beta' and tC' indexed by the clamped Q.
-__constant__ static const int16_t kBeta[64] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,
-                                              6,  7,  8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24,
-                                              26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56,
-                                              58, 60, 62, 64, 66, 68, 70, 72, 74, 76, 78, 80, 82, 84, 86, 88};
+// Table 43 (spec data): beta' and tC' indexed by the clamped Q.  beta' is
+// evaluated in closed form on the device (0 below 16, Q - 10 up to 28, then
+// 2Q - 38), checked against the table at compile time; tC' is read from a
+// table (kTc in constant memory, or a copy in LDS).
+constexpr int16_t kBetaTab[64] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,
+                                  6,  7,  8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24,
+                                  26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56,
+                                  58, 60, 62, 64, 66, 68, 70, 72, 74, 76, 78, 80, 82, 84, 86, 88};
+__host__ __device__ constexpr int beta_prime(int q) { return q < 16 ? 0 : (q <= 28 ? q - 10 : 2 * q - 38); }
+constexpr bool beta_closed_form() {
+  for (int q = 0; q < 64; q++)
+    if (beta_prime(q) != kBetaTab[q]) return false;
+  return true;
+}
+static_assert(beta_closed_form(), "beta' closed form");
 __constant__ static const int16_t kTc[66] = {0,  0,  0,  0,  0,  0,   0,   0,   0,   0,   0,   0,   0,   0,
                                             0,  0,  0,  0,  3,  4,   4,   4,   4,   5,   5,   5,   5,   7,
                                             7,  8,  9,  10, 10, 11,  13,  14,  15,  17,  19,  21,  24,  25,
                                             29, 33, 36, 41, 45, 51,  57,  64,  71,  80,  89,  100, 112, 125,
                                             141, 157, 177, 198, 222, 250, 280, 314, 352, 395};
+constexpr int TC_TAB_LEN = 66;
 
-// beta and tC of a packed edge descriptor (eqs. 1276-1279, src/deblock.rs:439-452).
-__device__ __forceinline__ void edge_thresholds(uint32_t e, int bd, int& beta, int& tc) {
+// beta and tC of a packed edge descriptor (eqs. 1276-1279, src/deblock.rs:439-452);
+// tct: the tC' table (kTc, or its copy in LDS).
+__device__ __forceinline__ void edge_thresholds(uint32_t e, int bd, const int16_t* tct, int& beta, int& tc) {
   int qb = (e >> E_QB_SHIFT) & 63, qt = (e >> E_QT_SHIFT) & 127;
-  beta = (int)kBeta[qb] * (1 << (bd - 8));
-  int tcp = kTc[qt];
+  beta = beta_prime(qb) * (1 << (bd - 8));
+  int tcp = tct[qt];
   if (bd < 10) {
     tc = (tcp + (1 << (9 - bd))) >> (10 - bd);
   } else {
@@ -557,10 +569,26 @@ __device__ __forceinline__ int alf_pair(uint32_t w, int a, int b, int c) {
   return f * (clampi(a - c, -k, k) + clampi(b - c, -k, k));
 }
 
+// The activity and transpose tables of the classification (src/alf.rs:822-1010)
+// as register constants: entry i of VAR_TAB in bits 4i..4i+3 of ALF_VAR_NIB,
+// entry i of TRANSPOSE in bits 2i..2i+1 of ALF_TRANSPOSE_2B (constant arrays
+// would be global loads, and their waits would also wait for stores in
+// flight).
+constexpr uint8_t ALF_VAR_TAB[16] = {0, 1, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3, 4};
+constexpr uint8_t ALF_TRANSPOSE[8] = {0, 1, 0, 2, 2, 3, 1, 3};
+constexpr uint64_t ALF_VAR_NIB = 0x4333333332222210ull;
+constexpr uint32_t ALF_TRANSPOSE_2B = 0xde84u;
+constexpr bool alf_tables_packed() {
+  for (int i = 0; i < 16; i++)
+    if (((ALF_VAR_NIB >> (4 * i)) & 0xf) != ALF_VAR_TAB[i]) return false;
+  for (int i = 0; i < 8; i++)
+    if (((ALF_TRANSPOSE_2B >> (2 * i)) & 3) != ALF_TRANSPOSE[i]) return false;
+  return true;
+}
+static_assert(alf_tables_packed(), "packed ALF classification tables");
+
 // 4x4-block classification (src/alf.rs:822-1010) from the Laplacian sums.
 __device__ __forceinline__ int alf_class_from_sums(int sh, int sv, int sd0, int sd1, int ac, int bd) {
-  const uint8_t VAR_TAB[16] = {0, 1, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3, 4};
-  const uint8_t TRANSPOSE[8] = {0, 1, 0, 2, 2, 3, 1, 3};
   int hv1, hv0, dir_hv, d1, d0, dir_d;
   if (sv > sh) { hv1 = sv; hv0 = sh; dir_hv = 1; } else { hv1 = sh; hv0 = sv; dir_hv = 3; }
   if (sd0 > sd1) { d1 = sd0; d0 = sd1; dir_d = 0; } else { d1 = sd1; d0 = sd0; dir_d = 2; }
@@ -571,8 +599,8 @@ __device__ __forceinline__ int alf_class_from_sums(int sh, int sv, int sd0, int 
   int64_t prod = (int64_t)(sh + sv) * ac;
   int64_t shifted = prod >> (bd - 1);
   shifted = shifted < 0 ? 0 : (shifted > 15 ? 15 : shifted);
-  int filt = VAR_TAB[shifted];
-  int tr = TRANSPOSE[dir1 * 2 + (dir2 >> 1)];
+  int filt = (int)((ALF_VAR_NIB >> (4 * (int)shifted)) & 0xf);
+  int tr = (int)((ALF_TRANSPOSE_2B >> (2 * (dir1 * 2 + (dir2 >> 1)))) & 3);
   if (dir_s != 0) filt += (((dir1 & 1) << 1) + dir_s) * 5;
   return filt | (tr << 5);
 }
@@ -879,7 +907,7 @@ __device__ __forceinline__ void alf_luma_block_lds(const uint16_t* S, int sox, i
   if (!alf_on) {
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
-      const uint2 q = *reinterpret_cast<const uint2*>(&S[(y4 + rr - soy) * SW + (x4 - sox)]);
+      const uint2 q = *reinterpret_cast<const uint2*>(&S[__mul24(y4 + rr - soy, SW) + (x4 - sox)]);
       outp[rr][0] = q.x;
       outp[rr][1] = q.y;
     }
@@ -888,7 +916,7 @@ __device__ __forceinline__ void alf_luma_block_lds(const uint16_t* S, int sox, i
 #pragma unroll
     for (int rr = 0; rr < 10; rr++)
 #pragma unroll
-      for (int cc = 0; cc < 10; cc++) Wn[rr][cc] = S[(y4 - 3 + rr - soy) * SW + (x4 - 3 + cc - sox)];
+      for (int cc = 0; cc < 10; cc++) Wn[rr][cc] = S[__mul24(y4 - 3 + rr - soy, SW) + (x4 - 3 + cc - sox)];
     int outv[4][4];
     alf_block(Wn, bank, y4 - y_ctb, ctb, carve, alf_bd, amax, outv);
 #pragma unroll
@@ -897,7 +925,7 @@ __device__ __forceinline__ void alf_luma_block_lds(const uint16_t* S, int sox, i
       outp[rr][1] = (uint32_t)outv[rr][2] | ((uint32_t)outv[rr][3] << 16);
     }
   } else {
-    const uint16_t* base = &S[(y4 - 3 - soy) * SW + (x4 - 4 - sox)];
+    const uint16_t* base = &S[__mul24(y4 - 3 - soy, SW) + (x4 - 4 - sox)];
     const int vb = alf_vb_kind(y4 - y_ctb, ctb);
     const int cls = alf_lds_block_class<SW>(base, vb, carve, alf_bd);
     const uint32_t* f = bank + ((cls & 31) * 4 + (cls >> 5)) * 12;

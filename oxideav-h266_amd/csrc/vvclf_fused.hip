@@ -63,6 +63,7 @@ struct fused_lds {
                         // inside the picture and the CTB's rectangle (u16 fields)
   uint32_t nlist[4];    // entries in llv, llh, lcv, lch
   uint16_t lut[1024];
+  int16_t tct[68];  // the deblocking tC' table, copied once per workgroup
   static_assert(MLV_W <= 64 && MLH_W <= 64 && MCV_W <= 64 && MCH_W <= 64, "list column field");
   static_assert(sizeof(maps_t) <= sizeof(sao_out_t), "maps alias the SAO output");
   static_assert(TW <= LW && TH <= LH && TW / 2 <= CW && TH / 2 <= CH, "output buffers fit the input windows");
@@ -157,7 +158,7 @@ __device__ __forceinline__ int cdiv_i(int a, int b) { return -fdiv(-a, b); }
 // HOR : edge rows gy in [gy0, gy1], segment columns gx in [gx0, gx1].
 template <bool VERT, int PITCH>
 __device__ __forceinline__ void lds_deblock_luma(const dev_job& j, lwin& L, const uint32_t* map, int mpitch, const mrect& mr,
-                                 const uint16_t* list, int nlist, int maxv) {
+                                 const uint16_t* list, int nlist, int maxv, const int16_t* tct) {
   const int tid = ltid();
   for (int base = 0; base < nlist; base += blockDim.x / 4) {
     const int it = base + (tid >> 2), line = tid & 3;
@@ -193,7 +194,7 @@ __device__ __forceinline__ void lds_deblock_luma(const dev_job& j, lwin& L, cons
         }
       }
       int beta, tc;
-      edge_thresholds(e, j.bd_dbk, beta, tc);
+      edge_thresholds(e, j.bd_dbk, tct, beta, tc);
       int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
       luma_terms mine = luma_line_terms(p, q, mflp, mflq);
       luma_terms t0 = quad_terms<0>(mine), t3 = quad_terms<3>(mine);
@@ -230,7 +231,8 @@ __device__ __forceinline__ void lds_deblock_luma(const dev_job& j, lwin& L, cons
 // HOR : edge rows sy in [sy0, sy1] (y = 8 sy), segment columns sx in [sx0, sx1] (x = 2 sx).
 template <bool VERT, int PITCH>
 __device__ __forceinline__ void lds_deblock_chroma(lwin cb, uint16_t* cr_p, const uint32_t* map0, const uint32_t* map1, int mpitch,
-                                   const mrect& mr, const uint16_t* list, int nlist, int bd_dbk, int maxv) {
+                                   const mrect& mr, const uint16_t* list, int nlist, int bd_dbk, int maxv,
+                                   const int16_t* tct) {
   const int tid = ltid();
   for (int base = 0; base < nlist; base += blockDim.x / 2) {
     const int it = base + (tid >> 1), line = tid & 1;
@@ -261,7 +263,7 @@ __device__ __forceinline__ void lds_deblock_chroma(lwin cb, uint16_t* cr_p, cons
         for (int i = 0; i < 8; i++) c[i] = VERT ? L.rd(x0 - 4 + i, y0) : L.rd(x0, y0 - 4 + i);
       }
       int beta, tc;
-      edge_thresholds(e, bd_dbk, beta, tc);
+      edge_thresholds(e, bd_dbk, tct, beta, tc);
       int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
       bool eligible, short_p;
       if (VERT) { eligible = mflp == 3 && mflq == 3; short_p = false; }
@@ -766,15 +768,15 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   __syncthreads();
   // ---- deblocking: all vertical edges, then all horizontal edges
   if (dbk) {
-    lds_deblock_luma<true, LD::LW>(j, L, s.m.lv, LD::MLV_W, mlv, s.m.llv, (int)s.nlist[0], maxv);
+    lds_deblock_luma<true, LD::LW>(j, L, s.m.lv, LD::MLV_W, mlv, s.m.llv, (int)s.nlist[0], maxv, s.tct);
     if (dbk_c)
       lds_deblock_chroma<true, LD::CW>(Cw, s.Cc[1], s.m.cv[0], s.m.cv[1], LD::MCV_W, mcv, s.m.lcv, (int)s.nlist[2], j.bd_dbk,
-                               maxv);
+                               maxv, s.tct);
     __syncthreads();
-    lds_deblock_luma<false, LD::LW>(j, L, s.m.lh, LD::MLH_W, mlh, s.m.llh, (int)s.nlist[1], maxv);
+    lds_deblock_luma<false, LD::LW>(j, L, s.m.lh, LD::MLH_W, mlh, s.m.llh, (int)s.nlist[1], maxv, s.tct);
     if (dbk_c)
       lds_deblock_chroma<false, LD::CW>(Cw, s.Cc[1], s.m.ch[0], s.m.ch[1], LD::MCH_W, mch, s.m.lch, (int)s.nlist[3], j.bd_dbk,
-                                maxv);
+                                maxv, s.tct);
   }
   __syncthreads();  // deblocking done; the edge descriptors' LDS is reused
   // ---- SAO into the second buffers.  S stores luma [x0-4, x0+TW+4) x
@@ -824,7 +826,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   }
   __syncthreads();
   // padded luma sample (x, y), |x - [x0, xe)| <= 3, |y - [y0, ye)| <= 3
-  auto Sl = [&](int x, int y) -> int { return s.o.S[(y - soy) * LD::SW + (x - sox)]; };
+  auto Sl = [&](int x, int y) -> int { return s.o.S[__mul24(y - soy, LD::SW) + (x - sox)]; };
   // luma: one 4x4 block per thread
   {
     const int amax = (1 << j.alf_bd) - 1;
@@ -833,9 +835,13 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     // results go to the tile's output buffer in LDS (pitch TW, in the input
     // window's space, dead since SAO) and leave in one pass at the end
     uint16_t* const ob = s.L;
+    // blocks on a grid of BW per row (compile-time divisor); blocks past the
+    // tile's width are skipped
+    constexpr int BW = TW / 4;
 #pragma unroll 1
-    for (int b = ltid(); b < bw * bh; b += NT) {
-      const int x4 = x0 + 4 * (b % bw), y4 = y0 + 4 * (b / bw);
+    for (int b = ltid(); b < BW * bh; b += NT) {
+      const int bx = b % BW, x4 = x0 + 4 * bx, y4 = y0 + 4 * (b / BW);
+      if (bx >= bw) continue;
       uint32_t outp[4][2];
       alf_luma_block_lds<LD::SW, HBD>(s.o.S, sox, soy, x4, y4, s.bank, bank_idx != (int)ALF_OFF8, y_ctb, ctb, carve,
                                       j.alf_bd, outp);
@@ -856,17 +862,21 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   // pairs (x, x+1) are aligned 32-bit reads and (x-1, x) / (x+1, x+2) one
   // alignbit each.
   if (chroma) {
-    const int cwid = cx1 - cx0, chei = cy1 - cy0, pw2 = (cwid + 1) >> 1, npair = pw2 * chei;
+    // pairs on a grid of PW per row (compile-time divisor); pairs past the
+    // tile's chroma width are skipped
+    constexpr int PW = TW / 4;
+    const int cwid = cx1 - cx0, chei = cy1 - cy0, npair = PW * chei;
     const int ctbc = ctb >> 1;
     const int amax = (1 << j.alf_bd) - 1, half = 1 << (j.alf_bd - 1);
     const s16x2 zero2 = {0, 0}, vmax2 = {(short)amax, (short)amax};
     for (int i = ltid(); i < 2 * npair; i += NT) {
       const int c = i >= npair ? 1 : 0, k = i - c * npair;
-      const int yy = k / pw2, x = cx0 + 2 * (k - yy * pw2), y = cy0 + yy;
+      const int yy = k / PW, x = cx0 + 2 * (k - yy * PW), y = cy0 + yy;
+      if (x >= cx0 + cwid) continue;
       const uint16_t* sc = s.o.Sc[c];
       // pair of padded chroma samples (xa, xa+1), xa - cx0 even
       auto P2 = [&](int xa, int ya) -> uint32_t {
-        return *reinterpret_cast<const uint32_t*>(&sc[(ya - scoy) * LD::SCW + (xa - scox)]);
+        return *reinterpret_cast<const uint32_t*>(&sc[__mul24(ya - scoy, LD::SCW) + (xa - scox)]);
       };
       uint32_t outp = P2(x, y);
       if (in_grid) {
@@ -925,10 +935,12 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
           for (int h = 0; h < 2; h++) {
             const int xl = 2 * (x + h);
             const int cen = Sl(xl, yl);
-            const int sum = cf[0] * (Sl(xl, yl - yp1) - cen) + cf[1] * (Sl(xl - 1, yl) - cen) +
-                            cf[2] * (Sl(xl + 1, yl) - cen) + cf[3] * (Sl(xl - 1, yl + yp1) - cen) +
-                            cf[4] * (Sl(xl, yl + yp1) - cen) + cf[5] * (Sl(xl + 1, yl + yp1) - cen) +
-                            cf[6] * (Sl(xl, yl + yp2) - cen);
+            // 24-bit products: coefficients are below 2^23 in magnitude
+            // (checked on the host) and differences within 17 bits
+            const int sum = __mul24(cf[0], Sl(xl, yl - yp1) - cen) + __mul24(cf[1], Sl(xl - 1, yl) - cen) +
+                            __mul24(cf[2], Sl(xl + 1, yl) - cen) + __mul24(cf[3], Sl(xl - 1, yl + yp1) - cen) +
+                            __mul24(cf[4], Sl(xl, yl + yp1) - cen) + __mul24(cf[5], Sl(xl + 1, yl + yp1) - cen) +
+                            __mul24(cf[6], Sl(xl, yl + yp2) - cen);
             const int base = (int)((outp >> (16 * h)) & 0xffffu);
             o2[h] = (uint32_t)clampi(base + clampi((sum + 64) >> 7, -half, half - 1), 0, amax);
           }
@@ -995,6 +1007,7 @@ __global__ void __launch_bounds__((fused_lds<TW, TH>::NT), 3) k_fused(const dev_
   const uint32_t begin = xcd * chunk, end = min(n_tiles, begin + chunk);
   int cur_job = -1;
   if (ltid() < 4) s.nlist[ltid()] = 0;
+  if (ltid() < TC_TAB_LEN) s.tct[ltid()] = kTc[ltid()];
   __syncthreads();
   // Software pipeline: the loads of tile t+1 are issued while tile t is
   // filtered (inside fused_tile), those of the first tile here.

@@ -491,6 +491,34 @@ int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t,
       t.alf_ctb_log2 = ac.ctb_log2_size_y;
       t.alf_bd = ac.bit_depth;
       if (16 + b.n_luma_apses > 255) return fail(VVCLF_ERR_INVALID, "alf: too many luma APS slots");
+      // The filters hold ALF coefficients in 16-bit lanes and multiply CC-ALF
+      // coefficients in 24 bits; conformant streams stay far inside both
+      // (|c| <= 128 and <= 64), larger values are refused rather than
+      // filtered differently.
+      for (uint32_t i = 0; i < b.n_luma_apses; i++) {
+        const vvclf_alf_aps* a = b.luma_apses[i];
+        if (!a || !a->alf_luma_filter_signal_flag) continue;
+        for (int c = 0; c < 25; c++)
+          for (int k = 0; k < 12; k++)
+            if (a->luma_coeff[c][k] < -32768 || a->luma_coeff[c][k] > 32767)
+              return fail(VVCLF_ERR_UNSUPPORTED, "alf: luma coefficient %d outside 16 bits", a->luma_coeff[c][k]);
+      }
+      if (const vvclf_alf_aps* ca = b.chroma_aps; ca && ca->alf_chroma_filter_signal_flag)
+        for (uint32_t a = 0; a < std::min(ca->n_chroma_coeff, (uint32_t)VVCLF_ALF_MAX_CHROMA_ALT); a++)
+          for (int k = 0; k < 6; k++)
+            if (ca->chroma_coeff[a][k] < -32768 || ca->chroma_coeff[a][k] > 32767)
+              return fail(VVCLF_ERR_UNSUPPORTED, "alf: chroma coefficient %d outside 16 bits", ca->chroma_coeff[a][k]);
+      for (int c = 0; c < 2; c++) {
+        const vvclf_alf_aps* cc = c == 0 ? b.cc_cb_aps : b.cc_cr_aps;
+        if (!cc) continue;
+        const uint32_t nf = std::min(c == 0 ? cc->n_cc_cb_coeff : cc->n_cc_cr_coeff, (uint32_t)VVCLF_ALF_MAX_CC_FILTERS);
+        for (uint32_t k = 0; k < nf; k++)
+          for (int t = 0; t < VVCLF_ALF_CC_NUM_COEFF; t++) {
+            const int32_t v = c == 0 ? cc->cc_cb_coeff[k][t] : cc->cc_cr_coeff[k][t];
+            if (v < -(1 << 23) || v >= (1 << 23))
+              return fail(VVCLF_ERR_UNSUPPORTED, "alf: CC-ALF coefficient %d outside 24 bits", v);
+          }
+      }
       build_alf_banks(ac, b, t);
       std::vector<bool> slot_ok(b.n_luma_apses, false);
       for (uint32_t i = 0; i < b.n_luma_apses; i++) slot_ok[i] = b.luma_apses[i] && b.luma_apses[i]->alf_luma_filter_signal_flag;

@@ -115,7 +115,7 @@ __device__ __forceinline__ void dbk_luma_lane(const dev_job& j, int t) {
     q[i] = VERT ? rdc(v, x0 + i, y0) : rdc(v, x0, y0 + i);
   }
   int beta, tc;
-  edge_thresholds(e, j.bd_dbk, beta, tc);
+  edge_thresholds(e, j.bd_dbk, kTc, beta, tc);
   int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
   luma_terms mine = luma_line_terms(p, q, mflp, mflq);
   luma_terms t0 = quad_terms<0>(mine), t3 = quad_terms<3>(mine);
@@ -163,7 +163,7 @@ __device__ __forceinline__ void dbk_chroma_lane(const dev_job& j, int t) {
 #pragma unroll
   for (int i = 0; i < 8; i++) c[i] = VERT ? rdc(v, x0 - 4 + i, y0) : rdc(v, x0, y0 - 4 + i);
   int beta, tc;
-  edge_thresholds(e, j.bd_dbk, beta, tc);
+  edge_thresholds(e, j.bd_dbk, kTc, beta, tc);
   int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
   bool eligible, short_p;
   if (VERT) { eligible = mflp == 3 && mflq == 3; short_p = false; }
@@ -497,9 +497,12 @@ __global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
           return lum.p[(size_t)clampi(yl + dy, ly0, ly1 - 1) * lum.pitch + clampi(xl + dx, lx0, lx1 - 1)];
         };
         const int cen = L(0, 0);
-        const int sum = cf[0] * (L(0, -yp1) - cen) + cf[1] * (L(-1, 0) - cen) + cf[2] * (L(1, 0) - cen) +
-                        cf[3] * (L(-1, yp1) - cen) + cf[4] * (L(0, yp1) - cen) + cf[5] * (L(1, yp1) - cen) +
-                        cf[6] * (L(0, yp2) - cen);
+        // 24-bit products: coefficients are below 2^23 in magnitude (checked
+        // on the host) and differences within 17 bits
+        const int sum = __mul24(cf[0], L(0, -yp1) - cen) + __mul24(cf[1], L(-1, 0) - cen) +
+                        __mul24(cf[2], L(1, 0) - cen) + __mul24(cf[3], L(-1, yp1) - cen) +
+                        __mul24(cf[4], L(0, yp1) - cen) + __mul24(cf[5], L(1, yp1) - cen) +
+                        __mul24(cf[6], L(0, yp2) - cen);
         const int base = (int)((outp >> (16 * h)) & 0xffffu);
         o2[h] = (uint32_t)clampi(base + clampi((sum + 64) >> 7, -half, half - 1), 0, maxv);
       }

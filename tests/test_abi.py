@@ -77,6 +77,32 @@ def test_short_luma_aps_is_invalid(vv):
     assert vv.lib.vvclf_frame_check(f.desc, None) == vv.ERR_INVALID
 
 
+def test_alf_coefficients_beyond_the_filter_lanes_are_unsupported(vv):
+    """The GPU filters hold ALF coefficients in 16-bit lanes and multiply
+    CC-ALF coefficients in 24 bits; values beyond those (no conformant stream
+    has them) are refused instead of being filtered differently, and the
+    largest values inside are accepted."""
+    def frame():
+        f = vv.SynthFrame(1, 416, 240, seed=3)
+        return f, f.desc.contents.alf_binding
+    for field, bad, good in [("luma", 32768, 32767), ("luma", -32769, -32768), ("chroma", 40000, -32768),
+                             ("cc", 1 << 23, (1 << 23) - 1), ("cc", -(1 << 23) - 1, -(1 << 23))]:
+        for value, want in [(bad, vv.ERR_UNSUPPORTED), (good, 0)]:
+            f, b = frame()
+            if field == "luma":
+                b.luma_apses[0].contents.luma_coeff[3][5] = value
+            elif field == "chroma":
+                assert b.chroma_aps and b.chroma_aps.contents.n_chroma_coeff > 0
+                b.chroma_aps.contents.chroma_coeff[0][2] = value
+            else:
+                assert b.cc_cb_aps and b.cc_cb_aps.contents.n_cc_cb_coeff > 0
+                b.cc_cb_aps.contents.cc_cb_coeff[0][4] = value
+            rc = vv.lib.vvclf_frame_check(f.desc, None)
+            assert rc == want, (field, value, rc, vv.lib.vvclf_last_error())
+            if want:
+                assert b"bits" in vv.lib.vvclf_last_error()
+
+
 def test_derive_lmcs_matches_oracle(vv, orc):
     rng = np.random.default_rng(0)
     for trial in range(300):
