@@ -126,6 +126,7 @@ def batched_run(vv, ctx, jobs, reps, warmup=2):
             "hbm_algorithmic_GBs": round(CHAIN_BYTES_PER_PX * px / (ms.value / 1e3) / 1e9, 2)}
 
 
+SERVE_STAGES = ["admission", "host_tables_and_upload", "wait_for_launch", "device", "download"]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 VALU_FILE = os.path.join(ROOT, "profiles", "valu_instr.json")
 # Issue ceiling of vector ALU work: 256 CUs x 2.4 GHz x one wave64 VALU
@@ -212,9 +213,8 @@ def served_run(vv, ctx, seconds, rate, n_streams=32, seed=7):
             "throughput_Mpixel_s": round(st.luma_pixels / st.wall_s / 1e6, 2) if st.wall_s else None,
             "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3), "max_ms": round(st.max_ms, 3),
             "admitted_bytes_peak": int(st.bytes_admitted_peak), "hbm_budget_bytes": int(cfg.hbm_budget_bytes),
-            "stage_p50_ms": {k: round(st.stage_p50_ms[i], 3) for i, k in
-                             enumerate(["admission", "host_tables_and_upload", "wait_for_launch", "device",
-                                        "download"])},
+            "stage_p50_ms": {k: round(st.stage_p50_ms[i], 3) for i, k in enumerate(SERVE_STAGES)},
+            "stage_p99_ms": {k: round(st.stage_p99_ms[i], 3) for i, k in enumerate(SERVE_STAGES)},
             "latency": "submit -> filtered samples back in host memory (includes host table derivation, "
                        "H2D and D2H copies)"}
 

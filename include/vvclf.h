@@ -440,6 +440,7 @@ typedef struct vvclf_server_stats {
    * [1] host table derivation + upload, [2] waiting for a launch, [3] on the
    * device, [4] download into the caller's planes */
   double stage_p50_ms[5];
+  double stage_p99_ms[5];      /* the same stages at the 99th percentile */
 } vvclf_server_stats;
 
 /* A server owns three HIP streams (upload, compute, download), a pool of

@@ -175,7 +175,7 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("failed", u64), ("rejected", u64), ("luma_pixels", u64),
                 ("launches", u64), ("bytes_admitted_peak", u64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("max_ms", C.c_double), ("mean_ms", C.c_double), ("wall_s", C.c_double),
-                ("stage_p50_ms", C.c_double * 5)]
+                ("stage_p50_ms", C.c_double * 5), ("stage_p99_ms", C.c_double * 5)]
 
 
 class SynthSpec(C.Structure):

@@ -68,7 +68,7 @@ namespace vvclf {
 
 // Host threads for the deblocking-descriptor walk of calls that handle one
 // picture at a time (the serving prep threads use one each).
-static int drop_in_threads() {
+int host_walk_threads() {
   static const int n = (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
   return n;
 }
@@ -472,7 +472,7 @@ int vvclf_job_create_flags(vvclf_ctx* ctx, const vvclf_frame* frame, uint32_t fl
   if (flags & VVCLF_JOB_GPU_DERIVE)
     return fail(VVCLF_ERR_UNSUPPORTED, "GPU derivation of the deblocking descriptors is switched off");
   job_opts o;
-  o.host_threads = drop_in_threads();
+  o.host_threads = host_walk_threads();
   return job_create_stages(ctx, frame, STAGE_ALL, out, nullptr, &o);
 }
 
@@ -504,7 +504,7 @@ int vvclf_job_deblock_table(const vvclf_job* job, uint32_t which, void* out, siz
 int vvclf_frame_check(const vvclf_frame* frame, uint64_t stats[8]) {
   uint64_t local[8];
   job_opts o;
-  o.host_threads = drop_in_threads();
+  o.host_threads = host_walk_threads();
   return job_create_stages(nullptr, frame, STAGE_ALL, nullptr, stats ? stats : local, &o);
 }
 
@@ -696,7 +696,7 @@ static int run_sync(vvclf_frame* f, uint32_t stages) {
   job_opts o;
   o.stream = ctx->stream;
   o.stage = &stage;
-  o.host_threads = drop_in_threads();
+  o.host_threads = host_walk_threads();
   vvclf_job* j = nullptr;
   int rc = job_create_stages(ctx, f, stages, &j, nullptr, &o);
   if (rc) return rc;

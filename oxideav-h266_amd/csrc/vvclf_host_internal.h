@@ -47,6 +47,9 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
                         int threads = 1);
 int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t, uint32_t stages);
 void build_lmcs_lut(const vvclf_lmcs_binding& b, std::vector<uint16_t>& lut);
+// Threads of the host CU walk (up to 8; only pictures of >= 4096 CU records
+// use more than one).
+int host_walk_threads();
 void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t);
 
 static constexpr int PLAIN_STAGES = 6;  // prep, deblock V, deblock H, SAO, ALF luma, ALF chroma

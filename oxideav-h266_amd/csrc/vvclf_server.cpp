@@ -280,6 +280,7 @@ void prep_loop(vvclf_server* s, int idx) {
       o.stream = ps;
       o.res = &r->res->t;
       o.stage = &s->prep_stage[idx];
+      o.host_threads = host_walk_threads();  // large pictures walk their CUs on several threads
       rc = job_create_stages(s->ctx, r->frame, STAGE_ALL, &r->job, nullptr, &o);
     }
     if (!rc && !job_fits_shape(r->job->d, 32, 32)) {  // 32x32 fits every job any shape fits
@@ -652,8 +653,9 @@ int vvclf_server_get_stats(vvclf_server* s, vvclf_server_stats* o) {
   for (int k = 0; k < 5; k++) {
     std::vector<double> v = s->stage_ms[k];
     if (v.empty()) continue;
-    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+    std::sort(v.begin(), v.end());
     o->stage_p50_ms[k] = v[v.size() / 2];
+    o->stage_p99_ms[k] = v[std::min(v.size() - 1, (size_t)(0.99 * (v.size() - 1) + 0.5))];
   }
   return VVCLF_OK;
 }
