@@ -28,17 +28,46 @@ int check_plane(const vvclf_plane& p, const char* name) {
   return VVCLF_OK;
 }
 
+// The side tables of a job, laid out at 256-byte offsets and assembled in
+// one copy pass: into a pinned buffer when one is given (no page faults on a
+// fresh heap block, and the upload is a direct DMA), else on the heap.  The
+// added tables must outlive materialize().
 struct side_builder {
+  struct part {
+    const void* p;
+    size_t n, off;
+  };
+  std::vector<part> parts;
   std::vector<uint8_t> blob;
+  size_t size = 0;
   size_t add(const void* p, size_t n) {
-    size_t off = (blob.size() + 255) / 256 * 256;
-    blob.resize(off + std::max<size_t>(n, 4));
-    if (n) memcpy(blob.data() + off, p, n);
+    const size_t off = (size + 255) / 256 * 256;
+    parts.push_back({p, n, off});
+    size = off + std::max<size_t>(n, 4);
     return off;
   }
-  template <class T>
-  size_t add(const std::vector<T>& v) {
+  template <class T, class A>
+  size_t add(const std::vector<T, A>& v) {
     return add(v.data(), v.size() * sizeof(T));
+  }
+  const uint8_t* materialize(pinned_buf* st) {
+    uint8_t* dst;
+    if (st) {
+      if (!st->ensure(size)) return nullptr;
+      dst = (uint8_t*)st->p;
+    } else {
+      blob.resize(size);
+      dst = blob.data();
+    }
+    // tables, and zeros in the gaps and in the 4 bytes an empty table reserves
+    size_t at = 0;
+    for (const part& q : parts) {
+      if (q.off > at) memset(dst + at, 0, q.off - at);
+      if (q.n) memcpy(dst + q.off, q.p, q.n);
+      at = q.off + q.n;
+    }
+    if (size > at) memset(dst + at, 0, size - at);
+    return dst;
   }
 };
 
@@ -221,7 +250,12 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   size_t o_sao = sb.add(t.sao), o_saor = sb.add(t.sao_rect);
   size_t o_actb = sb.add(t.alf_ctb), o_arect = sb.add(t.alf_rect), o_bank = sb.add(t.alf_bank);
   size_t o_cbank = sb.add(t.alf_cbank), o_ccbank = sb.add(t.alf_ccbank);
-  const size_t host_side = sb.blob.size();
+  const size_t host_side = sb.size;
+  const uint8_t* host_blob = sb.materialize(opts ? opts->side_stage : nullptr);
+  if (!host_blob) {
+    vvclf_job_destroy(j);
+    return fail(VVCLF_ERR_NOMEM, "pinned staging of the side tables (%zu bytes)", host_side);
+  }
   // GPU derivation: maps and lists live after the uploaded tables
   size_t tail = (host_side + 255) / 256 * 256;
   auto reserve = [&](size_t n) {
@@ -255,8 +289,8 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
     vvclf_job_destroy(j);
     return fail(VVCLF_ERR_NOMEM, "hipMalloc side tables: %s", hipGetErrorString(e));
   }
-  e = ps ? hipMemcpyAsync(j->dev_side, sb.blob.data(), host_side, hipMemcpyHostToDevice, ps)
-         : hipMemcpy(j->dev_side, sb.blob.data(), host_side, hipMemcpyHostToDevice);
+  e = ps ? hipMemcpyAsync(j->dev_side, host_blob, host_side, hipMemcpyHostToDevice, ps)
+         : hipMemcpy(j->dev_side, host_blob, host_side, hipMemcpyHostToDevice);
   if (e == hipSuccess && gpu_dbk)
     e = hipMemsetAsync((uint8_t*)j->dev_side + maps_begin, 0, maps_end - maps_begin, ps);
   if (e == hipSuccess && ps) e = hipStreamSynchronize(ps);
@@ -360,6 +394,39 @@ namespace {
 size_t set_samples(const vvclf_job* j) {
   return (size_t)j->d.py * j->h + 2 * (size_t)j->d.pc * j->hc;
 }
+
+// Pitched row copies of the three planes, split over up to
+// host_walk_threads() threads when they move at least 4 MB (a 4K picture's
+// 25 MB take several ms on one core).
+struct plane_copy {
+  uint8_t* dst;
+  size_t dpitch;
+  const uint8_t* src;
+  size_t spitch, rows, bytes;
+};
+void copy_planes(const plane_copy* pc, int n) {
+  size_t total = 0, rows = 0;
+  for (int i = 0; i < n; i++) {
+    total += pc[i].rows * pc[i].bytes;
+    rows += pc[i].rows;
+  }
+  const int nt = total >= ((size_t)4 << 20) ? host_walk_threads() : 1;
+  auto part = [&](int ti) {
+    // rows b..e of the concatenated planes
+    const size_t b = rows * ti / nt, e = rows * (ti + 1) / nt;
+    size_t base = 0;
+    for (int i = 0; i < n; i++) {
+      const size_t lo = std::max(b, base), hi = std::min(e, base + pc[i].rows);
+      for (size_t r = lo; r < hi; r++)
+        memcpy(pc[i].dst + (r - base) * pc[i].dpitch, pc[i].src + (r - base) * pc[i].spitch, pc[i].bytes);
+      base += pc[i].rows;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int ti = 1; ti < nt; ti++) pool.emplace_back(part, ti);
+  part(0);
+  for (auto& th : pool) th.join();
+}
 }  // namespace
 
 hipError_t upload_staged(vvclf_job* j, const vvclf_picture& pic, pinned_buf& st, hipStream_t s) {
@@ -369,15 +436,16 @@ hipError_t upload_staged(vvclf_job* j, const vvclf_picture& pic, pinned_buf& st,
   const size_t n = set_samples(j);
   if (!st.ensure(n * 2)) return hipErrorOutOfMemory;
   uint16_t* h = (uint16_t*)st.p;
-  for (uint32_t y = 0; y < j->h; y++)
-    memcpy(h + (size_t)y * j->d.py, pic.luma.samples + (size_t)y * pic.luma.stride, (size_t)j->w * 2);
   uint16_t* hc = h + (size_t)j->d.py * j->h;
-  for (int c = 0; c < 2 && j->wc; c++) {
+  plane_copy pc[3] = {{(uint8_t*)h, (size_t)j->d.py * 2, (const uint8_t*)pic.luma.samples, (size_t)pic.luma.stride * 2,
+                       j->h, (size_t)j->w * 2}};
+  int np = 1;
+  for (int c = 0; c < 2 && j->wc; c++, np++) {
     const vvclf_plane& pl = c ? pic.cr : pic.cb;
-    uint16_t* dst = hc + (size_t)c * j->d.pc * j->hc;
-    for (uint32_t y = 0; y < j->hc; y++)
-      memcpy(dst + (size_t)y * j->d.pc, pl.samples + (size_t)y * pl.stride, (size_t)j->wc * 2);
+    pc[np] = {(uint8_t*)(hc + (size_t)c * j->d.pc * j->hc), (size_t)j->d.pc * 2, (const uint8_t*)pl.samples,
+              (size_t)pl.stride * 2, j->hc, (size_t)j->wc * 2};
   }
+  copy_planes(pc, np);
   hipError_t e = hipMemcpyAsync(j->d.in.y, h, n * 2, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e;
@@ -391,15 +459,16 @@ hipError_t download_staged_issue(vvclf_job* j, pinned_buf& st, hipStream_t s) {
 
 void download_staged_finish(const vvclf_job* j, const pinned_buf& st, vvclf_picture& pic) {
   const uint16_t* h = (const uint16_t*)st.p;
-  for (uint32_t y = 0; y < j->h; y++)
-    memcpy(pic.luma.samples + (size_t)y * pic.luma.stride, h + (size_t)y * j->d.py, (size_t)j->w * 2);
   const uint16_t* hc = h + (size_t)j->d.py * j->h;
-  for (int c = 0; c < 2 && j->wc; c++) {
+  plane_copy pc[3] = {{(uint8_t*)pic.luma.samples, (size_t)pic.luma.stride * 2, (const uint8_t*)h, (size_t)j->d.py * 2,
+                       j->h, (size_t)j->w * 2}};
+  int np = 1;
+  for (int c = 0; c < 2 && j->wc; c++, np++) {
     vvclf_plane& pl = c ? pic.cr : pic.cb;
-    const uint16_t* src = hc + (size_t)c * j->d.pc * j->hc;
-    for (uint32_t y = 0; y < j->hc; y++)
-      memcpy(pl.samples + (size_t)y * pl.stride, src + (size_t)y * j->d.pc, (size_t)j->wc * 2);
+    pc[np] = {(uint8_t*)pl.samples, (size_t)pl.stride * 2, (const uint8_t*)(hc + (size_t)c * j->d.pc * j->hc),
+              (size_t)j->d.pc * 2, j->hc, (size_t)j->wc * 2};
   }
+  copy_planes(pc, np);
 }
 
 const uint16_t* job_final(const vvclf_job* j, int comp) {
@@ -692,10 +761,11 @@ uint64_t vvclf_job_luma_pixels(const vvclf_job* j) { return j ? (uint64_t)j->w *
 static int run_sync(vvclf_frame* f, uint32_t stages) {
   vvclf_ctx* ctx = default_ctx();
   if (!ctx) return VVCLF_ERR_HIP;
-  static thread_local pinned_buf stage;
+  static thread_local pinned_buf stage, side;
   job_opts o;
   o.stream = ctx->stream;
   o.stage = &stage;
+  o.side_stage = &side;
   o.host_threads = host_walk_threads();
   vvclf_job* j = nullptr;
   int rc = job_create_stages(ctx, f, stages, &j, nullptr, &o);

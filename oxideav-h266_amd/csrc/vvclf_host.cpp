@@ -2,10 +2,12 @@
 // host-side derivation of the compact per-edge / per-CTB tables the kernels
 // read, job lifetime, and the synchronous drop-in entry points.
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
 
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <memory>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -46,69 +48,143 @@ int hip_fail(hipError_t e, const char* what) {
 // ---------------------------------------------------------------------------
 namespace {
 
+constexpr size_t kMapThreshold = (size_t)1 << 20;
+}  // namespace
+
+void* zeroed_alloc_bytes(size_t bytes) {
+  if (bytes < kMapThreshold) return std::calloc(bytes ? bytes : 1, 1);
+  void* p = mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_POPULATE, -1, 0);
+  return p == MAP_FAILED ? nullptr : p;
+}
+
+void zeroed_free_bytes(void* p, size_t bytes) {
+  if (!p) return;
+  if (bytes < kMapThreshold) std::free(p);
+  else munmap(p, bytes);
+}
+
+namespace {
+
+// Runs fn(0 .. nt-1), fn(0) on the calling thread.
+template <class F>
+void run_threads(int nt, F&& fn) {
+  std::vector<std::thread> pool;
+  for (int ti = 1; ti < nt; ti++) pool.emplace_back(fn, ti);
+  fn(0);
+  for (auto& th : pool) th.join();
+}
+
 struct cu_grid {
-  std::vector<int32_t> cells;
+  std::unique_ptr<int32_t[]> cells;
   uint32_t w = 0, h = 0;
-  // CuGrid::build, src/deblock.rs:637-660: later records overwrite earlier ones.
-  void build(uint32_t pic_w, uint32_t pic_h, const vvclf_deblock_cu* cus, uint32_t n) {
+  // CuGrid::build, src/deblock.rs:637-660: later records overwrite earlier
+  // ones.  Row bands on nt threads, each visiting the records in order, so
+  // that every cell still ends with the last record covering it.
+  void build(uint32_t pic_w, uint32_t pic_h, const vvclf_deblock_cu* cus, uint32_t n, int nt) {
     w = (pic_w + 3) / 4;
     h = (pic_h + 3) / 4;
-    cells.assign((size_t)w * h, -1);
-    for (uint32_t i = 0; i < n; i++) {
-      const vvclf_deblock_cu& c = cus[i];
-      uint32_t x0 = c.x / 4, y0 = c.y / 4;
-      uint32_t x1 = std::min((c.x + c.w + 3) / 4, w), y1 = std::min((c.y + c.h + 3) / 4, h);
-      for (uint32_t cy = y0; cy < y1; cy++)
-        for (uint32_t cx = x0; cx < x1; cx++) cells[(size_t)cy * w + cx] = (int32_t)i;
-    }
+    cells.reset(new int32_t[(size_t)w * h]);
+    run_threads(nt, [&](int ti) {
+      const uint32_t yb = (uint32_t)((uint64_t)h * ti / nt), ye = (uint32_t)((uint64_t)h * (ti + 1) / nt);
+      std::fill(cells.get() + (size_t)yb * w, cells.get() + (size_t)ye * w, -1);
+      for (uint32_t i = 0; i < n; i++) {
+        const vvclf_deblock_cu& c = cus[i];
+        const uint32_t y0 = std::max(c.y / 4, yb), y1 = std::min(std::min((c.y + c.h + 3) / 4, h), ye);
+        if (y0 >= y1) continue;
+        const uint32_t x0 = c.x / 4, x1 = std::min((c.x + c.w + 3) / 4, w);
+        for (uint32_t cy = y0; cy < y1; cy++)
+          for (uint32_t cx = x0; cx < x1; cx++) cells[(size_t)cy * w + cx] = (int32_t)i;
+      }
+    });
   }
-  walk::grid_view view() const { return walk::grid_view{cells.data(), w, h}; }
+  walk::grid_view view() const { return walk::grid_view{cells.get(), w, h}; }
 };
 
 // One direction's descriptors of one plane (claim marks stripped by
-// finish_map).
+// finish_rows).  A fresh map holds zeros without having written them.
 struct edge_map {
-  std::vector<uint32_t> e;
+  map_vec e;
   uint32_t w = 0, h = 0;
   void init(uint32_t w_, uint32_t h_) {
     w = w_;
     h = h_;
-    e.assign((size_t)w * h, 0);
+    e = map_vec();
+    e.resize((size_t)w * h);
   }
   walk::map_view view() { return walk::map_view{e.data(), w, h}; }
 };
 
-// Strips the claim marks of a map, lists its nonzero descriptors as
-// tag | row << 16 | column (row-major), and for a luma map (luma_dir 1
-// vertical, 2 horizontal) checks every pair of consecutive same-direction
-// edges on a line with luma_pair_safe.  Returns false if a pair interacts.
-bool finish_map(std::vector<uint32_t>& e, uint32_t w, uint32_t h, uint32_t tag, int luma_dir,
-                std::vector<uint32_t>& out) {
-  out.clear();
+// Rows [y0, y1) of a map: strips the claim marks, lists the nonzero
+// descriptors as tag | row << 16 | column (row-major), and for a luma map
+// checks that consecutive same-direction edges of a line cannot reach each
+// other's used samples (luma_dir 1: along a row, vertical edges; 2: down a
+// column, horizontal edges, whose first and last entry per column are kept
+// for the check across band borders).
+struct finish_band {
+  uint32_t y0 = 0, y1 = 0;
+  std::vector<uint32_t> list;
   bool safe = true;
-  std::vector<int32_t> prev_y;
-  std::vector<uint32_t> prev_e;
+  std::vector<int32_t> fy, ly;
+  std::vector<uint32_t> fe, le;
+};
+
+void finish_rows(uint32_t* e, uint32_t w, uint32_t tag, int luma_dir, finish_band& b) {
+  b.list.clear();
   if (luma_dir == 2) {
-    prev_y.assign(w, -1);
-    prev_e.assign(w, 0);
+    b.fy.assign(w, -1);
+    b.ly.assign(w, -1);
+    b.fe.assign(w, 0);
+    b.le.assign(w, 0);
   }
-  for (uint32_t y = 0; y < h; y++) {
-    uint32_t* row = e.data() + (size_t)y * w;
+  for (uint32_t y = b.y0; y < b.y1; y++) {
+    uint32_t* row = e + (size_t)y * w;
     int32_t px = -1;
     uint32_t pe = 0;
     for (uint32_t x = 0; x < w; x++) {
       const uint32_t v = row[x] & ~walk::SLOT_CLAIMED;
       row[x] = v;
       if (!v) continue;
-      out.push_back(tag | (y << 16) | x);
+      b.list.push_back(tag | (y << 16) | x);
       if (luma_dir == 1) {
-        if (px >= 0 && !walk::luma_pair_safe(pe, v, 4 * (x - (uint32_t)px), true)) safe = false;
+        if (px >= 0 && !walk::luma_pair_safe(pe, v, 4 * (x - (uint32_t)px), true)) b.safe = false;
         px = (int32_t)x;
         pe = v;
       } else if (luma_dir == 2) {
-        if (prev_y[x] >= 0 && !walk::luma_pair_safe(prev_e[x], v, 4 * (y - (uint32_t)prev_y[x]), false)) safe = false;
-        prev_y[x] = (int32_t)y;
-        prev_e[x] = v;
+        if (b.ly[x] >= 0) {
+          if (!walk::luma_pair_safe(b.le[x], v, 4 * (y - (uint32_t)b.ly[x]), false)) b.safe = false;
+        } else {
+          b.fy[x] = (int32_t)y;
+          b.fe[x] = v;
+        }
+        b.ly[x] = (int32_t)y;
+        b.le[x] = v;
+      }
+    }
+  }
+}
+
+// Joins the bands of one map: lists in row order, and the horizontal-edge
+// check across band borders (each column's last entry above a band against
+// its first entry in it).
+bool join_bands(std::vector<finish_band>& bands, uint32_t w, int luma_dir, std::vector<uint32_t>& out) {
+  bool safe = true;
+  size_t n = 0;
+  for (const auto& b : bands) {
+    safe &= b.safe;
+    n += b.list.size();
+  }
+  out.clear();
+  out.reserve(n);
+  for (const auto& b : bands) out.insert(out.end(), b.list.begin(), b.list.end());
+  if (luma_dir == 2 && bands.size() > 1) {
+    for (uint32_t x = 0; x < w; x++) {
+      int32_t cy = -1;
+      uint32_t ce = 0;
+      for (const auto& b : bands) {
+        if (b.fy[x] < 0) continue;
+        if (cy >= 0 && !walk::luma_pair_safe(ce, b.fe[x], 4 * (uint32_t)(b.fy[x] - cy), false)) safe = false;
+        cy = b.ly[x];
+        ce = b.le[x];
       }
     }
   }
@@ -121,8 +197,10 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
                         int threads) {
   const vvclf_deblock_params& prm = f.deblock;
   uint32_t ctb = 1u << prm.ctb_log2_size_y;
+  // pictures of at least 4096 CU records use up to 16 threads
+  const int nt = f.n_cus >= 4096 ? std::max(1, std::min(threads, 16)) : 1;
   cu_grid grid;
-  grid.build(W, H, f.cus, f.n_cus);
+  grid.build(W, H, f.cus, f.n_cus, nt);
   for (uint32_t i = 0; i < f.n_cus; i++) {
     const vvclf_deblock_cu& c = f.cus[i];
     if ((c.x | c.y | c.w | c.h) & 3) return fail(VVCLF_ERR_UNSUPPORTED, "deblock: CU %u not on the 4x4 grid", i);
@@ -154,7 +232,7 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
     uint32_t nc = f.has_chroma_cus ? f.n_chroma_cus : f.n_cus;
     const cu_grid* cg = &grid;
     if (f.has_chroma_cus) {
-      cgrid.build(W, H, ccus, nc);
+      cgrid.build(W, H, ccus, nc, nt);
       cg = &cgrid;
       for (uint32_t i = 0; i < nc; i++) {
         const vvclf_deblock_cu& c = ccus[i];
@@ -182,7 +260,6 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
     t.has_chroma_dbk = true;
   }
   // Every thread walks its share of the records through all passes.
-  const int nt = f.n_cus >= 4096 ? std::max(1, std::min(threads, 16)) : 1;
   std::vector<std::array<uint64_t, 4>> hists((size_t)nt, std::array<uint64_t, 4>{0, 0, 0, 0});
   std::atomic<bool> bad{false};
   int32_t bad_xy[2] = {0, 0};
@@ -217,25 +294,49 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
   uint64_t hist[4] = {0, 0, 0, 0};
   for (const auto& h : hists)
     for (int i = 0; i < 4; i++) hist[i] += h[i];
-  // finish the maps (one thread each): claim marks off, work lists, and the
-  // check that lets a direction pass run concurrently in place
+  // finish the maps in row bands (nt per map, all bands on nt threads):
+  // claim marks off, work lists, and the check that lets a direction pass
+  // run concurrently in place
   std::vector<uint32_t> lists[6];
   bool ok[6] = {true, true, true, true, true, true};
   {
-    std::vector<std::thread> fin;
-    auto go = [&](int k, edge_map& m, uint32_t tag, int dir) {
-      auto fn = [&, k, tag, dir]() { ok[k] = finish_map(m.e, m.w, m.h, tag, dir, lists[k]); };
-      if (nt > 1) fin.emplace_back(fn);
-      else fn();
+    struct map_job {
+      edge_map* m;
+      uint32_t tag;
+      int dir;
+      std::vector<finish_band> bands;
     };
-    go(0, lv, 0, 1);
-    go(1, lh, 0, 2);
+    std::vector<map_job> maps;
+    maps.push_back({&lv, 0, 1, {}});
+    maps.push_back({&lh, 0, 2, {}});
     if (t.has_chroma_dbk)
       for (int c = 0; c < 2; c++) {
-        go(2 + c, cv[c], (uint32_t)c << 31, 0);
-        go(4 + c, ch[c], (uint32_t)c << 31, 0);
+        maps.push_back({&cv[c], (uint32_t)c << 31, 0, {}});
+        maps.push_back({&ch[c], (uint32_t)c << 31, 0, {}});
       }
-    for (auto& th : fin) th.join();
+    std::vector<std::pair<int, int>> tasks;
+    for (int k = 0; k < (int)maps.size(); k++) {
+      const uint32_t h = maps[k].m->h;
+      const int nb = (int)std::max(1u, std::min((uint32_t)nt, h / 16));
+      maps[k].bands.resize(nb);
+      for (int bi = 0; bi < nb; bi++) {
+        maps[k].bands[bi].y0 = (uint32_t)((uint64_t)h * bi / nb);
+        maps[k].bands[bi].y1 = (uint32_t)((uint64_t)h * (bi + 1) / nb);
+        tasks.push_back({k, bi});
+      }
+    }
+    std::atomic<size_t> next{0};
+    run_threads(nt, [&](int) {
+      for (size_t i; (i = next.fetch_add(1)) < tasks.size();) {
+        map_job& mj = maps[tasks[i].first];
+        finish_rows(mj.m->e.data(), mj.m->w, mj.tag, mj.dir, mj.bands[tasks[i].second]);
+      }
+    });
+    // lists in the order the kernels expect: luma V, luma H, Cb then Cr V,
+    // Cb then Cr H
+    const int slot[6] = {0, 1, 2, 4, 3, 5};
+    for (int k = 0; k < (int)maps.size(); k++)
+      ok[slot[k]] = join_bands(maps[k].bands, maps[k].m->w, maps[k].dir, lists[slot[k]]);
   }
   if (!ok[0] || !ok[1])
     return fail(VVCLF_ERR_UNSUPPORTED,

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 
+#include <cstdlib>
+#include <new>
+#include <utility>
 #include <vector>
 
 #include "../../include/vvclf.h"
@@ -14,10 +17,42 @@ namespace vvclf {
 enum : uint32_t { STAGE_LMCS = 1, STAGE_DBK = 2, STAGE_SAO = 4, STAGE_ALF = 8, STAGE_ALL = 15 };
 
 // Host copies of the derived tables, before upload.
+// Zero-filled storage that does not write its zeros: blocks of 1 MB and more
+// are anonymous mappings populated by the kernel in one call (zero pages
+// without a fault per page), smaller ones come from calloc; elements are
+// default-initialised.
+void* zeroed_alloc_bytes(size_t bytes);
+void zeroed_free_bytes(void* p, size_t bytes);
+template <class T>
+struct zeroed_alloc {
+  using value_type = T;
+  zeroed_alloc() = default;
+  template <class U>
+  zeroed_alloc(const zeroed_alloc<U>&) {}
+  T* allocate(size_t n) {
+    void* p = zeroed_alloc_bytes(n * sizeof(T));
+    if (!p) throw std::bad_alloc();
+    return static_cast<T*>(p);
+  }
+  void deallocate(T* p, size_t n) { zeroed_free_bytes(p, n * sizeof(T)); }
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+  bool operator==(const zeroed_alloc&) const { return true; }
+  bool operator!=(const zeroed_alloc&) const { return false; }
+};
+// A deblocking descriptor map; a fresh one resized to n holds n zeros.
+using map_vec = std::vector<uint32_t, zeroed_alloc<uint32_t>>;
+
 struct host_tables {
   // deblock
   uint32_t gw = 0, gh = 0, cvw = 0, cvh = 0, chw = 0, chh = 0;
-  std::vector<uint32_t> e_lv, e_lh, e_cv[2], e_ch[2];
+  map_vec e_lv, e_lh, e_cv[2], e_ch[2];
   // nonzero descriptors: luma gy << 16 | gx; chroma comp << 31 | sy << 16 | sx
   std::vector<uint32_t> l_lv, l_lh, l_cv, l_ch;
   bool has_chroma_dbk = false;
@@ -74,6 +109,7 @@ struct job_opts {
   hipStream_t stream = nullptr;  // stream-ordered allocation and copies (memory pool)
   const resident_tables* res = nullptr;
   pinned_buf* stage = nullptr;   // upload through this pinned buffer
+  pinned_buf* side_stage = nullptr;  // side tables assembled in this pinned buffer (else on the heap)
   int host_threads = 1;          // threads deriving the deblocking descriptors (host walk)
   // derive the deblocking descriptors on the GPU (derive_deblock_gpu); no
   // caller sets it: the path is switched off until a device memory fault it

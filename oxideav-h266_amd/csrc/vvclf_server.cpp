@@ -138,7 +138,7 @@ struct vvclf_server {
   std::vector<hipStream_t> prep_streams;
   // pinned staging: one buffer per prep thread (uploads), two for the
   // retire thread (downloads, double-buffered)
-  std::vector<pinned_buf> prep_stage;
+  std::vector<pinned_buf> prep_stage, prep_side;
   pinned_buf down_stage[2];
   hipEvent_t down_done[2] = {nullptr, nullptr};
   hipStream_t s_comp = nullptr, s_down = nullptr;
@@ -280,6 +280,7 @@ void prep_loop(vvclf_server* s, int idx) {
       o.stream = ps;
       o.res = &r->res->t;
       o.stage = &s->prep_stage[idx];
+      o.side_stage = &s->prep_side[idx];
       o.host_threads = host_walk_threads();  // large pictures walk their CUs on several threads
       rc = job_create_stages(s->ctx, r->frame, STAGE_ALL, &r->job, nullptr, &o);
     }
@@ -512,6 +513,7 @@ int vvclf_server_create(vvclf_ctx* ctx, const vvclf_server_config* cfg, vvclf_se
   for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&s->down_done[i], hipEventDisableTiming);
   s->prep_streams.resize(s->cfg.prep_threads);
   s->prep_stage.resize(s->cfg.prep_threads);
+  s->prep_side.resize(s->cfg.prep_threads);
   for (auto& ps : s->prep_streams)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ps, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -547,6 +549,7 @@ int vvclf_server_destroy(vvclf_server* s) {
   }
   hipDeviceSynchronize();
   for (auto& b : s->prep_stage) b.release();
+  for (auto& b : s->prep_side) b.release();
   for (int k = 0; k < 2; k++) {
     s->down_stage[k].release();
     if (s->down_done[k]) hipEventDestroy(s->down_done[k]);

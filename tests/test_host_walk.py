@@ -1,0 +1,29 @@
+"""CPU check that the multi-threaded host CU walk derives exactly the
+deblocking descriptor maps, work lists and bS histogram of the one-thread
+walk (tests/host_walk_check.cpp, compiled here from the package's host
+sources).  The one-thread walk itself is pinned against the oracle by the
+GPU parity tests."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "oxideav-h266_amd", "csrc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or not os.path.isdir(os.path.join(ROCM, "include", "hip")),
+                    reason="needs g++ and the HIP headers")
+def test_threaded_walk_matches_one_thread(tmp_path):
+    exe = tmp_path / "host_walk_check"
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{CSRC}",
+           os.path.join(ROOT, "tests", "host_walk_check.cpp"), os.path.join(CSRC, "vvclf_host.cpp"),
+           os.path.join(CSRC, "synth.cpp"), f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-lpthread",
+           "-o", str(exe)]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("identical") == 9
