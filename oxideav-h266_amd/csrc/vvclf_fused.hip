@@ -62,6 +62,7 @@ struct fused_lds {
   uint64_t safe[9][3];  // per CTB and component: [x0, x1) x [y0, y1) whose 8-neighbourhoods lie
                         // inside the picture and the CTB's rectangle (u16 fields)
   uint32_t nlist[4];    // entries in llv, llh, lcv, lch
+  uint32_t wide;        // the tile's windows hold a sample of 2^15 or more
   uint16_t lut[1024];
   int16_t tct[68];  // the deblocking tC' table, copied once per workgroup
   static_assert(MLV_W <= 64 && MLH_W <= 64 && MCV_W <= 64 && MCH_W <= 64, "list column field");
@@ -657,6 +658,7 @@ struct tile_loads {
     // samples beyond the bit depth) are rewritten from the global table.
     const bool lut_lds = j.bd <= 10;
     bool far = false;
+    uint32_t any = 0;  // OR of the samples written (the ALF's 16-bit lanes need them below 2^15)
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
@@ -667,6 +669,7 @@ struct tile_loads {
           if (lut_lds)
             w = make_uint4(lut_pair(s.lut, w.x), lut_pair(s.lut, w.y), lut_pair(s.lut, w.z), lut_pair(s.lut, w.w));
         }
+        any |= w.x | w.y | w.z | w.w;
         *reinterpret_cast<uint4*>(&s.L[yy * LD::LW + 8 * ch]) = w;
       }
     }
@@ -691,9 +694,15 @@ struct tile_loads {
 #pragma unroll
         for (int k = 0; k < PERC; k++) {
           const int i = tid + k * NT, yy = i / NC4, ch = i - yy * NC4;
-          if (i < per_plane && ch < nchc) *reinterpret_cast<uint2*>(&s.Cc[c][yy * LD::CW + 4 * ch]) = vc[c][k];
+          if (i < per_plane && ch < nchc) {
+            *reinterpret_cast<uint2*>(&s.Cc[c][yy * LD::CW + 4 * ch]) = vc[c][k];
+            any |= vc[c][k].x | vc[c][k].y;
+          }
         }
     }
+    // LMCS outputs and the deblocking / SAO outputs stay within the bit
+    // depth, so a sample of 2^15 or more reaching the ALF is one written here
+    if (any & 0x80008000u) s.wide = 1u;
     if (g.dbk) {
       slv.commit(s.m.lv, g.mlv, s.m.llv, &s.nlist[0], 0);
       slh.commit(s.m.lh, g.mlh, s.m.llh, &s.nlist[1], 0);
@@ -766,6 +775,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   const uint64_t r = tr.alf_rect;
   ld.commit(j, g, tr, s);
   __syncthreads();
+  const bool wide = !HBD && s.wide != 0u;  // read before the store phase clears it
   // ---- deblocking: all vertical edges, then all horizontal edges
   if (dbk) {
     lds_deblock_luma<true, LD::LW>(j, L, s.m.lv, LD::MLV_W, mlv, s.m.llv, (int)s.nlist[0], maxv, s.tct);
@@ -843,8 +853,12 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       const int bx = b % BW, x4 = x0 + 4 * bx, y4 = y0 + 4 * (b / BW);
       if (bx >= bw) continue;
       uint32_t outp[4][2];
-      alf_luma_block_lds<LD::SW, HBD>(s.o.S, sox, soy, x4, y4, s.bank, bank_idx != (int)ALF_OFF8, y_ctb, ctb, carve,
-                                      j.alf_bd, outp);
+      if (!HBD && wide)  // 16-bit lanes could wrap: the 32-bit form
+        alf_luma_block_lds<LD::SW, true>(s.o.S, sox, soy, x4, y4, s.bank, bank_idx != (int)ALF_OFF8, y_ctb, ctb,
+                                         carve, j.alf_bd, outp);
+      else
+        alf_luma_block_lds<LD::SW, HBD>(s.o.S, sox, soy, x4, y4, s.bank, bank_idx != (int)ALF_OFF8, y_ctb, ctb,
+                                        carve, j.alf_bd, outp);
 #pragma unroll
       for (int rr = 0; rr < 4; rr++)
         *reinterpret_cast<uint2*>(&ob[(y4 - y0 + rr) * TW + (x4 - x0)]) = make_uint2(outp[rr][0], outp[rr][1]);
@@ -881,7 +895,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       uint32_t outp = P2(x, y);
       if (in_grid) {
         const int alt = (int)((cw >> (c == 0 ? 8 : 16)) & 0xff);
-        if (alt != (int)ALF_OFF8 && !HBD) {
+        if (alt != (int)ALF_OFF8 && !HBD && !wide) {
           const uint32_t* f = s.cbank + alt * 6;
           int sft, d1, d2;
           alf_t46(y - ry * ctbc, ctbc, sft, d1, d2);
@@ -956,6 +970,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
   // tile's loads, so that waiting for those loads does not wait for these
   // stores (one in-order counter covers both)
   __syncthreads();
+  if (ltid() == 0) s.wide = 0u;  // for the next tile's commit (a barrier follows)
   {
     const int tid = ltid();
     auto put = [&](const uint16_t* src, uint16_t* dst, int xa, int xb) {
@@ -1007,6 +1022,7 @@ __global__ void __launch_bounds__((fused_lds<TW, TH>::NT), 3) k_fused(const dev_
   const uint32_t begin = xcd * chunk, end = min(n_tiles, begin + chunk);
   int cur_job = -1;
   if (ltid() < 4) s.nlist[ltid()] = 0;
+  if (ltid() == 0) s.wide = 0u;
   if (ltid() < TC_TAB_LEN) s.tct[ltid()] = kTc[ltid()];
   __syncthreads();
   // Software pipeline: the loads of tile t+1 are issued while tile t is

@@ -388,7 +388,29 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
         for (int k = 0; k < 6; k++) P[rr][k] = v[2 * k] | (v[2 * k + 1] << 16);
       }
     }
-    alf_block_pk(P, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outp);
+    // 16-bit lanes are exact for samples below 2^15; a window holding a
+    // larger sample (beyond the bit depth) takes the 32-bit form
+    uint32_t any = 0;
+#pragma unroll
+    for (int rr = 0; rr < 10; rr++)
+#pragma unroll
+      for (int k = 0; k < 6; k++) any |= P[rr][k];
+    if (any & 0x80008000u) {
+      int W[10][10];
+#pragma unroll
+      for (int rr = 0; rr < 10; rr++)
+#pragma unroll
+        for (int cc = 0; cc < 10; cc++) W[rr][cc] = (int)((P[rr][(cc + 1) >> 1] >> (16 * ((cc + 1) & 1))) & 0xffffu);
+      int outv[4][4];
+      alf_block(W, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outv);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        outp[rr][0] = (uint32_t)outv[rr][0] | ((uint32_t)outv[rr][1] << 16);
+        outp[rr][1] = (uint32_t)outv[rr][2] | ((uint32_t)outv[rr][3] << 16);
+      }
+    } else {
+      alf_block_pk(P, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outp);
+    }
   }
   if (x4 + 4 <= src.w) {
 #pragma unroll
@@ -434,6 +456,7 @@ __global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
       const uint32_t* f = j.alf_cbank + alt * 6;
       int sft, y1, y2;
       alf_t46(y - ry * ctbc, ctbc, sft, y1, y2);
+      bool done = false;
       if (j.alf_bd <= 12 && x - 2 >= cx0 && x + 3 <= cx1 && y - 2 >= cy0 && y + 2 <= cy1) {
         // neighbourhood inside the rectangle: packed pairs (see the fused form)
         auto P2 = [&](int dx, int dy) -> uint32_t {
@@ -443,23 +466,28 @@ __global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
         const uint32_t am2 = P2(-2, y1), a0 = P2(0, y1), ap2 = P2(2, y1);
         const uint32_t bm2 = P2(-2, -y1), b0 = P2(0, -y1), bp2 = P2(2, -y1);
         const uint32_t u2 = P2(0, y2), l2 = P2(0, -y2);
-        const s16x2 cur = as_s16x2(m0);
-        int a_0 = 1 << (sft - 1), a_1 = a_0;
-        auto tap = [&](int t, uint32_t a, uint32_t b) {
-          const uint32_t w = f[t], kk = w >> 16;
-          alf_pk_tap(w & 0xffffu, w << 16, kk | ((0u - kk) << 16), cur, a, b, a_0, a_1);
-        };
-        tap(0, u2, l2);
-        tap(1, __builtin_amdgcn_alignbit(ap2, a0, 16), __builtin_amdgcn_alignbit(b0, bm2, 16));
-        tap(2, a0, b0);
-        tap(3, __builtin_amdgcn_alignbit(a0, am2, 16), __builtin_amdgcn_alignbit(bp2, b0, 16));
-        tap(4, p2, m2);
-        tap(5, __builtin_amdgcn_alignbit(p2, m0, 16), __builtin_amdgcn_alignbit(m0, m2, 16));
-        const s16x2 d = __builtin_amdgcn_cvt_pk_i16(a_0 >> sft, a_1 >> sft);
-        const s16x2 zero2 = {0, 0}, vmax2 = {(short)maxv, (short)maxv};
-        outp = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(__builtin_elementwise_add_sat(cur, d), zero2),
-                                                 vmax2));
-      } else {
+        // 16-bit lanes are exact for samples below 2^15 (else the 32-bit form)
+        done = ((m2 | m0 | p2 | am2 | a0 | ap2 | bm2 | b0 | bp2 | u2 | l2) & 0x80008000u) == 0;
+        if (done) {
+          const s16x2 cur = as_s16x2(m0);
+          int a_0 = 1 << (sft - 1), a_1 = a_0;
+          auto tap = [&](int t, uint32_t a, uint32_t b) {
+            const uint32_t w = f[t], kk = w >> 16;
+            alf_pk_tap(w & 0xffffu, w << 16, kk | ((0u - kk) << 16), cur, a, b, a_0, a_1);
+          };
+          tap(0, u2, l2);
+          tap(1, __builtin_amdgcn_alignbit(ap2, a0, 16), __builtin_amdgcn_alignbit(b0, bm2, 16));
+          tap(2, a0, b0);
+          tap(3, __builtin_amdgcn_alignbit(a0, am2, 16), __builtin_amdgcn_alignbit(bp2, b0, 16));
+          tap(4, p2, m2);
+          tap(5, __builtin_amdgcn_alignbit(p2, m0, 16), __builtin_amdgcn_alignbit(m0, m2, 16));
+          const s16x2 d = __builtin_amdgcn_cvt_pk_i16(a_0 >> sft, a_1 >> sft);
+          const s16x2 zero2 = {0, 0}, vmax2 = {(short)maxv, (short)maxv};
+          outp = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(__builtin_elementwise_add_sat(cur, d), zero2),
+                                                   vmax2));
+        }
+      }
+      if (!done) {
         uint32_t o2[2];
         for (int h = 0; h < 2; h++) {
           const int xs = x + h;

@@ -120,3 +120,22 @@ def binding_of(vv, luma=(), chroma=None, cc_cb=None, cc_cr=None):
     if cc_cr is not None:
         b.cc_cr_aps = C.pointer(cc_cr)
     return b
+
+
+def put_wide_samples(vv, frames, seed, share=200):
+    """Sets about 1/share of every plane's samples, at the same places and to
+    the same values in each frame, to values of 2^15 and more (beyond every
+    bit depth; a decoder never produces them, the reference filters them all
+    the same) and to 4095."""
+    rng = np.random.default_rng(seed)
+    planes = [vv.picture_arrays(f.desc.contents.pic) for f in frames]
+    for p in range(3):
+        a0 = planes[0][p]
+        if a0.size == 0:
+            continue
+        n = max(1, a0.size // share)
+        ys = rng.integers(0, a0.shape[0], n)
+        xs = rng.integers(0, a0.shape[1], n)
+        v = rng.choice([32768, 40000, 65535, 4095], n).astype(np.uint16)
+        for pl in planes:
+            pl[p][ys, xs] = v

@@ -65,6 +65,39 @@ HBD_MIX = [(1, 416, 240, 21, {"bit_depth": 16}), (0, 256, 128, 22, {"bit_depth":
            (1, 208, 120, 23, {}), (3, 416, 240, 24, {"tiles": 1, "bit_depth": 16})]
 
 
+WIDE = [(1, 416, 240, 21, {}), (0, 256, 128, 22, {"rich": 1}), (1, 416, 240, 23, {"bit_depth": 12, "rich": 1}),
+        (1, 208, 120, 24, {"lmcs": 0})]
+
+
+@pytest.mark.parametrize("tw,th", [(128, 64), (64, 64)])
+def test_batch_samples_beyond_the_bit_depth(vv, orc, tw, th):
+    """Samples of 2^15 and more in every plane: tiles holding one filter in
+    32-bit lanes, and every picture still matches the oracle."""
+    from helpers import put_wide_samples
+    say(f"batch wide samples tile {tw}x{th}")
+    gpu, ref = [], []
+    for cfg, w, h, seed, kw in WIDE:
+        fg = vv.SynthFrame(cfg, w, h, seed=seed, **kw)
+        fo = vv.SynthFrame(cfg, w, h, seed=seed, **kw)
+        put_wide_samples(vv, [fg, fo], seed)
+        assert orc.oracle_apply_in_loop_filters(fo.desc) == 0
+        gpu.append(fg)
+        ref.append(fo)
+    ctx = vv.Context()
+    jobs = [vv.Job(ctx, f.desc) for f in gpu]
+    arr = (C.c_void_p * len(jobs))(*[j.h for j in jobs])
+    tune = vv.Tune(tw, th, 256, 0)
+    b = C.c_void_p()
+    vv.check(vv.lib.vvclf_batch_create(ctx.h, arr, len(jobs), C.byref(tune), C.byref(b)))
+    vv.check(vv.lib.vvclf_batch_run(b, ctx.stream))
+    ctx.synchronize()
+    check_all(vv, jobs, gpu, ref, f"batch wide samples {tw}x{th}", cases=WIDE)
+    vv.lib.vvclf_batch_destroy(b)
+    for j in jobs:
+        j.close()
+    ctx.close()
+
+
 @pytest.mark.parametrize("tw,th", [(64, 64), (32, 32)])
 def test_batch_high_bit_depth(vv, orc, tw, th):
     say(f"batch >12-bit tile {tw}x{th}")

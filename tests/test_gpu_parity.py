@@ -6,7 +6,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import Pic, alf_cfg, alf_picture, binding_of, cu, cu_array, dparams, sao_band, sao_cfg, sao_edge, sao_picture
+from helpers import Pic, alf_cfg, put_wide_samples, alf_picture, binding_of, cu, cu_array, dparams, sao_band, sao_cfg, sao_edge, sao_picture
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +59,21 @@ def test_full_chain_small(vv, orc, cfg, w, h, seed, kw):
 @pytest.mark.parametrize("cfg,w,h,seed", [(0, 1920, 1080, 0), (1, 3840, 2160, 1000), (3, 1920, 1080, 3000)])
 def test_full_chain_full_size(vv, orc, cfg, w, h, seed):
     full_chain(vv, orc, cfg, w, h, seed)
+
+
+@pytest.mark.parametrize("cfg,w,h,seed,kw", [(1, 416, 240, 31, {}), (0, 256, 128, 32, {"rich": 1}),
+                                           (1, 416, 240, 33, {"bit_depth": 12, "rich": 1}),
+                                           (1, 208, 120, 34, {"lmcs": 0})])
+def test_samples_beyond_the_bit_depth(vv, orc, cfg, w, h, seed, kw):
+    """Samples of 2^15 and more (which the 16-bit ALF lanes would wrap) in
+    every plane: the drop-in call still matches the oracle."""
+    say(f"wide samples cfg={cfg} {w}x{h} seed={seed} {kw}")
+    fo = vv.SynthFrame(cfg, w, h, seed=seed, **kw)
+    fg = vv.SynthFrame(cfg, w, h, seed=seed, **kw)
+    put_wide_samples(vv, [fo, fg], seed)
+    assert orc.oracle_apply_in_loop_filters(fo.desc) == 0
+    assert vv.lib.vvclf_apply_in_loop_filters(fg.desc) == 0, vv.lib.vvclf_last_error()
+    assert_same(fo.output_arrays(), fg.output_arrays(), f"wide samples cfg {cfg} {w}x{h} {kw}")
 
 
 def test_stages_standalone(vv, orc):
