@@ -96,10 +96,13 @@ struct rect_stage {
   uint32_t v[PER];
   // map == nullptr: nothing to load (every element 0)
   __device__ __forceinline__ void issue(const uint32_t* map, int mw, const mrect& r) {
+    // 64-bit base once; element offsets in 32 bits (rows < MH, map widths
+    // below 2^24)
+    const uint32_t* base = map ? map + (size_t)r.y0 * mw + r.x0 : nullptr;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = ltid() + k * NT, yy = i / MW, xx = i - yy * MW;
-      v[k] = (map && yy < r.ny && xx < r.nx) ? gld(&map[(size_t)(r.y0 + yy) * mw + r.x0 + xx]) : 0u;
+      const int i = ltid() + k * NT, yy = i / MW, xx = i - __mul24(yy, MW);
+      v[k] = (map && yy < r.ny && xx < r.nx) ? gld(base + (__mul24(yy, mw) + xx)) : 0u;
     }
   }
   // also lists the nonzero words as tag | (row << 6) | column, reserving the
@@ -111,7 +114,7 @@ struct rect_stage {
     uint32_t tot = 0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = ltid() + k * NT, yy = i / MW, xx = i - yy * MW;
+      const int i = ltid() + k * NT, yy = i / MW, xx = i - __mul24(yy, MW);
       const bool in = yy < r.ny && xx < r.nx;
       if (in) dst[i] = v[k];
       m[k] = __ballot(in && v[k] != 0);
@@ -125,7 +128,7 @@ struct rect_stage {
     for (int k = 0; k < PER; k++) {
       const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[k] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[k], 0u));
       if ((m[k] >> (ltid() & 63)) & 1) {
-        const int i = ltid() + k * NT, yy = i / MW, xx = i - yy * MW;
+        const int i = ltid() + k * NT, yy = i / MW, xx = i - __mul24(yy, MW);
         list[base + below] = (uint16_t)(tag | (uint32_t)(yy << 6) | (uint32_t)xx);
       }
       base += (uint32_t)__popcll(m[k]);
@@ -346,7 +349,7 @@ __device__ __forceinline__ void lds_sao(const dev_job& j, const uint64_t (*saop)
     dst.p += pl * dd1;
     // column-major: a wave's strips share few CTB columns, and so mostly one
     // SAO type and class
-    const int kk = i / NS, st = i - kk * NS;
+    const int kk = i / NS, st = i - __mul24(kk, NS);
     const int ys = dst.oy + st * R;
     if (kk >= nk || ys >= dst.oy + dst.h) continue;
     const int nr = min(R, dst.oy + dst.h - ys);
@@ -583,8 +586,8 @@ struct tile_loads {
     const uint16_t* src = j.in.y + (size_t)L.oy * j.py + L.ox;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
-      v[k] = (i < total && ch < nch) ? gld(reinterpret_cast<const uint4*>(src + (size_t)yy * j.py + 8 * ch))
+      const int i = tid + k * NT, yy = i / NCH, ch = i - __mul24(yy, NCH);
+      v[k] = (i < total && ch < nch) ? gld(reinterpret_cast<const uint4*>(src + (__mul24(yy, j.py) + 8 * ch)))
                                      : make_uint4(0, 0, 0, 0);
     }
     const int nchc = (g.Cw.w + 3) >> 2, per_plane = g.Cw.h * NC4;
@@ -593,9 +596,9 @@ struct tile_loads {
       const uint16_t* csrc = (c ? j.in.cr : j.in.cb) + (size_t)g.Cw.oy * j.pc + g.Cw.ox;
 #pragma unroll
       for (int k = 0; k < PERC; k++) {
-        const int i = tid + k * NT, yy = i / NC4, ch = i - yy * NC4;
+        const int i = tid + k * NT, yy = i / NC4, ch = i - __mul24(yy, NC4);
         vc[c][k] = (g.chroma && i < per_plane && ch < nchc)
-                       ? gld(reinterpret_cast<const uint2*>(csrc + (size_t)yy * j.pc + 4 * ch))
+                       ? gld(reinterpret_cast<const uint2*>(csrc + (__mul24(yy, j.pc) + 4 * ch)))
                        : make_uint2(0, 0);
       }
     }
@@ -661,7 +664,7 @@ struct tile_loads {
     uint32_t any = 0;  // OR of the samples written (the ALF's 16-bit lanes need them below 2^15)
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
+      const int i = tid + k * NT, yy = i / NCH, ch = i - __mul24(yy, NCH);
       if (i < total && ch < nch) {
         uint4 w = v[k];
         if (j.do_lmcs) {
@@ -676,7 +679,7 @@ struct tile_loads {
     if (far) {
 #pragma unroll
       for (int k = 0; k < PER; k++) {
-        const int i = tid + k * NT, yy = i / NCH, ch = i - yy * NCH;
+        const int i = tid + k * NT, yy = i / NCH, ch = i - __mul24(yy, NCH);
         const uint4 w = v[k];
         if (i < total && ch < nch && (!lut_lds || ((w.x | w.y | w.z | w.w) & 0xfc00fc00u) != 0)) {
           uint32_t w4[4] = {w.x, w.y, w.z, w.w};
@@ -693,7 +696,7 @@ struct tile_loads {
       for (int c = 0; c < 2; c++)
 #pragma unroll
         for (int k = 0; k < PERC; k++) {
-          const int i = tid + k * NT, yy = i / NC4, ch = i - yy * NC4;
+          const int i = tid + k * NT, yy = i / NC4, ch = i - __mul24(yy, NC4);
           if (i < per_plane && ch < nchc) {
             *reinterpret_cast<uint2*>(&s.Cc[c][yy * LD::CW + 4 * ch]) = vc[c][k];
             any |= vc[c][k].x | vc[c][k].y;
@@ -885,7 +888,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     const s16x2 zero2 = {0, 0}, vmax2 = {(short)amax, (short)amax};
     for (int i = ltid(); i < 2 * npair; i += NT) {
       const int c = i >= npair ? 1 : 0, k = i - c * npair;
-      const int yy = k / PW, x = cx0 + 2 * (k - yy * PW), y = cy0 + yy;
+      const int yy = k / PW, x = cx0 + 2 * (k - __mul24(yy, PW)), y = cy0 + yy;
       if (x >= cx0 + cwid) continue;
       const uint16_t* sc = s.o.Sc[c];
       // pair of padded chroma samples (xa, xa+1), xa - cx0 even
@@ -987,7 +990,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     constexpr int LCH = TW / 8, NLC = (TH * LCH + NT - 1) / NT;
 #pragma unroll
     for (int q = 0; q < NLC; q++) {
-      const int i = tid + q * NT, yy = i / LCH, xx = 8 * (i - yy * LCH);
+      const int i = tid + q * NT, yy = i / LCH, xx = 8 * (i - __mul24(yy, LCH));
       if (i < TH * LCH && y0 + yy < y1 && x0 + xx < x1)
         put(&s.L[yy * TW + xx], j.out.y + (size_t)(y0 + yy) * j.py + x0 + xx, x0 + xx, x1);
     }
@@ -999,7 +1002,7 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
       const ptrdiff_t dcr = j.out.cr - j.out.cb;
 #pragma unroll
       for (int q = 0; q < NCC; q++) {
-        const int i = tid + q * NT, row = i / CCH, xx = 8 * (i - row * CCH);
+        const int i = tid + q * NT, row = i / CCH, xx = 8 * (i - __mul24(row, CCH));
         const int c = row >= TH / 2 ? 1 : 0, yy = row - c * (TH / 2);
         if (i < TH * CCH && cy0 + yy < cy1 && cx0 + xx < cx1)
           put(&oc[row * (TW / 2) + xx], j.out.cb + c * dcr + (size_t)(cy0 + yy) * j.pc + cx0 + xx, cx0 + xx, cx1);
