@@ -368,10 +368,12 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
     uint32_t P[10][6];
     if (x4 - 3 >= cx0 && x4 + 6 <= cx1 && y4 - 3 >= cy0 && y4 + 6 <= cy1) {
       // inside the rectangle: three aligned 8-byte reads per row (x4 - 4 is
-      // a multiple of 4 and plane pitches are multiples of 64)
+      // a multiple of 4 and plane pitches are multiples of 64); rows as
+      // 24-bit offsets from the window's first row
+      const uint16_t* w0 = src.p + (size_t)(y4 - 3) * src.pitch + x4 - 4;
 #pragma unroll
       for (int rr = 0; rr < 10; rr++) {
-        const uint2* q = reinterpret_cast<const uint2*>(src.p + (size_t)(y4 - 3 + rr) * src.pitch + x4 - 4);
+        const uint2* q = reinterpret_cast<const uint2*>(w0 + __mul24(rr, src.pitch));
         const uint2 a = q[0], b = q[1], c = q[2];
         P[rr][0] = a.x; P[rr][1] = a.y; P[rr][2] = b.x; P[rr][3] = b.y; P[rr][4] = c.x; P[rr][5] = c.y;
       }
@@ -459,8 +461,9 @@ __global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
       bool done = false;
       if (j.alf_bd <= 12 && x - 2 >= cx0 && x + 3 <= cx1 && y - 2 >= cy0 && y + 2 <= cy1) {
         // neighbourhood inside the rectangle: packed pairs (see the fused form)
+        const uint16_t* c0p = src.p + (size_t)y * src.pitch + x;  // rows as 24-bit offsets from here
         auto P2 = [&](int dx, int dy) -> uint32_t {
-          return *reinterpret_cast<const uint32_t*>(src.p + (size_t)(y + dy) * src.pitch + x + dx);
+          return *reinterpret_cast<const uint32_t*>(c0p + (__mul24(dy, src.pitch) + dx));
         };
         const uint32_t m2 = P2(-2, 0), m0 = outp, p2 = P2(2, 0);
         const uint32_t am2 = P2(-2, y1), a0 = P2(0, y1), ap2 = P2(2, y1);
@@ -518,19 +521,22 @@ __global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
       int yp1, yp2;
       alf_t47((y - y_ctb_c) * 2, ctb, lbb, yp1, yp2);
       const int half = 1 << (j.alf_bd - 1);
+      // the four luma rows (rect-clamped), as 24-bit offsets from the centre row
+      const int rc = clampi(yl, ly0, ly1 - 1);
+      const uint16_t* lc = lum.p + (size_t)rc * lum.pitch;
+      auto lrow = [&](int dy) { return lc + __mul24(clampi(yl + dy, ly0, ly1 - 1) - rc, lum.pitch); };
+      const uint16_t *la = lrow(-yp1), *lb = lrow(yp1), *ld = lrow(yp2);
       uint32_t o2[2];
       for (int h = 0; h < 2; h++) {
         const int xl = 2 * (x + h);
-        auto L = [&](int dx, int dy) -> int {
-          return lum.p[(size_t)clampi(yl + dy, ly0, ly1 - 1) * lum.pitch + clampi(xl + dx, lx0, lx1 - 1)];
-        };
-        const int cen = L(0, 0);
+        auto cx = [&](int dx) { return clampi(xl + dx, lx0, lx1 - 1); };
+        const int cen = lc[cx(0)];
         // 24-bit products: coefficients are below 2^23 in magnitude (checked
         // on the host) and differences within 17 bits
-        const int sum = __mul24(cf[0], L(0, -yp1) - cen) + __mul24(cf[1], L(-1, 0) - cen) +
-                        __mul24(cf[2], L(1, 0) - cen) + __mul24(cf[3], L(-1, yp1) - cen) +
-                        __mul24(cf[4], L(0, yp1) - cen) + __mul24(cf[5], L(1, yp1) - cen) +
-                        __mul24(cf[6], L(0, yp2) - cen);
+        const int sum = __mul24(cf[0], la[cx(0)] - cen) + __mul24(cf[1], lc[cx(-1)] - cen) +
+                        __mul24(cf[2], lc[cx(1)] - cen) + __mul24(cf[3], lb[cx(-1)] - cen) +
+                        __mul24(cf[4], lb[cx(0)] - cen) + __mul24(cf[5], lb[cx(1)] - cen) +
+                        __mul24(cf[6], ld[cx(0)] - cen);
         const int base = (int)((outp >> (16 * h)) & 0xffffu);
         o2[h] = (uint32_t)clampi(base + clampi((sum + 64) >> 7, -half, half - 1), 0, maxv);
       }
