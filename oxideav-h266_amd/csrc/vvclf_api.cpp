@@ -95,13 +95,6 @@ vvclf_ctx* default_ctx() {
 
 namespace vvclf {
 
-// Host threads for the deblocking-descriptor walk of calls that handle one
-// picture at a time (the serving prep threads use one each).
-int host_walk_threads() {
-  static const int n = (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
-  return n;
-}
-
 int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvclf_job** out,
                       uint64_t* dry_stats, const job_opts* opts, host_tables* keep) {
   const resident_tables* res = opts ? opts->res : nullptr;
@@ -422,10 +415,7 @@ void copy_planes(const plane_copy* pc, int n) {
       base += pc[i].rows;
     }
   };
-  std::vector<std::thread> pool;
-  for (int ti = 1; ti < nt; ti++) pool.emplace_back(part, ti);
-  part(0);
-  for (auto& th : pool) th.join();
+  parallel_for(nt, part);
 }
 }  // namespace
 

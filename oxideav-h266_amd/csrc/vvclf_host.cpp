@@ -7,6 +7,9 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <functional>
 #include <memory>
 #include <cstdio>
 #include <cstring>
@@ -65,13 +68,51 @@ void zeroed_free_bytes(void* p, size_t bytes) {
 
 namespace {
 
-// Runs fn(0 .. nt-1), fn(0) on the calling thread.
+// The worker pool of parallel_for.  A call's job lives on the caller's
+// stack: workers take a reference under the mutex before touching it, and
+// the caller returns only once the job has left the queue, every index has
+// run and no worker holds it.
+struct pf_job {
+  const std::function<void(int)>* fn;
+  int n;
+  std::atomic<int> next{0};
+  int done = 0;  // indices finished (under the pool mutex)
+  int refs = 0;  // workers holding the job (under the pool mutex)
+};
+
+struct host_pool {
+  std::mutex mu;
+  std::condition_variable work_cv, done_cv;
+  std::deque<pf_job*> q;
+  explicit host_pool(int workers) {
+    for (int i = 0; i < workers; i++) std::thread([this] { loop(); }).detach();
+  }
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      work_cv.wait(lk, [&] { return !q.empty(); });
+      pf_job* j = q.front();
+      j->refs++;
+      lk.unlock();
+      int ran = 0;
+      for (int i; (i = j->next.fetch_add(1)) < j->n; ran++) (*j->fn)(i);
+      lk.lock();
+      if (!q.empty() && q.front() == j) q.pop_front();  // indices exhausted
+      j->done += ran;
+      j->refs--;
+      done_cv.notify_all();
+    }
+  }
+};
+
+host_pool& the_pool() {
+  static host_pool* p = new host_pool(std::max(0, host_walk_threads() - 1));  // workers live with the process
+  return *p;
+}
+
 template <class F>
 void run_threads(int nt, F&& fn) {
-  std::vector<std::thread> pool;
-  for (int ti = 1; ti < nt; ti++) pool.emplace_back(fn, ti);
-  fn(0);
-  for (auto& th : pool) th.join();
+  parallel_for(nt, std::function<void(int)>(fn));
 }
 
 struct cu_grid {
@@ -193,6 +234,35 @@ bool join_bands(std::vector<finish_band>& bands, uint32_t w, int luma_dir, std::
 
 }  // namespace
 
+int host_walk_threads() {
+  static const int n = (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+  return n;
+}
+
+void parallel_for(int n, const std::function<void(int)>& fn) {
+  if (n <= 0) return;
+  if (n == 1) {
+    fn(0);
+    return;
+  }
+  host_pool& p = the_pool();
+  pf_job j;
+  j.fn = &fn;
+  j.n = n;
+  {
+    std::lock_guard<std::mutex> g(p.mu);
+    p.q.push_back(&j);
+  }
+  p.work_cv.notify_all();
+  int ran = 0;
+  for (int i; (i = j.next.fetch_add(1)) < n; ran++) fn(i);
+  std::unique_lock<std::mutex> lk(p.mu);
+  const auto it = std::find(p.q.begin(), p.q.end(), &j);
+  if (it != p.q.end()) p.q.erase(it);  // no worker takes it from here on
+  j.done += ran;
+  p.done_cv.wait(lk, [&] { return j.done == n && j.refs == 0; });
+}
+
 int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc, uint32_t Hc, host_tables& t,
                         int threads) {
   const vvclf_deblock_params& prm = f.deblock;
@@ -284,10 +354,7 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
       }
     }
   };
-  std::vector<std::thread> pool;
-  for (int ti = 1; ti < nt; ti++) pool.emplace_back(work, ti);
-  work(0);
-  for (auto& th : pool) th.join();
+  run_threads(nt, work);
   if (bad)
     return fail(VVCLF_ERR_INVALID, "deblock: CU records overlap at (%d,%d): they must tile the picture", bad_xy[0],
                 bad_xy[1]);

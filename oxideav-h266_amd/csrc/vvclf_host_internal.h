@@ -5,6 +5,7 @@
 #include <stdarg.h>
 
 #include <cstdlib>
+#include <functional>
 #include <new>
 #include <utility>
 #include <vector>
@@ -85,6 +86,10 @@ void build_lmcs_lut(const vvclf_lmcs_binding& b, std::vector<uint16_t>& lut);
 // Threads of the host CU walk (up to 8; only pictures of >= 4096 CU records
 // use more than one).
 int host_walk_threads();
+// Runs fn(0) .. fn(n - 1) on the calling thread and a process-wide pool of
+// host_walk_threads() - 1 workers (started on first use), returning when all
+// have finished.  Concurrent callers share the workers.
+void parallel_for(int n, const std::function<void(int)>& fn);
 void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t);
 
 static constexpr int PLAIN_STAGES = 6;  // prep, deblock V, deblock H, SAO, ALF luma, ALF chroma
