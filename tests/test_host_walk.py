@@ -1,8 +1,9 @@
-"""CPU check that the multi-threaded host CU walk derives exactly the
-deblocking descriptor maps, work lists and bS histogram of the one-thread
-walk (tests/host_walk_check.cpp, compiled here from the package's host
-sources).  The one-thread walk itself is pinned against the oracle by the
-GPU parity tests."""
+"""CPU checks of the host side's threading: the multi-threaded CU walk derives
+exactly the deblocking descriptor maps, work lists and bS histogram of the
+one-thread walk (tests/host_walk_check.cpp), and the worker pool runs every
+index once under concurrent callers (tests/pool_check.cpp); both compiled
+here from the package's host sources.  The one-thread walk itself is pinned
+against the oracle by the GPU parity tests."""
 import os
 import shutil
 import subprocess
@@ -27,3 +28,18 @@ def test_threaded_walk_matches_one_thread(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("identical") == 9
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or not os.path.isdir(os.path.join(ROCM, "include", "hip")),
+                    reason="needs g++ and the HIP headers")
+def test_worker_pool_runs_every_index_once(tmp_path):
+    """The host worker pool behind the threaded walk and the staging copies,
+    with six threads calling it at once."""
+    exe = tmp_path / "pool_check"
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{CSRC}",
+           os.path.join(ROOT, "tests", "pool_check.cpp"), os.path.join(CSRC, "vvclf_host.cpp"), f"-L{ROCM}/lib",
+           "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-lpthread", "-o", str(exe)]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "missed or repeated indices: 0" in r.stdout
