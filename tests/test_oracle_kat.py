@@ -492,6 +492,62 @@ def test_alf_luma_five_point_stencil(orc, vv):
     assert (pic.y[rows] == want[rows]).all()
 
 
+# eq. 1490 pairs (src/alf.rs:1051-1056) with y1 = 1, y2 = 2 (Table 46 away
+# from the virtual boundary): each chroma tap k adds f[k] times the sum of two
+# clipped differences at these offsets (dx, dy).
+CHROMA_TAPS = [((0, 2), (0, -2)), ((1, 1), (-1, -1)), ((0, 1), (0, -1)),
+               ((-1, 1), (1, -1)), ((2, 0), (-2, 0)), ((1, 0), (-1, 0))]
+
+
+@pytest.mark.parametrize("clip_idx", [0, 2])
+@pytest.mark.parametrize("tap", range(6))
+def test_alf_chroma_single_tap_closed_form(orc, vv, tap, clip_idx):
+    # closed form: with a single nonzero chroma coefficient the 5x5 diamond
+    # reduces to out = s + ((f * (clip(a - s) + clip(b - s)) + 64) >> 7) for
+    # the tap's two mirrored neighbours a, b (eqs. 1490-1492), with picture
+    # edge replication.  Cb takes alternative 0 with f = +37 on `tap`, Cr takes
+    # alternative 1 with f = -29 on the next tap, so the alternative index and
+    # the coefficient sign are pinned too.  AlfClip[10][clip_idx] is 2^10 or
+    # 2^5 (Table 8).  Two CTBs side by side check neighbours across the CTB
+    # edge; rows 28..31 of each 32-row chroma CTB use Table 46's modified
+    # offsets and are left out.
+    rng = np.random.default_rng(10 + tap)
+    W, H = 128, 64
+    pic = Pic(vv, W, H, 0, bd=10, chroma=0)
+    pic.cb[:] = rng.integers(0, 1024, pic.cb.shape)
+    pic.cr[:] = rng.integers(0, 1024, pic.cr.shape)
+    pre = {"cb": pic.cb.astype(np.int64), "cr": pic.cr.astype(np.int64)}
+    aps = vv.AlfAps()
+    aps.alf_chroma_filter_signal_flag = 1
+    aps.alf_chroma_clip_flag = 1
+    aps.alf_chroma_num_alt_filters_minus1 = 1
+    aps.n_chroma_coeff = 2
+    plan = {"cb": (0, tap, 37), "cr": (1, (tap + 1) % 6, -29)}
+    for alt, k, f in plan.values():
+        aps.chroma_coeff[alt][k] = f
+        aps.chroma_clip_idx[alt][k] = clip_idx
+    b = binding_of(vv, chroma=aps)
+    ctb = vv.AlfCtb()
+    ctb.cb_on = ctb.cr_on = 1
+    ctb.cb_alt_idx, ctb.cr_alt_idx = 0, 1
+    ap, _k = alf_picture(vv, 2, 1, {(0, 0): ctb, (1, 0): ctb})
+    run_alf(orc, vv, pic, ap, alf_cfg(vv, bd=10, ctb_log2=6, cb=True, cr=True), b)
+    lim = 1 << (10 if clip_idx == 0 else 5)
+    rows = [y for y in range(32) if y < 28]
+    for name, (alt, k, f) in plan.items():
+        s = pre[name]
+        pad = np.pad(s, 2, mode="edge")
+        h, w = s.shape
+        acc = np.zeros_like(s)
+        for dx, dy in CHROMA_TAPS[k]:
+            nb = pad[2 + dy:2 + dy + h, 2 + dx:2 + dx + w]
+            acc += np.clip(nb - s, -lim, lim)
+        want = np.clip(s + ((f * acc + 64) >> 7), 0, 1023)
+        got = getattr(pic, name)
+        assert (got[rows] == want[rows]).all(), (name, k)
+    assert (pic.y == 0).all()
+
+
 def test_cc_alf_single_tap(orc, vv):
     # closed form: CC-ALF with only the right-neighbour tap (k = 2) set to 64
     # adds clip((64 * (L[2y][2x+1] - L[2y][2x]) + 64) >> 7) to the chroma
