@@ -160,6 +160,100 @@ def test_deblock_flat_plane_is_identity(orc, vv, seed):
     assert (y == 500).all() and (cb == 300).all() and (cr == 700).all()
 
 
+def _motion_edge(orc, vv, p_cell, q_cell, p_kind="inter", q_kind="inter"):
+    """Two uncoded 16x8 CUs meeting at x = 16 over a 100 | 130 step, with a
+    4x4 motion grid whose cells left of the edge hold `p_cell` and right of it
+    `q_cell` (src/deblock.rs:351-363).  Returns the deblocked luma."""
+    pic = Pic(vv, 32, 8, 0)
+    pic.y[:, :16] = 100
+    pic.y[:, 16:] = 130
+    cus = []
+    for x, kind in ((0, p_kind), (16, q_kind)):
+        c = cu(vv, x, 0, 16, 8, 37, intra=(kind == "intra"), coded=False)
+        c.ibc = int(kind == "ibc")
+        c.ciip = int(kind == "ciip")
+        cus.append(c)
+    cells = (vv.MvCell * 16)()
+    for cy in range(2):
+        for cx in range(8):
+            src = p_cell if cx < 4 else q_cell
+            for name, val in src.items():
+                if name in ("mv_l0", "mv_l1"):
+                    getattr(cells[cy * 8 + cx], name)[:] = val
+                else:
+                    setattr(cells[cy * 8 + cx], name, val)
+    grid = vv.MvGrid(C.cast(cells, C.POINTER(vv.MvCell)), 8, 2)
+    arr = cu_array(vv, cus)
+    orc.oracle_apply_deblocking_clipped(pic.ptr(), arr, 2, None, 0, C.byref(grid), C.byref(dparams(vv)), 1,
+                                        None, 0, None, 0)
+    return pic.y.copy()
+
+
+def _uni(poc, mv, l1=False):
+    return {"pred_l1": 1, "poc_l1": poc, "mv_l1": mv} if l1 else {"pred_l0": 1, "poc_l0": poc, "mv_l0": mv}
+
+
+def _bi(poc0, mv0, poc1, mv1):
+    return {"pred_l0": 1, "pred_l1": 1, "poc_l0": poc0, "poc_l1": poc1, "mv_l0": mv0, "mv_l1": mv1}
+
+
+# Known answers of the section 8.8.3.5 motion arms (src/deblock.rs:984-1043):
+# bS = 1 iff the number of MVs differs, the reference pictures differ (by
+# identity, whichever list holds them), or an MV component differs by >= 8 in
+# 1/16-sample units; for bi-prediction from one picture twice, only when both
+# the list-aligned and the crossed pairing differ.
+MOTION_CASES = [
+    ("uni mv dx 7", _uni(4, (0, 0)), _uni(4, (7, 0)), False),
+    ("uni mv dx 8", _uni(4, (0, 0)), _uni(4, (8, 0)), True),
+    ("uni mv dy -8", _uni(4, (3, 5)), _uni(4, (3, -3)), True),
+    ("uni mv dx -7 dy 7", _uni(4, (10, 10)), _uni(4, (3, 17)), False),
+    ("uni same picture other list", _uni(4, (1, 2)), _uni(4, (1, 2), l1=True), False),
+    ("uni other picture", _uni(4, (1, 2)), _uni(8, (1, 2)), True),
+    ("uni vs bi", _uni(4, (0, 0)), _bi(4, (0, 0), 8, (0, 0)), True),
+    ("no mv either side", {}, {}, False),
+    ("bi lists swapped", _bi(4, (5, 5), 8, (-9, 2)), _bi(8, (-9, 2), 4, (5, 5)), False),
+    ("bi lists swapped mv 8", _bi(4, (5, 5), 8, (-9, 2)), _bi(8, (-9, 2), 4, (13, 5)), True),
+    ("bi other picture", _bi(4, (0, 0), 8, (0, 0)), _bi(4, (0, 0), 12, (0, 0)), True),
+    ("bi same pic crossed equal", _bi(4, (0, 0), 4, (16, 0)), _bi(4, (16, 0), 4, (0, 0)), False),
+    ("bi same pic both differ", _bi(4, (0, 0), 4, (0, 0)), _bi(4, (8, 0), 4, (8, 0)), True),
+    ("bi same pic one pairing", _bi(4, (0, 0), 4, (8, 0)), _bi(4, (0, 0), 4, (0, 0)), True),
+]
+
+
+@pytest.mark.parametrize("name,p,q,filtered", MOTION_CASES, ids=[m[0] for m in MOTION_CASES])
+def test_deblock_motion_bs_arms(orc, vv, name, p, q, filtered):
+    # With both sides uncoded inter, the edge is filtered iff the motion arms
+    # give bS = 1: the 100 | 130 step then moves; with bS = 0 it stays.
+    y = _motion_edge(orc, vv, p, q)
+    changed = (y[:, :16] != 100).any() or (y[:, 16:] != 130).any()
+    assert changed == filtered
+
+
+@pytest.mark.parametrize("bv_dx,filtered", [(7, False), (-7, False), (8, True), (-8, True)])
+def test_deblock_ibc_bs_arms(orc, vv, bv_dx, filtered):
+    # src/deblock.rs:1093-1106: IBC on both sides compares the block vectors
+    # (held in the L0 slot) with the same >= 8 threshold; IBC against inter is
+    # a prediction-mode difference and always gives bS = 1.
+    p, q = _uni(0, (4, 4)), _uni(0, (4 + bv_dx, 4))
+    y = _motion_edge(orc, vv, p, q, "ibc", "ibc")
+    assert ((y[:, :16] != 100).any() or (y[:, 16:] != 130).any()) == filtered
+    same = _uni(0, (4, 4))
+    y = _motion_edge(orc, vv, same, same, "ibc", "inter")
+    assert (y[:, :16] != 100).any()
+
+
+def test_deblock_ciip_is_bs_2(orc, vv):
+    # src/deblock.rs:1072-1078: CIIP on either side takes the intra arm
+    # (bS = 2, tC from Q + 2), so it filters exactly like an intra
+    # neighbour, and differently from a bS = 1 motion edge on this step.
+    same = _uni(4, (0, 0))
+    intra = _motion_edge(orc, vv, same, same, "intra", "inter")
+    ciip = _motion_edge(orc, vv, same, same, "inter", "ciip")
+    bs1 = _motion_edge(orc, vv, _uni(4, (0, 0)), _uni(4, (8, 0)))
+    assert (ciip == intra).all()
+    assert (intra != bs1).any()
+
+
 def test_deblock_tc_zero_is_identity(orc, vv):
     # closed form: tC' = 0 for Q < 18 (Table 43) -> the segment is untouched
     pic = Pic(vv, 32, 8, 0)
