@@ -562,6 +562,39 @@ __device__ __forceinline__ void alf_t47(int yl, int ctb, bool lbb, int& y1, int&
   }
 }
 
+__device__ __forceinline__ int lo16(uint32_t w) { return (int)(w & 0xffffu); }
+__device__ __forceinline__ int hi16(uint32_t w) { return (int)(w >> 16); }
+
+// The seven CC-ALF taps of one filter (src/alf.rs:447-523, eqs. 1513-1518).
+// The tap sum  sum_k f[k] * (L_k - cen)  is evaluated as
+// sum_k f[k] * L_k - cen * sum_k f[k]: the two agree modulo 2^32, the width
+// the sum is kept in, and the second needs no per-tap difference.  With
+// |f[k]| < 2^20 (checked on the host) every factor fits the signed 24-bit
+// multiplier: samples are below 2^16 and |sum_k f[k]| < 7 * 2^20.
+struct cc_taps {
+  int f[7], fsum;
+  __device__ __forceinline__ explicit cc_taps(const int32_t* cf) {
+    fsum = 0;
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+      f[t] = cf[t];
+      fsum += f[t];
+    }
+  }
+  // with the tap sum already known
+  __device__ __forceinline__ cc_taps(const int32_t* cf, int sum_of_taps) : fsum(sum_of_taps) {
+#pragma unroll
+    for (int t = 0; t < 7; t++) f[t] = cf[t];
+  }
+  // taps in eq. 1513 order: (0,-yP1), (-1,0), (1,0), (-1,yP1), (0,yP1), (1,yP1), (0,yP2)
+  __device__ __forceinline__ int sum(int a, int l, int r, int bm, int b0, int bp, int d, int cen) const {
+    const uint32_t s = (uint32_t)__mul24(f[0], a) + (uint32_t)__mul24(f[1], l) + (uint32_t)__mul24(f[2], r) +
+                       (uint32_t)__mul24(f[3], bm) + (uint32_t)__mul24(f[4], b0) + (uint32_t)__mul24(f[5], bp) +
+                       (uint32_t)__mul24(f[6], d);
+    return (int)(s - (uint32_t)__mul24(fsum, cen));
+  }
+};
+
 // One clipped tap pair: f * (clamp(a - c, -k, k) + clamp(b - c, -k, k)).
 __device__ __forceinline__ int alf_pair(uint32_t w, int a, int b, int c) {
   int f = (int)(int16_t)(w & 0xffff);

@@ -886,6 +886,16 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
     const int ctbc = ctb >> 1;
     const int amax = (1 << j.alf_bd) - 1, half = 1 << (j.alf_bd - 1);
     const s16x2 zero2 = {0, 0}, vmax2 = {(short)amax, (short)amax};
+    // each component's CC-ALF tap sum for this tile (one filter per CTB)
+    int ccsum0 = 0, ccsum1 = 0;
+    if (in_grid) {
+      const int f0 = (int)((cw >> 24) & 0xf), f1 = (int)((cw >> 28) & 0xf);
+#pragma unroll
+      for (int t = 0; t < 7; t++) {
+        if (f0 != (int)ALF_OFF4) ccsum0 += s.ccbank[f0 * 7 + t];
+        if (f1 != (int)ALF_OFF4) ccsum1 += s.ccbank[ALF_CCBANK_WORDS + f1 * 7 + t];
+      }
+    }
     for (int i = ltid(); i < 2 * npair; i += NT) {
       const int c = i >= npair ? 1 : 0, k = i - c * npair;
       const int yy = k / PW, x = cx0 + 2 * (k - __mul24(yy, PW)), y = cy0 + yy;
@@ -948,16 +958,12 @@ __device__ __forceinline__ void fused_tile(const dev_job& j, fused_lds<TW, TH>& 
           const bool lbb = !(yl + ctb >= H && (H - y_ctb_c * 2) <= ctb - 4);
           int yp1, yp2;
           alf_t47((y - y_ctb_c) * 2, ctb, lbb, yp1, yp2);
+          const cc_taps k(cf, c == 0 ? ccsum0 : ccsum1);
           uint32_t o2[2];
           for (int h = 0; h < 2; h++) {
             const int xl = 2 * (x + h);
-            const int cen = Sl(xl, yl);
-            // 24-bit products: coefficients are below 2^23 in magnitude
-            // (checked on the host) and differences within 17 bits
-            const int sum = __mul24(cf[0], Sl(xl, yl - yp1) - cen) + __mul24(cf[1], Sl(xl - 1, yl) - cen) +
-                            __mul24(cf[2], Sl(xl + 1, yl) - cen) + __mul24(cf[3], Sl(xl - 1, yl + yp1) - cen) +
-                            __mul24(cf[4], Sl(xl, yl + yp1) - cen) + __mul24(cf[5], Sl(xl + 1, yl + yp1) - cen) +
-                            __mul24(cf[6], Sl(xl, yl + yp2) - cen);
+            const int sum = k.sum(Sl(xl, yl - yp1), Sl(xl - 1, yl), Sl(xl + 1, yl), Sl(xl - 1, yl + yp1),
+                                  Sl(xl, yl + yp1), Sl(xl + 1, yl + yp1), Sl(xl, yl + yp2), Sl(xl, yl));
             const int base = (int)((outp >> (16 * h)) & 0xffffu);
             o2[h] = (uint32_t)clampi(base + clampi((sum + 64) >> 7, -half, half - 1), 0, amax);
           }

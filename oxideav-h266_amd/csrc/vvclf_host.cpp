@@ -660,7 +660,8 @@ int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t,
       t.alf_bd = ac.bit_depth;
       if (16 + b.n_luma_apses > 255) return fail(VVCLF_ERR_INVALID, "alf: too many luma APS slots");
       // The filters hold ALF coefficients in 16-bit lanes and multiply CC-ALF
-      // coefficients in 24 bits; conformant streams stay far inside both
+      // coefficients and their 7-tap sum in 24 bits (so each within 21 bits);
+      // conformant streams stay far inside both
       // (|c| <= 128 and <= 64), larger values are refused rather than
       // filtered differently.
       for (uint32_t i = 0; i < b.n_luma_apses; i++) {
@@ -683,8 +684,8 @@ int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t,
         for (uint32_t k = 0; k < nf; k++)
           for (int t = 0; t < VVCLF_ALF_CC_NUM_COEFF; t++) {
             const int32_t v = c == 0 ? cc->cc_cb_coeff[k][t] : cc->cc_cr_coeff[k][t];
-            if (v < -(1 << 23) || v >= (1 << 23))
-              return fail(VVCLF_ERR_UNSUPPORTED, "alf: CC-ALF coefficient %d outside 24 bits", v);
+            if (v < -(1 << 20) || v >= (1 << 20))
+              return fail(VVCLF_ERR_UNSUPPORTED, "alf: CC-ALF coefficient %d outside 21 bits", v);
           }
       }
       build_alf_banks(ac, b, t);

@@ -526,19 +526,27 @@ __global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
       const uint16_t* lc = lum.p + (size_t)rc * lum.pitch;
       auto lrow = [&](int dy) { return lc + __mul24(clampi(yl + dy, ly0, ly1 - 1) - rc, lum.pitch); };
       const uint16_t *la = lrow(-yp1), *lb = lrow(yp1), *ld = lrow(yp2);
+      const cc_taps k(cf);
+      const int xl = 2 * x;  // a multiple of 4
+      int sum[2];
+      if (xl - 1 >= lx0 && xl + 3 <= lx1 - 1) {
+        // luma columns xl-1 .. xl+3 inside the rectangle: aligned pair reads
+        const uint2 A = *reinterpret_cast<const uint2*>(la + xl), C = *reinterpret_cast<const uint2*>(lc + xl);
+        const uint2 B = *reinterpret_cast<const uint2*>(lb + xl), D = *reinterpret_cast<const uint2*>(ld + xl);
+        const uint32_t Cm = *reinterpret_cast<const uint32_t*>(lc + xl - 2);
+        const uint32_t Bm = *reinterpret_cast<const uint32_t*>(lb + xl - 2);
+        sum[0] = k.sum(lo16(A.x), hi16(Cm), hi16(C.x), hi16(Bm), lo16(B.x), hi16(B.x), lo16(D.x), lo16(C.x));
+        sum[1] = k.sum(lo16(A.y), hi16(C.x), hi16(C.y), hi16(B.x), lo16(B.y), hi16(B.y), lo16(D.y), lo16(C.y));
+      } else {
+        for (int h = 0; h < 2; h++) {
+          auto cx = [&](int dx) { return clampi(xl + 2 * h + dx, lx0, lx1 - 1); };
+          sum[h] = k.sum(la[cx(0)], lc[cx(-1)], lc[cx(1)], lb[cx(-1)], lb[cx(0)], lb[cx(1)], ld[cx(0)], lc[cx(0)]);
+        }
+      }
       uint32_t o2[2];
       for (int h = 0; h < 2; h++) {
-        const int xl = 2 * (x + h);
-        auto cx = [&](int dx) { return clampi(xl + dx, lx0, lx1 - 1); };
-        const int cen = lc[cx(0)];
-        // 24-bit products: coefficients are below 2^23 in magnitude (checked
-        // on the host) and differences within 17 bits
-        const int sum = __mul24(cf[0], la[cx(0)] - cen) + __mul24(cf[1], lc[cx(-1)] - cen) +
-                        __mul24(cf[2], lc[cx(1)] - cen) + __mul24(cf[3], lb[cx(-1)] - cen) +
-                        __mul24(cf[4], lb[cx(0)] - cen) + __mul24(cf[5], lb[cx(1)] - cen) +
-                        __mul24(cf[6], ld[cx(0)] - cen);
         const int base = (int)((outp >> (16 * h)) & 0xffffu);
-        o2[h] = (uint32_t)clampi(base + clampi((sum + 64) >> 7, -half, half - 1), 0, maxv);
+        o2[h] = (uint32_t)clampi(base + clampi((sum[h] + 64) >> 7, -half, half - 1), 0, maxv);
       }
       outp = o2[0] | (o2[1] << 16);
     }

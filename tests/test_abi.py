@@ -79,14 +79,16 @@ def test_short_luma_aps_is_invalid(vv):
 
 def test_alf_coefficients_beyond_the_filter_lanes_are_unsupported(vv):
     """The GPU filters hold ALF coefficients in 16-bit lanes and multiply
-    CC-ALF coefficients in 24 bits; values beyond those (no conformant stream
-    has them) are refused instead of being filtered differently, and the
-    largest values inside are accepted."""
+    CC-ALF coefficients (and the sum of a filter's seven) in 24 bits, which
+    holds for coefficients within 21 bits; values beyond those (no conformant
+    stream has them: the syntax gives at most 64 in magnitude) are refused
+    instead of being filtered differently, and the largest values inside are
+    accepted."""
     def frame():
         f = vv.SynthFrame(1, 416, 240, seed=3)
         return f, f.desc.contents.alf_binding
     for field, bad, good in [("luma", 32768, 32767), ("luma", -32769, -32768), ("chroma", 40000, -32768),
-                             ("cc", 1 << 23, (1 << 23) - 1), ("cc", -(1 << 23) - 1, -(1 << 23))]:
+                             ("cc", 1 << 20, (1 << 20) - 1), ("cc", -(1 << 20) - 1, -(1 << 20))]:
         for value, want in [(bad, vv.ERR_UNSUPPORTED), (good, 0)]:
             f, b = frame()
             if field == "luma":
