@@ -124,6 +124,90 @@ def test_asymmetric_long_filter_7_3_vertical(orc, vv):
     assert row[24] == 100
 
 
+# Long-filter arms on a flat 100 | 104 step at x = 32 (intra, qp 45, 8-bit),
+# worked by hand from src/deblock.rs:1552-1672: every refMiddle arm
+# (eqs. 1389-1392) gives 102 here ((1640 + 8) >> 4, or (820 + 4) >> 3 for
+# 5/3), refP = 100, refQ = 104, so p'_i = 100 + ((2 f_i + 32) >> 6) and
+# q'_j = (6688 - 2 g_j) >> 6 with F7 = 59 50 41 32 23 14 5, F5 = 58 45 32 19 6
+# and F3 = 53 32 11 (no clipping: tC is large).  A side is 3 when its CU is
+# 16 wide, and 5 through the sub-block cap at the CU edge (src/deblock.rs:
+# 857-866): P when the P CU has sub-blocks, Q when the Q CU has more than one
+# along the edge normal; the interior sub-block edges sit in flat samples.
+P7 = [100, 100, 101, 101, 101, 102, 102]  # x = 25 .. 31
+P5 = [100, 101, 101, 101, 102]  # x = 27 .. 31
+Q7 = [102, 102, 103, 103, 103, 104, 104]  # x = 32 .. 38
+Q5 = [102, 103, 103, 103, 104]  # x = 32 .. 36
+P3 = [100, 101, 102]  # x = 29 .. 31
+Q3 = [102, 103, 104]  # x = 32 .. 34
+
+
+@pytest.mark.parametrize("p_w,p_sb,q_w,q_sb,want_p,want_q",
+                         [(32, False, 32, False, P7, Q7), (32, True, 32, True, P5, Q5),
+                          (32, False, 32, True, P7, Q5), (32, True, 32, False, P5, Q7),
+                          (32, True, 16, False, P5, Q3), (16, False, 32, True, P3, Q5)],
+                         ids=["7-7", "5-5", "7-5", "5-7", "5-3", "3-5"])
+def test_long_filter_arms_on_a_step(orc, vv, p_w, p_sb, q_w, q_sb, want_p, want_q):
+    pic = Pic(vv, 64, 8, 0)
+    pic.y[:, :32] = 100
+    pic.y[:, 32:] = 104
+    p_cu, q_cu = cu(vv, 32 - p_w, 0, p_w, 8, 45), cu(vv, 32, 0, q_w, 8, 45)
+    for c, sb in ((p_cu, p_sb), (q_cu, q_sb)):
+        if sb:
+            c.has_num_sb, c.num_sb_x, c.num_sb_y = 1, 2, 1
+    cus = ([cu(vv, 0, 0, 16, 8, 45)] if p_w == 16 else []) + [p_cu, q_cu] + \
+        ([cu(vv, 48, 0, 16, 8, 45)] if q_w == 16 else [])  # the extra CUs' edges lie in flat samples
+    deblock(orc, vv, pic, cus, dparams(vv, ctb_log2=7))
+    for y in range(8):
+        row = [int(v) for v in pic.y[y]]
+        assert row[32 - len(want_p):32] == want_p
+        assert row[32:32 + len(want_q)] == want_q
+        assert row[31 - len(want_p)] == 100 and row[32 + len(want_q)] == 104  # beyond the arm: untouched
+        assert set(row[:31 - len(want_p)]) == {100} and set(row[33 + len(want_q):]) == {104}
+
+
+@pytest.mark.parametrize("vertical", [True, False], ids=["ver", "hor"])
+def test_chroma_strong_filter_on_a_step(orc, vv, vertical):
+    # A flat 100 | 108 chroma step between two intra CUs whose chroma blocks
+    # are 16 samples across the edge (maxFilterLength 3 on both sides), away
+    # from a CTB row: the strong-filter decision holds (flat sides, |p0 - q0|
+    # = 8 below (5 tC + 1) >> 1), and eqs. 1411-1416 (src/deblock.rs:
+    # 1986-2023) worked by hand give p2..p0 = (7P+Q+4)>>3, (6P+2Q+4)>>3,
+    # (5P+3Q+4)>>3 = 101, 102, 103 and q0..q2 = 105, 106, 107.  The weak
+    # filter would move p0 and q0 only.
+    W, H = (64, 16) if vertical else (16, 64)
+    pic = Pic(vv, W, H, 128)
+    for plane in (pic.cb, pic.cr):
+        if vertical:
+            plane[:, :16], plane[:, 16:] = 100, 108
+        else:
+            plane[:16, :], plane[16:, :] = 100, 108
+    cus = [cu(vv, 0, 0, 32, 16, 45), cu(vv, 32, 0, 32, 16, 45)] if vertical else \
+        [cu(vv, 0, 0, 16, 32, 45), cu(vv, 0, 32, 16, 32, 45)]
+    deblock(orc, vv, pic, cus, dparams(vv, ctb_log2=7))
+    want = [100, 101, 102, 103, 105, 106, 107, 108]  # positions 12 .. 19 across the edge
+    for plane in (pic.cb, pic.cr):
+        lines = plane if vertical else plane.T
+        for line in lines:
+            assert [int(v) for v in line[12:20]] == want
+            assert set(int(v) for v in line[:13]) == {100} and set(int(v) for v in line[19:]) == {108}
+
+
+def test_chroma_strong_filter_at_a_ctb_row(orc, vv):
+    # The same 100 | 108 step on a horizontal chroma edge that lies on a luma
+    # CTB row (CTB 64, edge at luma y = 64): maxFilterLengthP is 1 and the
+    # asymmetric arm of eqs. 1417-1420 (src/deblock.rs:2041-2055) runs.  By
+    # hand: p0 = (5P+3Q+4)>>3 = 103, q0..q2 = 105, 106, 107 as in the full arm,
+    # but p1 and p2 keep their input (100).
+    pic = Pic(vv, 16, 128, 128)
+    for plane in (pic.cb, pic.cr):
+        plane[:32, :], plane[32:, :] = 100, 108
+    deblock(orc, vv, pic, [cu(vv, 0, 0, 16, 64, 45), cu(vv, 0, 64, 16, 64, 45)], dparams(vv, ctb_log2=6))
+    for plane in (pic.cb, pic.cr):
+        for col in plane.T:
+            assert [int(v) for v in col[28:36]] == [100, 100, 100, 103, 105, 106, 107, 108]
+            assert set(int(v) for v in col[:31]) == {100} and set(int(v) for v in col[35:]) == {108}
+
+
 def test_ctb_row_boundary_caps_luma_p_side_at_3(orc, vv):
     # src/deblock.rs:2589-2638
     pic = Pic(vv, 8, 128, 128)
