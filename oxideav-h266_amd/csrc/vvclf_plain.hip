@@ -203,25 +203,33 @@ __global__ void __launch_bounds__(256) k_dbk(dev_job j, int luma_blocks) {
 // the picture or the CTB's rectangle) go sample by sample.
 // ---------------------------------------------------------------------------
 constexpr int SAO_ROWS = 1;
+static_assert(SAO_ROWS == 1, "k_sao gives each lane of a wave its own row");
+// log2 of the lanes per row of a k_sao wave for a CTB 2^log2c samples wide
+__host__ __device__ inline int sao_lanes_log2(int log2c) { return log2c < 3 ? 0 : (log2c > 9 ? 6 : log2c - 3); }
 
 __global__ void __launch_bounds__(256) k_sao(dev_job j) {
   const int comp = blockIdx.z;
   plane_view src = pick(j.a, j, comp), dst = pick(j.b, j, comp);
-  const int ys = blockIdx.y * SAO_ROWS;
-  const int x0 = (blockIdx.x * 256 + threadIdx.x) * 8;
-  if (ys >= src.h || x0 >= src.w) return;  // lanes leave in x order: a lane's right neighbour lane is
-                                           // present whenever x0 + 8 is inside the picture
-  const int nr = min(SAO_ROWS, src.h - ys);
   const int sub = comp ? 1 : 0;
   const int log2c = j.sao_ctb_log2 - sub;
+  // A wave covers a CTB-wide strip: 2^l2l lanes of 8 samples per row and
+  // 64 >> l2l rows, so that its lanes read one CTB's parameters (two when the
+  // CTB holds fewer than 512 samples) and take one filter path.  The four
+  // waves of a workgroup are stacked vertically.
+  const int l2l = sao_lanes_log2(log2c);
+  const int lane = threadIdx.x & 63, seg = lane & ((1 << l2l) - 1);
+  const int ys = ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * (64 >> l2l) + (lane >> l2l);
+  const int x0 = (((int)blockIdx.x << l2l) + seg) * 8;
+  if (ys >= src.h || x0 >= src.w) return;  // lanes of a row leave in x order: a lane's right neighbour
+                                           // lane is present whenever x0 + 8 is inside the picture
+  const int nr = min(SAO_ROWS, src.h - ys);
   const bool active = comp == 0 ? j.sao_luma : j.sao_chroma;
   const int maxv = (1 << j.sao_bd) - 1;
   // Rows ys-1 .. ys+SAO_ROWS (clamped) as 8-sample reads, and per row the
   // sample beyond the wave's end (x0-1 on lane 0, x0+8 elsewhere, clamped),
   // all issued with the parameter read; rows are padded to the pitch (a
   // multiple of 64), so the reads stay inside the plane past its width.
-  const int lane = threadIdx.x & 63;
-  const int ex = lane == 0 ? max(x0 - 1, 0) : min(x0 + 8, src.w - 1);
+  const int ex = seg == 0 ? max(x0 - 1, 0) : min(x0 + 8, src.w - 1);
   uint4 q[SAO_ROWS + 2];
   uint32_t e[SAO_ROWS + 2];
 #pragma unroll
@@ -245,8 +253,8 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
     const uint4& v = q[r];
     Rw[r].w[0] = v.x; Rw[r].w[1] = v.y; Rw[r].w[2] = v.z; Rw[r].w[3] = v.w;
     const uint32_t up = (uint32_t)__shfl_up((int)(v.w >> 16), 1), down = (uint32_t)__shfl_down((int)(v.x & 0xffffu), 1);
-    Rw[r].l = lane == 0 ? (x0 > 0 ? e[r] : (v.x & 0xffffu)) : up;
-    Rw[r].r = x0 + 8 >= src.w ? (v.w >> 16) : (lane == 63 ? e[r] : down);
+    Rw[r].l = seg == 0 ? (x0 > 0 ? e[r] : (v.x & 0xffffu)) : up;
+    Rw[r].r = x0 + 8 >= src.w ? (v.w >> 16) : (seg == (1 << l2l) - 1 ? e[r] : down);
   }
   const sao_run run = sao_run_of(prm);
   const bool packed = run.fits && j.sao_bd <= 15;
@@ -585,7 +593,17 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
     if (lb + cb) hipLaunchKernelGGL(k_dbk<false>, dim3(lb + cb), dim3(256), 0, s, j, (int)lb);
   }
   mark(3);
-  if (j.do_sao) hipLaunchKernelGGL(k_sao, dim3(cdiv(j.w, 2048), cdiv(j.h, SAO_ROWS), ncomp), dim3(256), 0, s, j);
+  if (j.do_sao) {
+    // strips of 8 << l2l samples, 4 * (64 >> l2l) rows per workgroup; the
+    // grid covers the largest plane in each direction
+    unsigned gx = 0, gy = 0;
+    for (int c = 0; c < ncomp; c++) {
+      const int l = sao_lanes_log2(j.sao_ctb_log2 - (c ? 1 : 0));
+      gx = std::max(gx, cdiv(c ? j.wc : j.w, 8u << l));
+      gy = std::max(gy, cdiv(c ? j.hc : j.h, 4u * (64u >> l)));
+    }
+    hipLaunchKernelGGL(k_sao, dim3(gx, gy, ncomp), dim3(256), 0, s, j);
+  }
   mark(4);
   if (j.do_alf) {
     int tl = j.ctb_log2 < 6 ? j.ctb_log2 : 6;
