@@ -440,12 +440,23 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
 // ---------------------------------------------------------------------------
 // ALF chroma + CC-ALF: one thread per chroma sample, B -> out.
 // ---------------------------------------------------------------------------
+// log2 of the lanes per row of a k_alf_chroma wave for a chroma CTB 2^log2c
+// samples wide (two samples per lane)
+__host__ __device__ inline int alf_chroma_lanes_log2(int log2c) {
+  return log2c < 1 ? 0 : (log2c > 7 ? 6 : log2c - 1);
+}
+
 __global__ void __launch_bounds__(256) k_alf_chroma(dev_job j) {
   // two horizontally adjacent samples (x, x+1), x even, per thread
+  // a wave covers a chroma-CTB-wide strip (2^l2l lanes of two samples per
+  // row, 64 >> l2l rows) so that its lanes share one CTB's filters and path;
+  // the four waves of a workgroup are stacked vertically
   const int comp = blockIdx.z + 1;
   plane_view src = pick(j.b, j, comp), dst = pick(j.out, j, comp);
-  const int x = 2 * (blockIdx.x * 64 + (threadIdx.x & 63));
-  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int l2l = alf_chroma_lanes_log2(j.ctb_log2 - 1);
+  const int lane = threadIdx.x & 63;
+  const int x = 2 * (((int)blockIdx.x << l2l) + (lane & ((1 << l2l) - 1)));
+  const int y = ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * (64 >> l2l) + (lane >> l2l);
   if (x >= src.w || y >= src.h) return;
   // rows are padded to the pitch (a multiple of 64): the pair read stays in
   // the plane even when x + 1 is past the width
@@ -617,8 +628,10 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
                          0, s, j, tl);
   }
   mark(5);
-  if (j.do_alf && ncomp == 3)
-    hipLaunchKernelGGL(k_alf_chroma, dim3(cdiv(j.wc, 128), cdiv(j.hc, 4), 2), dim3(256), 0, s, j);
+  if (j.do_alf && ncomp == 3) {
+    const int l = alf_chroma_lanes_log2(j.ctb_log2 - 1);
+    hipLaunchKernelGGL(k_alf_chroma, dim3(cdiv(j.wc, 2u << l), cdiv(j.hc, 4u * (64u >> l)), 2), dim3(256), 0, s, j);
+  }
   mark(6);
   return hipGetLastError();
 }
