@@ -26,11 +26,13 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def load(vv=None):
-    if not os.path.exists(LIB):
+def load(vv=None, path=None):
+    """Loads the oracle library (or, for tests of the tests, another build of
+    the same source at `path`) and declares its signatures."""
+    if path is None and not os.path.exists(LIB):
         build()
     vv = vv or load_package()
-    lib = C.CDLL(LIB)
+    lib = C.CDLL(path or LIB)
     P = C.POINTER
     u32, i32, u8 = C.c_uint32, C.c_int32, C.c_uint8
     sig = {
