@@ -9,16 +9,23 @@ once with its input and side information already resident in HBM.
 
 The same JSON line also carries:
 - "batched": the same 8 pictures in one persistent fused launch replayed from
-  a HIP graph (tile shape from the autotuner), and the 64-picture mixed batch
-  (16 each of 416x240, 832x480, 1920x1080, 3840x2160);
+  a HIP graph (tile shape from the autotuner); the 64-picture mixed batch
+  (16 each of 416x240, 832x480, 1920x1080, 3840x2160); and 4 pictures of
+  7680x4320, ALF-heavy with a dense edge grid (every CTB with luma ALF and
+  CC-ALF, 8 APS sets, minimum CU 4x8), each with its own roofline;
 - "served": continuous batching of Poisson arrivals (120 pictures/s over 32
-  streams of 832x480 / 1920x1080 / 3840x2160) with p50 / p99 latency;
+  streams of 832x480 / 1920x1080 / 3840x2160) with p50 / p99 latency, and a
+  saturation run (arrivals far above the device's rate);
 - "roofline" of the dominant plain kernel and of the fused kernel, from HIP
   events recorded on the launch stream, with the HBM traffic and the vector
   ALU instruction count per dispatch from the rocprofv3 summaries in
   profiles/ ("valu": issue rate against 256 CUs x 2.4 GHz x one wave64
   instruction per CU-cycle, the bound the integer filters actually meet);
-- "cpu_baseline": the CPU oracle timed on host cores.
+- "cpu_baseline": the CPU oracle (the C restatement of the reference filters)
+  timed on the host's cores, one thread and all threads, with the CPU model.
+
+Before any leg is timed, its output is compared with the oracle's output on
+the same inputs, bit for bit; a mismatch aborts the run.
 """
 import argparse
 import ctypes as C
@@ -28,6 +35,8 @@ import os
 import random
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
@@ -40,6 +49,8 @@ STAGE_BYTES_PER_PX = {"prep_lmcs": 6.0, "deblock_v": 6.0, "deblock_h": 6.0, "sao
 CHAIN_BYTES_PER_PX = 6.0
 KERNEL_NAMES = {"prep_lmcs": "k_prep", "deblock_v": "k_dbk<true>", "deblock_h": "k_dbk<false>", "sao": "k_sao",
                 "alf_luma": "k_alf_luma", "alf_chroma_ccalf": "k_alf_chroma"}
+# Host CPU share of one GPU on the benchmark machines.
+HOST_THREADS_CAP = 16
 
 
 def load(name, path):
@@ -53,32 +64,99 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(vv, n_pics, w, h, seconds):
-    """The CPU oracle (C restatement of the reference filters) on host cores,
-    one picture per thread, repeated until ~`seconds` of CPU work."""
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(HOST_THREADS_CAP, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def load_oracle(vv):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import binding
-    orc = binding.load(vv)
-    frames = [vv.SynthFrame(1, w, h, seed=1000 + i) for i in range(n_pics)]
-    threads = max(1, min(n_pics, os.cpu_count() or 1))
-    arr = (C.POINTER(vv.Frame) * n_pics)(*[f.desc for f in frames])
-    reps, wall, cpu = 0, 0.0, 0.0
-    while cpu < seconds:
+    return binding.load(vv)
+
+
+class Reference:
+    """Oracle outputs of generated pictures, computed on host threads."""
+
+    def __init__(self, vv, orc):
+        self.vv, self.orc = vv, orc
+
+    def outputs(self, specs):
+        vv = self.vv
+        frames = [vv.SynthFrame(cfg, w, h, seed=seed, **kw) for cfg, w, h, seed, kw in specs]
+        arr = (C.POINTER(vv.Frame) * len(frames))(*[f.desc for f in frames])
+        rc = self.orc.oracle_apply_in_loop_filters_mt(arr, len(frames), min(len(frames), host_threads()))
+        if rc != 0:
+            raise RuntimeError(f"oracle failed ({rc})")
+        outs = [f.output_arrays() for f in frames]
         for f in frames:
-            f.reset()
-        t0, c0 = time.perf_counter(), time.process_time()
-        rc = orc.oracle_apply_in_loop_filters_mt(arr, n_pics, threads)
-        wall += time.perf_counter() - t0
-        cpu += time.process_time() - c0
-        reps += 1
-        assert rc == 0
-    mpx = reps * n_pics * w * h / 1e6
-    return {"value": round(mpx / wall, 3), "unit": "Mpixel/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} x {n_pics} pictures {w}x{h} (the benchmark workload), oracle C restatement, "
-                      f"one picture per thread, {cpu:.1f} s CPU over {wall:.1f} s wall"}
+            f.close()
+        return outs
 
 
-def plain_run(vv, ctx, jobs, steps, warmup, W, H):
+def same(ref, got, what):
+    for name, a, b in zip(("luma", "cb", "cr"), ref, got):
+        if not np.array_equal(a, b):
+            raise SystemExit(f"bench: {what}: {name} differs from the oracle at {int((a != b).sum())} samples")
+
+
+def verify_jobs(vv, jobs, frames, refs, what):
+    for i, (j, f, r) in enumerate(zip(jobs, frames, refs)):
+        j.download(C.byref(f.desc.contents.pic))
+        same(r, f.output_arrays(), f"{what}, picture {i}")
+
+
+def cpu_baseline(vv, orc, seconds):
+    """The oracle on host cores: one thread on one 3840x2160 picture, and all
+    threads with one picture per thread, each repeated to about `seconds` of
+    CPU work."""
+    W, H = 3840, 2160
+    nt = host_threads()
+
+    def timed(n_pics, threads):
+        frames = [vv.SynthFrame(1, W, H, seed=1000 + i) for i in range(n_pics)]
+        arr = (C.POINTER(vv.Frame) * n_pics)(*[f.desc for f in frames])
+        reps, wall, cpu = 0, 0.0, 0.0
+        while cpu < seconds:
+            for f in frames:
+                f.reset()
+            t0, c0 = time.perf_counter(), time.process_time()
+            rc = orc.oracle_apply_in_loop_filters_mt(arr, n_pics, threads)
+            wall += time.perf_counter() - t0
+            cpu += time.process_time() - c0
+            reps += 1
+            assert rc == 0
+        for f in frames:
+            f.close()
+        return reps, wall, cpu
+
+    r1, w1, c1 = timed(1, 1)
+    rn, wn, cn = timed(nt, nt)
+    single = round(r1 * W * H / 1e6 / w1, 3)
+    allc = round(rn * nt * W * H / 1e6 / wn, 3)
+    return {"value": allc, "unit": "Mpixel/s", "cores": nt, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"{rn} x {nt} pictures 3840x2160 of the benchmark workload (one per thread), oracle C "
+                      f"restatement of the reference filters, {cn:.1f} s CPU over {wn:.1f} s wall",
+            "single_thread": {"value": single, "unit": "Mpixel/s", "cores": 1,
+                              "sample": f"{r1} x 1 picture 3840x2160, {c1:.1f} s CPU over {w1:.1f} s wall"}}
+
+
+def plain_run(vv, ctx, jobs, steps, warmup):
     """Per-picture kernel sequences (each job's captured graph replayed in
     turn), timed over `steps` passes; then a shorter pass with HIP events
     between the stage kernels for the per-stage split."""
@@ -109,13 +187,17 @@ def plain_run(vv, ctx, jobs, steps, warmup, W, H):
     return total_ms / steps, stage_ms
 
 
-def batched_run(vv, ctx, jobs, reps, warmup=2):
-    """Autotune, build the batch, time graph replays; returns dict."""
+def batched_run(vv, ctx, jobs, frames, refs, what, reps, warmup=2):
+    """Autotune, build the batch, check one graph replay against the oracle,
+    then time graph replays; returns a dict."""
     arr = (C.c_void_p * len(jobs))(*[j.h for j in jobs])
     tune = vv.Tune()
     vv.check(vv.lib.vvclf_autotune(ctx.h, arr, len(jobs), C.byref(tune)))
     b = C.c_void_p()
     vv.check(vv.lib.vvclf_batch_create(ctx.h, arr, len(jobs), C.byref(tune), C.byref(b)))
+    vv.check(vv.lib.vvclf_batch_graph_launch(b, ctx.stream))
+    ctx.synchronize()
+    verify_jobs(vv, jobs, frames, refs, f"{what} (fused {tune.tile_w}x{tune.tile_h})")
     ms = C.c_double()
     vv.check(vv.lib.vvclf_batch_time(b, ctx.stream, warmup, reps, 1, C.byref(ms)))
     vv.lib.vvclf_batch_destroy(b)
@@ -123,7 +205,8 @@ def batched_run(vv, ctx, jobs, reps, warmup=2):
     return {"ms_per_launch": round(ms.value, 4), "luma_pixels_per_launch": px,
             "value": round(px / (ms.value / 1e3) / 1e6, 2), "unit": "Mpixel/s",
             "tile": f"{tune.tile_w}x{tune.tile_h}", "wgs_per_cu": tune.wgs_per_cu,
-            "hbm_algorithmic_GBs": round(CHAIN_BYTES_PER_PX * px / (ms.value / 1e3) / 1e9, 2)}
+            "hbm_algorithmic_GBs": round(CHAIN_BYTES_PER_PX * px / (ms.value / 1e3) / 1e9, 2),
+            "checked_against_oracle": True}
 
 
 SERVE_STAGES = ["admission", "host_tables_and_upload", "wait_for_launch", "device", "download"]
@@ -134,12 +217,13 @@ VALU_FILE = os.path.join(ROOT, "profiles", "valu_instr.json")
 VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
 
 
-def profile_entry(path, kernel):
-    """The entry of a per-kernel rocprofv3 summary in profiles/ whose name
-    starts with `kernel` (template arguments may follow), or None."""
+def profile_entry(path, workload, kernel):
+    """The entry of a per-kernel rocprofv3 summary in profiles/ for
+    `workload` whose name starts with `kernel` (template arguments may
+    follow), or None."""
     try:
         with open(path) as fh:
-            d = json.load(fh)
+            d = json.load(fh).get(workload, {})
     except (OSError, ValueError):
         return None
     for k, v in d.items():
@@ -149,17 +233,18 @@ def profile_entry(path, kernel):
     return None
 
 
-def traffic_for(kernel):
-    """HBM bytes per dispatch of `kernel` on this workload (FETCH_SIZE + WRITE_SIZE, calibrated), from the
-    rocprofv3 summary in profiles/, or None when not profiled."""
-    v = profile_entry(TRAFFIC_FILE, kernel)
+def traffic_for(workload, kernel):
+    """HBM bytes per dispatch of `kernel` on `workload` (FETCH_SIZE + WRITE_SIZE,
+    calibrated), from the rocprofv3 summary in profiles/, or None when not
+    profiled."""
+    v = profile_entry(TRAFFIC_FILE, workload, kernel)
     return int(round(v["read_bytes"] + v["write_bytes"])) if v else None
 
 
-def valu_for(kernel, avg_launch_ms):
+def valu_for(workload, kernel, avg_launch_ms):
     """The kernel's vector-ALU issue rate against the VALU ceiling: SQ_INSTS_VALU
     per dispatch (rocprofv3 summary in profiles/) over the live launch time."""
-    v = profile_entry(VALU_FILE, kernel)
+    v = profile_entry(VALU_FILE, workload, kernel)
     if not v or not avg_launch_ms:
         return None
     rate = v["valu_wave_instr"] / (avg_launch_ms / 1e3)
@@ -168,9 +253,21 @@ def valu_for(kernel, avg_launch_ms):
             "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
 
 
-def served_run(vv, ctx, seconds, rate, n_streams=32, seed=7):
-    """Poisson arrivals at `rate` pictures/s over `n_streams` streams."""
-    sizes = [(832, 480), (1920, 1080), (3840, 2160)]
+def fused_roofline(bt, workload, px):
+    kern = "k_fused<%d, %d" % tuple(int(v) for v in bt["tile"].split("x"))
+    return {"bound": "hbm", "kernel": "k_fused", "achieved": bt["hbm_algorithmic_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(bt["hbm_algorithmic_GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic_for(workload, kern), "algorithmic_bytes_per_launch": CHAIN_BYTES_PER_PX * px,
+            "avg_launch_ms": bt["ms_per_launch"], "valu": valu_for(workload, kern, bt["ms_per_launch"])}
+
+
+SERVE_SIZES = [(832, 480), (1920, 1080), (3840, 2160)]
+
+
+def served_run(vv, ctx, ref, seconds, rate, n_streams=32, seed=7, max_jobs=None, verify=True):
+    """Poisson arrivals at `rate` pictures/s over `n_streams` streams (two
+    pictures per stream, reused); each picture's final output is compared with
+    the oracle's at the end."""
     rnd = random.Random(seed)
     cfg = vv.ServerConfig()
     cfg.hbm_budget_bytes = 64 << 30
@@ -179,21 +276,24 @@ def served_run(vv, ctx, seconds, rate, n_streams=32, seed=7):
     cfg.prep_threads = 6
     s = C.c_void_p()
     vv.check(vv.lib.vvclf_server_create(ctx.h, C.byref(cfg), C.byref(s)))
-    streams = []
+    streams, specs = [], []
     for k in range(n_streams):
-        w, h = sizes[rnd.randrange(3)]
+        w, h = SERVE_SIZES[rnd.randrange(3)]
         sid = C.c_uint32()
         vv.check(vv.lib.vvclf_server_open_stream(s, C.byref(sid)))
         # two pictures per stream sharing the stream's APS / LMCS state
-        frames = [vv.SynthFrame(4, w, h, seed=5000 + 10 * k + i, aps_seed=900 + k) for i in range(2)]
+        sp = [(4, w, h, 5000 + 10 * k + i, {"aps_seed": 900 + k}) for i in range(2)]
+        frames = [vv.SynthFrame(c, ww, hh, seed=sd, **kw) for c, ww, hh, sd, kw in sp]
         streams.append((sid.value, frames))
-    n_jobs = int(seconds * rate)
-    tickets, busy = [], {}
+        specs.extend(zip(frames, sp))
+    n_jobs = int(seconds * rate) if max_jobs is None else max_jobs
+    busy, used = {}, set()
     t_next = time.perf_counter()
     for i in range(n_jobs):
-        t_next += rnd.expovariate(rate)
-        while time.perf_counter() < t_next:
-            time.sleep(max(0.0, min(0.001, t_next - time.perf_counter())))
+        if rate:
+            t_next += rnd.expovariate(rate)
+            while time.perf_counter() < t_next:
+                time.sleep(max(0.0, min(0.001, t_next - time.perf_counter())))
         sid, frames = streams[rnd.randrange(n_streams)]
         f = frames[i % 2]
         prev = busy.get(id(f))
@@ -203,18 +303,25 @@ def served_run(vv, ctx, seconds, rate, n_streams=32, seed=7):
         t = C.c_uint64()
         vv.check(vv.lib.vvclf_server_submit(s, sid, f.desc, C.byref(t)))
         busy[id(f)] = t.value
-        tickets.append(t.value)
+        used.add(id(f))
     vv.check(vv.lib.vvclf_server_drain(s, 120000))
     st = vv.ServerStats()
     vv.check(vv.lib.vvclf_server_get_stats(s, C.byref(st)))
     vv.lib.vvclf_server_destroy(s)
-    return {"streams": n_streams, "offered_pictures_per_s": rate, "pictures": int(st.completed),
+    if verify:
+        todo = [(f, sp) for f, sp in specs if id(f) in used]
+        refs = ref.outputs([sp for _, sp in todo])
+        for (f, sp), r in zip(todo, refs):
+            same(r, f.output_arrays(), f"served picture {sp}")
+    return {"streams": n_streams, "offered_pictures_per_s": rate if rate else "saturation", "pictures": int(st.completed),
             "failed": int(st.failed), "launches": int(st.launches),
+            "pictures_per_launch": round(st.completed / st.launches, 2) if st.launches else None,
             "throughput_Mpixel_s": round(st.luma_pixels / st.wall_s / 1e6, 2) if st.wall_s else None,
             "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3), "max_ms": round(st.max_ms, 3),
             "admitted_bytes_peak": int(st.bytes_admitted_peak), "hbm_budget_bytes": int(cfg.hbm_budget_bytes),
             "stage_p50_ms": {k: round(st.stage_p50_ms[i], 3) for i, k in enumerate(SERVE_STAGES)},
             "stage_p99_ms": {k: round(st.stage_p99_ms[i], 3) for i, k in enumerate(SERVE_STAGES)},
+            "checked_against_oracle": bool(verify),
             "latency": "submit -> filtered samples back in host memory (includes host table derivation, "
                        "H2D and D2H copies)"}
 
@@ -227,13 +334,15 @@ def main():
     ap.add_argument("--pictures", type=int, default=8)
     ap.add_argument("--width", type=int, default=3840)
     ap.add_argument("--height", type=int, default=2160)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-mixed64", action="store_true")
+    ap.add_argument("--no-8k", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-rate", type=float, default=120.0)
+    ap.add_argument("--saturation-jobs", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (pictures shard by stream "
@@ -241,18 +350,27 @@ def main():
         sys.exit(2)
 
     vv = load("vvclf_amd", os.path.join(ROOT, "oxideav-h266_amd", "__init__.py"))
+    orc = load_oracle(vv)
+    ref = Reference(vv, orc)
     W, H, N = args.width, args.height, args.pictures
-    log("generating workload")
-    frames = [vv.SynthFrame(1, W, H, seed=1000 + i) for i in range(N)]
+    log("generating workload and its oracle outputs")
+    specs = [(1, W, H, 1000 + i, {}) for i in range(N)]
+    refs = ref.outputs(specs)
+    frames = [vv.SynthFrame(c, w, h, seed=s, **kw) for c, w, h, s, kw in specs]
     ctx = vv.Context(0)
     jobs = [vv.Job(ctx, f.desc) for f in frames]
     log("plain form")
-    ms_per_step, stage_ms = plain_run(vv, ctx, jobs, args.steps, args.warmup, W, H)
+    for j in jobs:
+        j.graph_launch(ctx.stream)
+    ctx.synchronize()
+    verify_jobs(vv, jobs, frames, refs, "plain form")
+    ms_per_step, stage_ms = plain_run(vv, ctx, jobs, args.steps, args.warmup)
     px = N * W * H
     value = px / (ms_per_step / 1e3) / 1e6
     dom = max(stage_ms, key=stage_ms.get)
     dom_bytes = STAGE_BYTES_PER_PX[dom] * W * H
     achieved = dom_bytes / (stage_ms[dom] / 1e3) / 1e9
+    wl = f"{N}x{W}x{H}"
     out = {
         "metric": "filtered Mpixel/s 4:2:0 10-bit (batched/served vs plain); p99 latency; HBM GB/s",
         "value": round(value, 2),
@@ -269,28 +387,23 @@ def main():
         "config": {"workload": f"{N}x {W}x{H} 4:2:0 10-bit, full chain (LMCS inverse, deblock incl. long filters, "
                                f"SAO, ALF 4 luma APS + chroma ALF + CC-ALF); plain per-picture stage kernels",
                    "pictures_per_step": N, "luma_pixels_per_step": px, "form": "plain"},
+        "checked_against_oracle": True,
         "chain_hbm_algorithmic_GBs": round(CHAIN_BYTES_PER_PX * px / (ms_per_step / 1e3) / 1e9, 2),
         "stage_ms_per_picture": {k: round(v, 5) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_for(KERNEL_NAMES[dom]),
+                     "traffic": traffic_for(wl, KERNEL_NAMES[dom]),
                      "algorithmic_bytes_per_launch": dom_bytes,
                      "avg_launch_ms": round(stage_ms[dom], 5),
-                     "valu": valu_for(KERNEL_NAMES[dom], stage_ms[dom])},
+                     "valu": valu_for(wl, KERNEL_NAMES[dom], stage_ms[dom])},
     }
+    out["batched"] = {}
     if not args.no_batched:
         log("batched form, same pictures")
-        bt = batched_run(vv, ctx, jobs, reps=max(5, args.steps // 2))
+        bt = batched_run(vv, ctx, jobs, frames, refs, "fused batch 8x4K", reps=max(5, args.steps // 2))
         bt["speedup_vs_plain"] = round(bt["value"] / value, 2)
-        bt["roofline"] = {"bound": "hbm", "kernel": "k_fused", "achieved": bt["hbm_algorithmic_GBs"],
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(bt["hbm_algorithmic_GBs"] / HBM_PEAK_GBS, 4),
-                          "traffic": traffic_for("k_fused<%d, %d" % tuple(int(v) for v in bt["tile"].split("x"))),
-                          "algorithmic_bytes_per_launch": CHAIN_BYTES_PER_PX * px,
-                          "avg_launch_ms": bt["ms_per_launch"],
-                          "valu": valu_for("k_fused<%d, %d" % tuple(int(v) for v in bt["tile"].split("x")),
-                                           bt["ms_per_launch"])}
-        out["batched"] = {"same_8x4k": bt}
+        bt["roofline"] = fused_roofline(bt, wl, px)
+        out["batched"]["same_8x4k"] = bt
     for j in jobs:
         j.close()
     jobs = []
@@ -300,36 +413,70 @@ def main():
         log("drop-in host-to-host")
         ts = []
         for rep in range(2):
-            for f in frames:
+            for f, r in zip(frames, refs):
                 f.reset()
                 t0 = time.perf_counter()
                 vv.check(vv.lib.vvclf_apply_in_loop_filters(f.desc))
                 ts.append(time.perf_counter() - t0)
+                if rep == 0:
+                    same(r, f.output_arrays(), "drop-in call")
         ts = ts[len(frames):]  # first pass warms the pools and staging buffers
         out["dropin_host_to_host"] = {"value": round(N * W * H / sum(ts) / 1e6, 2), "unit": "Mpixel/s",
                                       "ms_per_picture": round(1e3 * sum(ts) / len(ts), 3),
+                                      "checked_against_oracle": True,
                                       "note": "vvclf_apply_in_loop_filters per picture: host table derivation, "
                                               "upload, plain kernels, download"}
+    for f in frames:
+        f.close()
+    frames, refs = [], []
     if not args.no_batched and not args.no_mixed64:
         log("batched form, 64 mixed pictures")
         sizes = [(416, 240)] * 16 + [(832, 480)] * 16 + [(1920, 1080)] * 16 + [(3840, 2160)] * 16
-        mframes = [vv.SynthFrame(2, w, h, seed=2000 + i) for i, (w, h) in enumerate(sizes)]
+        mspecs = [(2, w, h, 2000 + i, {}) for i, (w, h) in enumerate(sizes)]
+        mrefs = ref.outputs(mspecs)
+        mframes = [vv.SynthFrame(c, w, h, seed=s, **kw) for c, w, h, s, kw in mspecs]
         mjobs = [vv.Job(ctx, f.desc) for f in mframes]
-        bm = batched_run(vv, ctx, mjobs, reps=5)
+        bm = batched_run(vv, ctx, mjobs, mframes, mrefs, "fused batch, 64 mixed", reps=5)
         # plain per-picture reference point on the same 64 pictures
-        ms_plain, _ = plain_run(vv, ctx, mjobs, 3, 1, 0, 0)
+        ms_plain, _ = plain_run(vv, ctx, mjobs, 3, 1)
         bm["plain_value"] = round(bm["luma_pixels_per_launch"] / (ms_plain / 1e3) / 1e6, 2)
         bm["speedup_vs_plain"] = round(bm["value"] / bm["plain_value"], 2)
+        bm["roofline"] = fused_roofline(bm, "64 mixed", bm["luma_pixels_per_launch"])
         out["batched"]["mixed_64"] = bm
         for j in mjobs:
             j.close()
+        for f in mframes:
+            f.close()
+        mrefs = None
+    if not args.no_batched and not args.no_8k:
+        log("batched form, 4x 7680x4320 ALF-heavy")
+        especs = [(3, 7680, 4320, 6000 + i, {}) for i in range(4)]
+        erefs = ref.outputs(especs)
+        eframes = [vv.SynthFrame(c, w, h, seed=s, **kw) for c, w, h, s, kw in especs]
+        ejobs = [vv.Job(ctx, f.desc) for f in eframes]
+        be = batched_run(vv, ctx, ejobs, eframes, erefs, "fused batch, 4x 8K", reps=10)
+        ms_plain, _ = plain_run(vv, ctx, ejobs, 3, 1)
+        be["plain_value"] = round(be["luma_pixels_per_launch"] / (ms_plain / 1e3) / 1e6, 2)
+        be["speedup_vs_plain"] = round(be["value"] / be["plain_value"], 2)
+        be["roofline"] = fused_roofline(be, "4x7680x4320", be["luma_pixels_per_launch"])
+        be["workload"] = ("4x 7680x4320 4:2:0 10-bit, every CTB luma ALF + CC-ALF, 8 luma APS sets, minimum CU 4x8, "
+                          "bS 2 at p 0.5")
+        out["batched"]["alf_heavy_4x8k"] = be
+        for j in ejobs:
+            j.close()
+        for f in eframes:
+            f.close()
+        erefs = None
     if args.serve_seconds > 0:
         log("serving")
-        out["served"] = served_run(vv, ctx, args.serve_seconds, args.serve_rate)
+        out["served"] = served_run(vv, ctx, ref, args.serve_seconds, args.serve_rate)
+        if args.saturation_jobs > 0:
+            log("serving at saturation")
+            out["served_saturation"] = served_run(vv, ctx, ref, 0, 0, max_jobs=args.saturation_jobs, seed=11)
     ctx.close()
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(vv, N, W, H, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(vv, orc, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -434,7 +434,9 @@ typedef struct vvclf_server_stats {
   uint64_t luma_pixels;        /* of completed jobs */
   uint64_t launches;           /* fused launches issued */
   uint64_t bytes_admitted_peak;
-  double p50_ms, p99_ms, max_ms, mean_ms; /* submit -> result in the caller's planes */
+  /* submit -> result in the caller's planes; quantiles come from 0.5 %-wide
+   * log-spaced buckets over every completed job */
+  double p50_ms, p99_ms, max_ms, mean_ms;
   double wall_s;               /* first submit -> last completion */
   /* median time per stage of completed jobs: [0] waiting for admission,
    * [1] host table derivation + upload, [2] waiting for a launch, [3] on the
@@ -458,9 +460,14 @@ int vvclf_server_close_stream(vvclf_server* srv, uint32_t stream_id);
  * until the ticket completes; the result is written into frame->pic.
  * VVCLF_ERR_BUDGET if the job alone exceeds the budget. */
 int vvclf_server_submit(vvclf_server* srv, uint32_t stream_id, vvclf_frame* frame, uint64_t* ticket);
-/* VVCLF_OK when done (or the job's own error), VVCLF_ERR_AGAIN if pending. */
+/* VVCLF_OK when done (or the job's own error), VVCLF_ERR_AGAIN if pending.
+ * A ticket's final status is reported once: the poll or wait that returns it
+ * releases the ticket, and later queries of it return VVCLF_ERR_INVALID.
+ * Finished tickets that nobody queries are released oldest first once more
+ * than 65536 of them are held. */
 int vvclf_server_poll(vvclf_server* srv, uint64_t ticket);
-/* Blocks until done or timeout: VVCLF_OK / the job's error / VVCLF_ERR_TIMEOUT. */
+/* Blocks until done or timeout: VVCLF_OK / the job's error / VVCLF_ERR_TIMEOUT
+ * (a timeout keeps the ticket). */
 int vvclf_server_wait(vvclf_server* srv, uint64_t ticket, uint32_t timeout_ms);
 int vvclf_server_drain(vvclf_server* srv, uint32_t timeout_ms);
 int vvclf_server_get_stats(vvclf_server* srv, vvclf_server_stats* out);

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -117,6 +118,38 @@ struct inflight {
   int slot = 0;
 };
 
+// Latency distribution in log-spaced buckets (0.5 % wide, 1 us .. 100 s):
+// fixed size however many jobs a server finishes, and cheap to copy.
+struct lat_hist {
+  static constexpr double kMinMs = 1e-3;
+  static constexpr double kRatio = 1.005;
+  static constexpr int kBuckets = 3700;
+  uint32_t counts[kBuckets] = {};
+  uint64_t n = 0;
+  double sum = 0, max = 0;
+  static int bucket(double ms) {
+    if (!(ms > kMinMs)) return 0;
+    int b = (int)(std::log(ms / kMinMs) / std::log(kRatio));
+    return b < 0 ? 0 : (b >= kBuckets ? kBuckets - 1 : b);
+  }
+  void add(double ms) {
+    counts[bucket(ms)]++;
+    n++;
+    sum += ms;
+    max = std::max(max, ms);
+  }
+  // the q-quantile (nearest rank), as its bucket's geometric centre
+  double quantile(double q) const {
+    if (!n) return 0;
+    uint64_t rank = (uint64_t)(q * (double)(n - 1) + 0.5) + 1, acc = 0;
+    for (int b = 0; b < kBuckets; b++) {
+      acc += counts[b];
+      if (acc >= rank) return std::min(max, kMinMs * std::pow(kRatio, b + 0.5));
+    }
+    return max;
+  }
+};
+
 }  // namespace
 
 struct vvclf_server {
@@ -127,7 +160,12 @@ struct vvclf_server {
   std::deque<request*> pending;
   std::deque<request*> ready;
   std::deque<inflight> running;
-  std::unordered_map<uint64_t, request*> tickets;
+  // Requests by ticket.  A ticket is consumed by the poll or wait that
+  // reports its final status; finished tickets nobody claims are dropped
+  // oldest first beyond kMaxUnclaimed.
+  std::unordered_map<uint64_t, std::shared_ptr<request>> tickets;
+  std::deque<uint64_t> finished_order;
+  static constexpr size_t kMaxUnclaimed = 1u << 16;
   std::vector<std::unique_ptr<srv_stream>> streams;
   uint64_t next_ticket = 1;
   uint64_t bytes_admitted = 0, bytes_peak = 0;
@@ -143,11 +181,10 @@ struct vvclf_server {
   hipEvent_t down_done[2] = {nullptr, nullptr};
   hipStream_t s_comp = nullptr, s_down = nullptr;
   launch_slot slots[2];
-  int next_slot = 0;
   // stats
   uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0, pixels = 0, launches = 0;
-  std::vector<double> lat_ms;
-  std::vector<double> stage_ms[5];
+  lat_hist lat;
+  lat_hist stage[5];
   clk::time_point first_submit, last_done;
   bool any_submit = false;
 };
@@ -176,17 +213,27 @@ void finish(vvclf_server* s, request* r, int status) {
     s->completed++;
     s->pixels += (uint64_t)r->frame->pic.luma.width * r->frame->pic.luma.height;
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-    s->lat_ms.push_back(ms(r->t_submit, r->t_done));
-    s->stage_ms[0].push_back(ms(r->t_submit, r->t_prep));
-    s->stage_ms[1].push_back(ms(r->t_prep, r->t_ready));
-    s->stage_ms[2].push_back(ms(r->t_ready, r->t_launch));
-    s->stage_ms[3].push_back(ms(r->t_launch, r->t_device));
-    s->stage_ms[4].push_back(ms(r->t_device, r->t_done));
+    s->lat.add(ms(r->t_submit, r->t_done));
+    s->stage[0].add(ms(r->t_submit, r->t_prep));
+    s->stage[1].add(ms(r->t_prep, r->t_ready));
+    s->stage[2].add(ms(r->t_ready, r->t_launch));
+    s->stage[3].add(ms(r->t_launch, r->t_device));
+    s->stage[4].add(ms(r->t_device, r->t_done));
   } else {
     s->failed++;
   }
+  // bound the finished tickets nobody has claimed (consumed ones are skipped)
+  s->finished_order.push_back(r->ticket);
+  while (s->finished_order.size() > vvclf_server::kMaxUnclaimed) {
+    s->tickets.erase(s->finished_order.front());
+    s->finished_order.pop_front();
+  }
   s->cv_done.notify_all();
 }
+
+// Drops a finished ticket whose final status has just been reported (caller
+// holds s->mu).
+void consume(vvclf_server* s, uint64_t ticket) { s->tickets.erase(ticket); }
 
 int ensure_resident(vvclf_server* s, srv_stream& st, const vvclf_frame& f, hipStream_t ps,
                     std::shared_ptr<res_obj>& out) {
@@ -341,8 +388,8 @@ void launch_loop(vvclf_server* s) {
         fl.reqs.push_back(s->ready.front());
         s->ready.pop_front();
       }
-      fl.slot = s->next_slot;
-      s->next_slot ^= 1;
+      // at most one launch is in flight here: take the slot it does not use
+      fl.slot = s->running.empty() ? 0 : (s->running.back().slot ^ 1);
     }
     // tile list, largest pictures first
     std::vector<uint32_t> order(fl.reqs.size());
@@ -557,7 +604,7 @@ int vvclf_server_destroy(vvclf_server* s) {
   for (auto ps : s->prep_streams) hipStreamDestroy(ps);
   hipStreamDestroy(s->s_comp);
   hipStreamDestroy(s->s_down);
-  for (auto& kv : s->tickets) delete kv.second;
+  s->tickets.clear();
   delete s;
   return VVCLF_OK;
 }
@@ -580,17 +627,16 @@ int vvclf_server_close_stream(vvclf_server* s, uint32_t id) {
 
 int vvclf_server_submit(vvclf_server* s, uint32_t id, vvclf_frame* frame, uint64_t* ticket) {
   if (!s || !frame || !ticket) return fail(VVCLF_ERR_INVALID, "submit: NULL argument");
-  request* r = new request();
+  auto sp = std::make_shared<request>();
+  request* r = sp.get();
   r->frame = frame;
   r->stream = id;
   r->t_submit = clk::now();
   std::lock_guard<std::mutex> g(s->mu);
-  if (id >= s->streams.size() || !s->streams[id]->open) {
-    delete r;
+  if (id >= s->streams.size() || !s->streams[id]->open)
     return fail(VVCLF_ERR_INVALID, "submit: stream %u is not open", id);
-  }
   r->ticket = s->next_ticket++;
-  s->tickets[r->ticket] = r;
+  s->tickets[r->ticket] = std::move(sp);
   if (!s->any_submit) {
     s->first_submit = r->t_submit;
     s->any_submit = true;
@@ -606,18 +652,22 @@ int vvclf_server_poll(vvclf_server* s, uint64_t t) {
   if (!s) return fail(VVCLF_ERR_INVALID, "poll: NULL");
   std::lock_guard<std::mutex> g(s->mu);
   auto it = s->tickets.find(t);
-  if (it == s->tickets.end()) return fail(VVCLF_ERR_INVALID, "poll: unknown ticket");
-  return it->second->done ? it->second->status : VVCLF_ERR_AGAIN;
+  if (it == s->tickets.end()) return fail(VVCLF_ERR_INVALID, "poll: unknown or already reported ticket");
+  if (!it->second->done) return VVCLF_ERR_AGAIN;
+  int st = it->second->status;
+  consume(s, t);
+  return st;
 }
 
 int vvclf_server_wait(vvclf_server* s, uint64_t t, uint32_t timeout_ms) {
   if (!s) return fail(VVCLF_ERR_INVALID, "wait: NULL");
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->tickets.find(t);
-  if (it == s->tickets.end()) return fail(VVCLF_ERR_INVALID, "wait: unknown ticket");
-  request* r = it->second;
+  if (it == s->tickets.end()) return fail(VVCLF_ERR_INVALID, "wait: unknown or already reported ticket");
+  std::shared_ptr<request> r = it->second;  // alive even if another caller consumes the ticket meanwhile
   if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return r->done; }))
     return VVCLF_ERR_TIMEOUT;
+  consume(s, t);
   return r->status;
 }
 
@@ -632,33 +682,32 @@ int vvclf_server_drain(vvclf_server* s, uint32_t timeout_ms) {
 
 int vvclf_server_get_stats(vvclf_server* s, vvclf_server_stats* o) {
   if (!s || !o) return fail(VVCLF_ERR_INVALID, "stats: NULL");
-  std::lock_guard<std::mutex> g(s->mu);
+  // copy the fixed-size histograms under the lock, reduce them outside it
+  auto h = std::make_unique<lat_hist[]>(6);
   memset(o, 0, sizeof(*o));
-  o->submitted = s->submitted;
-  o->completed = s->completed;
-  o->failed = s->failed;
-  o->rejected = s->rejected;
-  o->luma_pixels = s->pixels;
-  o->launches = s->launches;
-  o->bytes_admitted_peak = s->bytes_peak;
-  std::vector<double> l = s->lat_ms;
-  if (!l.empty()) {
-    std::sort(l.begin(), l.end());
-    auto pct = [&](double q) { return l[std::min(l.size() - 1, (size_t)(q * (l.size() - 1) + 0.5))]; };
-    o->p50_ms = pct(0.50);
-    o->p99_ms = pct(0.99);
-    o->max_ms = l.back();
-    double sum = 0;
-    for (double v : l) sum += v;
-    o->mean_ms = sum / l.size();
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    o->submitted = s->submitted;
+    o->completed = s->completed;
+    o->failed = s->failed;
+    o->rejected = s->rejected;
+    o->luma_pixels = s->pixels;
+    o->launches = s->launches;
+    o->bytes_admitted_peak = s->bytes_peak;
+    if (s->any_submit && s->completed)
+      o->wall_s = std::chrono::duration<double>(s->last_done - s->first_submit).count();
+    h[0] = s->lat;
+    for (int k = 0; k < 5; k++) h[k + 1] = s->stage[k];
   }
-  if (s->any_submit && s->completed) o->wall_s = std::chrono::duration<double>(s->last_done - s->first_submit).count();
+  if (h[0].n) {
+    o->p50_ms = h[0].quantile(0.50);
+    o->p99_ms = h[0].quantile(0.99);
+    o->max_ms = h[0].max;
+    o->mean_ms = h[0].sum / (double)h[0].n;
+  }
   for (int k = 0; k < 5; k++) {
-    std::vector<double> v = s->stage_ms[k];
-    if (v.empty()) continue;
-    std::sort(v.begin(), v.end());
-    o->stage_p50_ms[k] = v[v.size() / 2];
-    o->stage_p99_ms[k] = v[std::min(v.size() - 1, (size_t)(0.99 * (v.size() - 1) + 0.5))];
+    o->stage_p50_ms[k] = h[k + 1].quantile(0.50);
+    o->stage_p99_ms[k] = h[k + 1].quantile(0.99);
   }
   return VVCLF_OK;
 }
