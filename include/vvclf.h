@@ -309,6 +309,22 @@ int vvclf_lmcs_inverse_luma(vvclf_plane* luma, const vvclf_lmcs_binding* binding
  * the frame descriptor.  Order: LMCS inverse, deblock, SAO, ALF + CC-ALF. */
 int vvclf_apply_in_loop_filters(vvclf_frame* frame);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture around the call, tight serving loops,
+ * harnesses that forbid hipMalloc): the same result as
+ * vvclf_apply_in_loop_filters, with every device buffer of the call (three
+ * sample sets and the per-picture side tables) taken from a fixed scratch
+ * array of VVCLF_NOALLOC_SCRATCH_BYTES that each GPU holds until the process
+ * ends, and samples moved by pageable copies (no pinned buffer either); host
+ * working memory of the table derivation still comes from the C++ heap.
+ * Runs on the calling thread's current device.  Pictures whose sample sets
+ * and tables do not fit (e.g. above 3840x2160 4:2:0) return
+ * VVCLF_ERR_UNSUPPORTED before any device work.  Calls on one device are
+ * serialised by a lock, since they share the array; each call writes every
+ * byte it later reads. */
+#define VVCLF_NOALLOC_SCRATCH_BYTES (96ull << 20)
+int vvclf_apply_in_loop_filters_noalloc(vvclf_frame* frame);
+
 /* Replaces `derive_lmcs(data, bit_depth)`, src/lmcs.rs:302-404 (host-side). */
 int vvclf_derive_lmcs(const vvclf_lmcs_data* data, uint32_t bit_depth, vvclf_lmcs_derived* out);
 

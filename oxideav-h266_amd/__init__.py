@@ -208,6 +208,7 @@ SIGNATURES = {
     "vvclf_apply_alf_clipped": (C.c_int, [P(Picture), P(AlfPicture), P(AlfConfig), P(AlfApsBinding), P(Bounds)]),
     "vvclf_lmcs_inverse_luma": (C.c_int, [P(Plane), P(LmcsBinding)]),
     "vvclf_apply_in_loop_filters": (C.c_int, [P(Frame)]),
+    "vvclf_apply_in_loop_filters_noalloc": (C.c_int, [P(Frame)]),
     "vvclf_derive_lmcs": (C.c_int, [P(LmcsData), u32, P(LmcsDerived)]),
     "vvclf_frame_check": (C.c_int, [P(Frame), P(u64)]),
     "vvclf_last_error": (C.c_char_p, []),

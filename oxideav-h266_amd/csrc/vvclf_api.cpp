@@ -210,24 +210,39 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
     delete j;
     return hip_fail(e, "hipSetDevice");
   }
-  // sample planes: in, a, b, out
+  // sample planes: in, a, b, out (with a caller's arena: in, a, b, and out
+  // sharing in's memory)
+  const bool arena = opts && opts->arena;
   size_t luma_elems = (size_t)d.py * H, chroma_elems = (size_t)d.pc * Hc;
   size_t set_bytes = (luma_elems + 2 * chroma_elems) * 2;
   set_bytes = (set_bytes + 255) / 256 * 256;
-  j->bytes_planes = set_bytes * 4;
+  const int n_sets = arena ? 3 : 4;
+  j->bytes_planes = set_bytes * n_sets;
   j->pool_stream = ps;
-  e = ps ? hipMallocAsync(&j->dev_planes, j->bytes_planes, ps) : hipMalloc(&j->dev_planes, j->bytes_planes);
-  if (e != hipSuccess) {
-    delete j;
-    return fail(VVCLF_ERR_NOMEM, "hipMalloc planes (%zu bytes): %s", (size_t)j->bytes_planes, hipGetErrorString(e));
+  j->arena_backed = arena;
+  if (arena) {
+    if (j->bytes_planes > opts->arena_bytes) {
+      const size_t need = j->bytes_planes;
+      delete j;
+      return fail(VVCLF_ERR_UNSUPPORTED, "picture needs %zu bytes of sample buffers, more than the %zu-byte scratch",
+                  need, opts->arena_bytes);
+    }
+    j->dev_planes = opts->arena;
+  } else {
+    e = ps ? hipMallocAsync(&j->dev_planes, j->bytes_planes, ps) : hipMalloc(&j->dev_planes, j->bytes_planes);
+    if (e != hipSuccess) {
+      delete j;
+      return fail(VVCLF_ERR_NOMEM, "hipMalloc planes (%zu bytes): %s", (size_t)j->bytes_planes, hipGetErrorString(e));
+    }
   }
   dev_planes* sets[4] = {&d.in, &d.a, &d.b, &d.out};
-  for (int s = 0; s < 4; s++) {
+  for (int s = 0; s < n_sets; s++) {
     uint8_t* base = (uint8_t*)j->dev_planes + s * set_bytes;
     sets[s]->y = (uint16_t*)base;
     sets[s]->cb = Wc ? (uint16_t*)(base + luma_elems * 2) : nullptr;
     sets[s]->cr = Wc ? (uint16_t*)(base + (luma_elems + chroma_elems) * 2) : nullptr;
   }
+  if (arena) d.out = d.in;
   if (!t.do_sao) d.b = d.a;
   // side tables
   side_builder sb;
@@ -276,11 +291,21 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
     }
   }
   j->bytes_side = gpu_dbk ? tail : host_side;
-  e = ps ? hipMallocAsync(&j->dev_side, j->bytes_side, ps) : hipMalloc(&j->dev_side, j->bytes_side);
-  if (e != hipSuccess) {
-    j->dev_side = nullptr;
-    vvclf_job_destroy(j);
-    return fail(VVCLF_ERR_NOMEM, "hipMalloc side tables: %s", hipGetErrorString(e));
+  if (arena) {
+    if (gpu_dbk || j->bytes_planes + j->bytes_side > opts->arena_bytes) {
+      const size_t need = j->bytes_planes + j->bytes_side;
+      vvclf_job_destroy(j);
+      return fail(VVCLF_ERR_UNSUPPORTED, "picture needs %zu bytes of sample buffers and tables, more than the "
+                  "%zu-byte scratch", need, opts->arena_bytes);
+    }
+    j->dev_side = opts->arena + j->bytes_planes;  // bytes_planes is a multiple of 256
+  } else {
+    e = ps ? hipMallocAsync(&j->dev_side, j->bytes_side, ps) : hipMalloc(&j->dev_side, j->bytes_side);
+    if (e != hipSuccess) {
+      j->dev_side = nullptr;
+      vvclf_job_destroy(j);
+      return fail(VVCLF_ERR_NOMEM, "hipMalloc side tables: %s", hipGetErrorString(e));
+    }
   }
   e = ps ? hipMemcpyAsync(j->dev_side, host_blob, host_side, hipMemcpyHostToDevice, ps)
          : hipMemcpy(j->dev_side, host_blob, host_side, hipMemcpyHostToDevice);
@@ -572,6 +597,10 @@ int vvclf_job_destroy(vvclf_job* j) {
   hipSetDevice(j->ctx->device);
   if (j->exec) hipGraphExecDestroy(j->exec);
   if (j->graph) hipGraphDestroy(j->graph);
+  if (j->arena_backed) {  // the caller's memory: nothing to free
+    delete j;
+    return VVCLF_OK;
+  }
   if (j->pool_stream) {  // back to the stream-ordered pool, no device-wide sync
     if (j->dev_planes) hipFreeAsync(j->dev_planes, j->pool_stream);
     if (j->dev_side) hipFreeAsync(j->dev_side, j->pool_stream);

@@ -120,6 +120,13 @@ struct job_opts {
   // caller sets it: the path is switched off until a device memory fault it
   // raised is explained
   bool gpu_derive = false;
+  // Device memory supplied by the caller instead of allocated: the job's
+  // sample sets (three of them, the output sharing the input's, since the
+  // input is dead after the first kernel of the plain form) and side tables
+  // are carved from [arena, arena + arena_bytes); job creation fails with
+  // VVCLF_ERR_UNSUPPORTED when they do not fit.
+  uint8_t* arena = nullptr;
+  size_t arena_bytes = 0;
 };
 
 // Device destinations of the GPU deblocking derivation, inside a job's side
@@ -198,6 +205,7 @@ struct vvclf_job {
   uint32_t stages;
   bool fused_out = false;  // last run was the fused form (result in `out`)
   hipStream_t pool_stream = nullptr;  // set when allocated from the stream-ordered pool
+  bool arena_backed = false;          // device memory belongs to the caller (job_opts::arena)
   hipGraph_t graph = nullptr;         // the plain sequence, captured on first graph launch
   hipGraphExec_t exec = nullptr;
   // host copies of the per-CTB ALF words and rectangles (tile records)
