@@ -328,6 +328,69 @@ int vvclf_apply_in_loop_filters_noalloc(vvclf_frame* frame);
 /* Replaces `derive_lmcs(data, bit_depth)`, src/lmcs.rs:302-404 (host-side). */
 int vvclf_derive_lmcs(const vvclf_lmcs_data* data, uint32_t bit_depth, vvclf_lmcs_derived* out);
 
+/* ------------------------------------------------------------------------ */
+/* The steps either side of the chain: LMCS forward mapping and chroma       */
+/* residual scaling during reconstruction (before the filters), and the      */
+/* conformance crop and plane hashes of the output (after them).             */
+/* ------------------------------------------------------------------------ */
+typedef struct vvclf_rect {
+  uint32_t x, y, w, h;
+} vvclf_rect;
+
+/* Replaces CtuWalker::lmcs_forward_map_luma_rect(out, x0, y0, w, h),
+ * src/ctu.rs:1701-1725, called once per rectangle (the inter CUs whose
+ * prediction is forward-mapped before their residual is added): every luma
+ * sample s inside a rectangle (clipped to the plane) becomes
+ * Clip3(0, 2^BitDepth - 1, forward_map_luma_sample(s)), src/lmcs.rs:186-195.
+ * Rectangles must not overlap.  No-op when binding->derived is NULL.  A
+ * sample of 2^BitDepth or more inside a rectangle (the reference indexes past
+ * its 16 pieces and panics) returns VVCLF_ERR_INVALID and leaves the plane
+ * unchanged. */
+int vvclf_lmcs_forward_luma(vvclf_plane* luma, const vvclf_lmcs_binding* binding, const vvclf_rect* rects,
+                            uint32_t n_rects);
+
+/* One chroma transform block reconstructed with LMCS chroma residual
+ * scaling. */
+typedef struct vvclf_crs_block {
+  uint32_t c_idx;        /* 1 Cb, 2 Cr */
+  uint32_t x, y, w, h;   /* the transform block, chroma samples */
+  uint32_t x_cu, y_cu;   /* (xCuCb, yCuCb): luma origin of the CU holding the sizeY-aligned corner */
+  uint8_t avail_l, avail_t, pad0, pad1; /* availL / availT of that CU */
+  uint32_t size_y;       /* sizeY = Min(CtbSizeY, 64) */
+  uint32_t pad2;
+  uint64_t res_offset;   /* index of the block's first residual in `residuals` (w * h, row-major) */
+} vvclf_crs_block;
+
+/* Replaces, per block, CtuWalker::lmcs_chroma_var_scale_for_cu
+ * (src/ctu.rs:1752-1815: invAvgLuma from the left column / top row of the
+ * luma plane, eqs. 1216/1217, then idxYInv and ChromaScaleCoeff), the residual
+ * fold lmcs_scale_chroma_residuals (src/ctu.rs:734-742, eqs. 1219/1220) and
+ * the add recSamples = Clip1(predSamples + resSamples) of the chroma TB: the
+ * chroma planes of `pic` hold the prediction on entry and the reconstruction
+ * on return; the luma plane (mapped-domain reconstructed luma) is read only.
+ * Blocks must not overlap (VVCLF_ERR_INVALID otherwise).  No-op when
+ * binding->derived is NULL. */
+int vvclf_lmcs_chroma_residual(vvclf_picture* pic, const vvclf_lmcs_binding* binding,
+                               const vvclf_crs_block* blocks, uint32_t n_blocks, const int32_t* residuals,
+                               uint64_t n_residuals);
+
+/* The conformance crop of a decoded picture, src/stream.rs:816-842: offsets
+ * (left, right, top, bottom) in chroma units of the conformance window, or
+ * NULL for none, give out = (left * SubWidthC, top * SubHeightC, width,
+ * height) in luma samples. */
+typedef struct vvclf_crop {
+  uint32_t x, y, w, h;
+} vvclf_crop;
+int vvclf_conformance_crop(uint32_t pic_w, uint32_t pic_h, uint32_t chroma_format_idc, const uint32_t* window,
+                           vvclf_crop* out);
+/* MD5 of each cropped plane in the conformance-stream convention,
+ * tests/jvet_conformance.rs:400-430: rows of the crop, one byte per sample at
+ * 8 bits, two bytes little-endian above; chroma crop = luma crop / 2 (4:2:0).
+ * md5[c] receives plane c (luma, Cb, Cr); chroma entries are left untouched
+ * when chroma_format_idc is 0.  Host-side. */
+int vvclf_picture_md5(const vvclf_picture* pic, uint32_t chroma_format_idc, uint32_t bit_depth,
+                      const vvclf_crop* crop, uint8_t md5[3][16]);
+
 /* ======================================================================== */
 /* Layer 2: device-resident jobs, batches and serving (no reference          */
 /* counterpart; the reference is single-threaded CPU code).                  */
@@ -404,6 +467,10 @@ int vvclf_job_download(vvclf_job* job, vvclf_picture* out);
  * pitch in samples. */
 int vvclf_job_output(vvclf_job* job, uint16_t** luma, uint16_t** cb, uint16_t** cr,
                      uint32_t* pitch_luma, uint32_t* pitch_chroma);
+/* vvclf_picture_md5 of the job's filtered output: the crop is cut and packed
+ * (one or two bytes per sample) on the device, so only the hashed bytes come
+ * back.  Waits for the job's work on the device. */
+int vvclf_job_output_md5(vvclf_job* job, uint32_t bit_depth, const vvclf_crop* crop, uint8_t md5[3][16]);
 /* Device bytes a job holds (side information + sample planes). */
 uint64_t vvclf_job_device_bytes(const vvclf_job* job);
 /* Luma pixels of the job's picture. */

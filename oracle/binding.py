@@ -41,6 +41,11 @@ def load(vv=None, path=None):
         "oracle_lmcs_inverse_map": (u32, [P(vv.LmcsDerived), u32, u32]),
         "oracle_lmcs_forward_map": (C.c_int64, [P(vv.LmcsDerived), u32]),
         "oracle_lmcs_inverse_luma": (None, [P(vv.Plane), P(vv.LmcsBinding)]),
+        "oracle_lmcs_forward_luma_rect": (C.c_int, [P(vv.Plane), P(vv.LmcsBinding), u32, u32, u32, u32]),
+        "oracle_lmcs_chroma_var_scale": (u32, [P(vv.Plane), P(vv.LmcsBinding), u32, u32, C.c_int, C.c_int, u32]),
+        "oracle_lmcs_scale_chroma_residual_sample": (u32, [P(vv.LmcsDerived), C.c_int64, C.c_int64, u32]),
+        "oracle_lmcs_chroma_residual": (C.c_int, [P(vv.Picture), P(vv.LmcsBinding), P(vv.CrsBlock), u32,
+                                                  P(C.c_int32)]),
         "oracle_beta_prime": (i32, [i32]),
         "oracle_tc_prime": (i32, [i32]),
         "oracle_scale_tc": (i32, [i32, u32]),

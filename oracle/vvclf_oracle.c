@@ -113,6 +113,95 @@ void oracle_lmcs_inverse_luma(vvclf_plane* p, const vvclf_lmcs_binding* b) {
     }
 }
 
+/* src/ctu.rs:1701-1725 — lmcs_forward_map_luma_rect: forward_map_luma_sample
+ * (src/lmcs.rs:186-195) clamped to [0, 2^BitDepth - 1] over a rectangle
+ * clipped to the plane.  A sample >= 2^BitDepth would index past the 16
+ * pieces (the reference panics): -1, nothing changed. */
+int oracle_lmcs_forward_luma_rect(vvclf_plane* p, const vvclf_lmcs_binding* b, uint32_t x0, uint32_t y0, uint32_t w,
+                                  uint32_t h) {
+  if (!b || !b->derived) return 0;
+  const vvclf_lmcs_derived* d = b->derived;
+  int64_t hi = ((int64_t)1 << d->bit_depth) - 1;
+  uint32_t y1 = y0 + h < p->height ? y0 + h : p->height, x1 = x0 + w < p->width ? x0 + w : p->width;
+  for (uint32_t y = y0; y < y1; y++)
+    for (uint32_t x = x0; x < x1; x++)
+      if (p->samples[(size_t)y * p->stride + x] > hi) return -1;
+  for (uint32_t y = y0; y < y1; y++)
+    for (uint32_t x = x0; x < x1; x++) {
+      uint16_t* s = &p->samples[(size_t)y * p->stride + x];
+      *s = (uint16_t)clampl(oracle_lmcs_forward_map(d, *s), 0, hi);
+    }
+  return 0;
+}
+
+/* src/ctu.rs:1790-1814 — varScale of a chroma block from the mapped-domain
+ * luma: left column x_cu - 1 / top row y_cu - 1 of sizeY samples (clamped to
+ * the plane), rounded mean (eq. 1216) or mid-grey (eq. 1217), idxYInv
+ * (src/lmcs.rs:209-220), ChromaScaleCoeff (eq. 1218). */
+uint32_t oracle_lmcs_chroma_var_scale(const vvclf_plane* l, const vvclf_lmcs_binding* b, uint32_t x_cu, uint32_t y_cu,
+                                      int avail_l, int avail_t, uint32_t size_y) {
+  const vvclf_lmcs_derived* d = b->derived;
+  uint64_t sum = 0;
+  uint32_t cnt = 0;
+  if (avail_l) {
+    for (uint32_t i = 0; i < size_y; i++) {
+      uint32_t yy = y_cu + i < l->height - 1 ? y_cu + i : l->height - 1;
+      sum += l->samples[(size_t)yy * l->stride + (x_cu - 1)];
+    }
+    cnt += size_y;
+  }
+  if (avail_t) {
+    for (uint32_t i = 0; i < size_y; i++) {
+      uint32_t xx = x_cu + i < l->width - 1 ? x_cu + i : l->width - 1;
+      sum += l->samples[(size_t)(y_cu - 1) * l->stride + xx];
+    }
+    cnt += size_y;
+  }
+  uint32_t avg;
+  if (cnt > 0) {
+    uint32_t sh = 0; /* cnt.trailing_zeros() */
+    while (!((cnt >> sh) & 1u)) sh++;
+    avg = (uint32_t)((sum + (cnt >> 1)) >> sh);
+  } else {
+    avg = 1u << (d->bit_depth - 1);
+  }
+  return d->chroma_scale_coeff[oracle_lmcs_idx_y_inv(d, avg, b->min_bin_idx, b->max_bin_idx)];
+}
+
+/* src/lmcs.rs:257-279 — eqs. 1219/1220. */
+uint32_t oracle_lmcs_scale_chroma_residual_sample(const vvclf_lmcs_derived* d, int64_t pred, int64_t res,
+                                                  uint32_t var_scale) {
+  int64_t lo = -((int64_t)1 << d->bit_depth), hi = ((int64_t)1 << d->bit_depth) - 1;
+  int64_t r = clampl(res, lo, hi);
+  int64_t scaled = 0;
+  if (r != 0) {
+    uint64_t mag = ((uint64_t)(r < 0 ? -r : r) * var_scale + (1u << 10)) >> 11;
+    scaled = r < 0 ? -(int64_t)mag : (int64_t)mag;
+  }
+  return (uint32_t)clampl(pred + scaled, 0, hi);
+}
+
+/* Per chroma TB: oracle_lmcs_chroma_var_scale, the residual fold of
+ * lmcs_scale_chroma_residuals (src/ctu.rs:734-742) and the
+ * Clip1(pred + res) add, in block order. */
+int oracle_lmcs_chroma_residual(vvclf_picture* pic, const vvclf_lmcs_binding* b, const vvclf_crs_block* blocks,
+                                uint32_t n, const int32_t* res) {
+  if (!b || !b->derived) return 0;
+  for (uint32_t k = 0; k < n; k++) {
+    const vvclf_crs_block* blk = &blocks[k];
+    vvclf_plane* c = blk->c_idx == 1 ? &pic->cb : &pic->cr;
+    uint32_t vs = oracle_lmcs_chroma_var_scale(&pic->luma, b, blk->x_cu, blk->y_cu, blk->avail_l, blk->avail_t,
+                                               blk->size_y);
+    for (uint32_t y = 0; y < blk->h; y++)
+      for (uint32_t x = 0; x < blk->w; x++) {
+        uint16_t* s = &c->samples[(size_t)(blk->y + y) * c->stride + blk->x + x];
+        *s = (uint16_t)oracle_lmcs_scale_chroma_residual_sample(b->derived, *s,
+                                                                res[blk->res_offset + (size_t)y * blk->w + x], vs);
+      }
+  }
+  return 0;
+}
+
 /* ======================================================================== */
 /* Deblocking                                                                */
 /* ======================================================================== */

@@ -28,6 +28,19 @@ int64_t oracle_lmcs_forward_map(const vvclf_lmcs_derived* d, uint32_t pred_sampl
 /* src/ctu.rs:11837-11849 */
 void oracle_lmcs_inverse_luma(vvclf_plane* luma, const vvclf_lmcs_binding* b);
 
+/* src/ctu.rs:1701-1725; -1 (the reference panics) for a sample >= 2^BitDepth */
+int oracle_lmcs_forward_luma_rect(vvclf_plane* p, const vvclf_lmcs_binding* b, uint32_t x0, uint32_t y0, uint32_t w,
+                                  uint32_t h);
+/* src/ctu.rs:1790-1814 */
+uint32_t oracle_lmcs_chroma_var_scale(const vvclf_plane* luma, const vvclf_lmcs_binding* b, uint32_t x_cu,
+                                      uint32_t y_cu, int avail_l, int avail_t, uint32_t size_y);
+/* src/lmcs.rs:257-279 */
+uint32_t oracle_lmcs_scale_chroma_residual_sample(const vvclf_lmcs_derived* d, int64_t pred, int64_t res,
+                                                  uint32_t var_scale);
+/* src/ctu.rs:734-742 and the chroma TB add, per block */
+int oracle_lmcs_chroma_residual(vvclf_picture* pic, const vvclf_lmcs_binding* b, const vvclf_crs_block* blocks,
+                                uint32_t n, const int32_t* res);
+
 /* src/deblock.rs:424-452 */
 int32_t oracle_beta_prime(int32_t q);
 int32_t oracle_tc_prime(int32_t q);

@@ -150,6 +150,21 @@ class LmcsBinding(C.Structure):
                 ("pad1", u8)]
 
 
+class Rect(C.Structure):
+    _fields_ = [("x", u32), ("y", u32), ("w", u32), ("h", u32)]
+
+
+class CrsBlock(C.Structure):
+    """One chroma TB with LMCS chroma residual scaling (src/ctu.rs:1752-1815)."""
+    _fields_ = [("c_idx", u32), ("x", u32), ("y", u32), ("w", u32), ("h", u32), ("x_cu", u32), ("y_cu", u32),
+                ("avail_l", u8), ("avail_t", u8), ("pad0", u8), ("pad1", u8), ("size_y", u32), ("pad2", u32),
+                ("res_offset", u64)]
+
+
+class Crop(C.Structure):
+    _fields_ = [("x", u32), ("y", u32), ("w", u32), ("h", u32)]
+
+
 class Frame(C.Structure):
     """Flattened walker state of apply_in_loop_filters_with_alf, src/ctu.rs:11822-12066."""
     _fields_ = [("pic", Picture), ("chroma_format_idc", u32), ("pad0", u32), ("lmcs", LmcsBinding),
@@ -192,7 +207,8 @@ STRUCTS = {"vvclf_plane": Plane, "vvclf_picture": Picture, "vvclf_deblock_cu": D
            "vvclf_sao_config": SaoConfig, "vvclf_bounds": Bounds, "vvclf_alf_ctb": AlfCtb,
            "vvclf_alf_picture": AlfPicture, "vvclf_alf_config": AlfConfig, "vvclf_alf_aps": AlfAps,
            "vvclf_alf_aps_binding": AlfApsBinding, "vvclf_lmcs_data": LmcsData, "vvclf_lmcs_derived": LmcsDerived,
-           "vvclf_lmcs_binding": LmcsBinding, "vvclf_frame": Frame, "vvclf_tune": Tune,
+           "vvclf_lmcs_binding": LmcsBinding, "vvclf_frame": Frame, "vvclf_rect": Rect, "vvclf_crs_block": CrsBlock,
+           "vvclf_crop": Crop, "vvclf_tune": Tune,
            "vvclf_server_config": ServerConfig, "vvclf_server_stats": ServerStats}
 
 P = C.POINTER
@@ -210,6 +226,11 @@ SIGNATURES = {
     "vvclf_apply_in_loop_filters": (C.c_int, [P(Frame)]),
     "vvclf_apply_in_loop_filters_noalloc": (C.c_int, [P(Frame)]),
     "vvclf_derive_lmcs": (C.c_int, [P(LmcsData), u32, P(LmcsDerived)]),
+    "vvclf_lmcs_forward_luma": (C.c_int, [P(Plane), P(LmcsBinding), P(Rect), u32]),
+    "vvclf_lmcs_chroma_residual": (C.c_int, [P(Picture), P(LmcsBinding), P(CrsBlock), u32, P(i32), u64]),
+    "vvclf_conformance_crop": (C.c_int, [u32, u32, u32, P(u32), P(Crop)]),
+    "vvclf_picture_md5": (C.c_int, [P(Picture), u32, u32, P(Crop), P((u8 * 16) * 3)]),
+    "vvclf_job_output_md5": (C.c_int, [VP, u32, P(Crop), P((u8 * 16) * 3)]),
     "vvclf_frame_check": (C.c_int, [P(Frame), P(u64)]),
     "vvclf_last_error": (C.c_char_p, []),
     "vvclf_abi_sizeof": (C.c_size_t, [C.c_char_p]),

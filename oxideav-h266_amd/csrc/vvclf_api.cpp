@@ -71,7 +71,10 @@ struct side_builder {
   }
 };
 
-vvclf_ctx* default_ctx() {
+}  // namespace
+
+// The context of the drop-in calls (device 0), created on first use.
+vvclf_ctx* vvclf::default_ctx() {
   static std::mutex mu;
   static vvclf_ctx* ctx = nullptr;
   std::lock_guard<std::mutex> g(mu);
@@ -90,8 +93,6 @@ vvclf_ctx* default_ctx() {
   }
   return ctx;
 }
-
-}  // namespace
 
 namespace vvclf {
 
@@ -505,7 +506,7 @@ size_t vvclf_abi_sizeof(const char* n) {
   S(vvclf_sao_ctb) S(vvclf_sao_ctb_params) S(vvclf_sao_picture) S(vvclf_sao_config) S(vvclf_bounds)
   S(vvclf_alf_ctb) S(vvclf_alf_picture) S(vvclf_alf_config) S(vvclf_alf_aps) S(vvclf_alf_aps_binding)
   S(vvclf_lmcs_data) S(vvclf_lmcs_derived) S(vvclf_lmcs_binding) S(vvclf_frame) S(vvclf_tune)
-  S(vvclf_server_config) S(vvclf_server_stats)
+  S(vvclf_server_config) S(vvclf_server_stats) S(vvclf_rect) S(vvclf_crs_block) S(vvclf_crop)
 #undef S
   return 0;
 }

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -53,6 +54,19 @@ struct dstatus {
   uint32_t unsafe;  // same-direction luma edges interact
   uint32_t total[6];
 };
+
+// The host builds these records and the kernels read them: both compile
+// passes must lay them out alike (checked for the host and the gfx950 pass).
+static_assert(sizeof(walk::plane_params) == 32 && sizeof(walk::grid_view) == 16 && sizeof(walk::map_view) == 16,
+              "walk views");
+static_assert(sizeof(dpass) == 104 && offsetof(dpass, cus) == 32 && offsetof(dpass, grid) == 48 &&
+                  offsetof(dpass, mv) == 64 && offsetof(dpass, nf) == 72 && offsetof(dpass, m) == 88,
+              "dpass layout");
+static_assert(sizeof(dmap) == 40 && offsetof(dmap, list) == 32, "dmap layout");
+static_assert(sizeof(dstatus) == 40 && offsetof(dstatus, total) == 16, "dstatus layout");
+static_assert(sizeof(vvclf_deblock_cu) == 92 && sizeof(vvclf_mv_cell) == 28 && sizeof(vvclf_mv_grid) == 16 &&
+                  offsetof(vvclf_mv_grid, cells_w) == 8,
+              "ABI records read by the kernels");
 
 __global__ void __launch_bounds__(256) k_fill(int32_t* p, size_t n, int32_t v) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;

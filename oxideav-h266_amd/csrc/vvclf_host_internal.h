@@ -73,6 +73,10 @@ struct host_tables {
   std::vector<uint16_t> lmcs_lut;
 };
 
+// The context of the drop-in calls (device 0), created on first use; NULL
+// when no device is usable.
+vvclf_ctx* default_ctx();
+
 int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 const char* last_error();
