@@ -391,6 +391,61 @@ int vvclf_conformance_crop(uint32_t pic_w, uint32_t pic_h, uint32_t chroma_forma
 int vvclf_picture_md5(const vvclf_picture* pic, uint32_t chroma_format_idc, uint32_t bit_depth,
                       const vvclf_crop* crop, uint8_t md5[3][16]);
 
+/* ------------------------------------------------------------------------ */
+/* Per-CTB side information as a compact record: the syntax elements a      */
+/* slice decoder has parsed for one CTB, from which the SAO and ALF CTB      */
+/* parameters follow (the inference of decode_sao_ctb,                       */
+/* src/sao_syntax.rs:154-284, and of decode_alf_ctu,                         */
+/* src/alf_syntax.rs:170-290, without the CABAC engine).                     */
+/* ------------------------------------------------------------------------ */
+/* 16 bytes per CTB, raster order; bit i of the record is bit (i & 7) of
+ * bytes[i >> 3]:
+ *   0 sao_merge_left_flag, 1 sao_merge_up_flag
+ *   2..32   luma:  sao_type_idx (2) | sao_offset_abs[0..3] (5 each) |
+ *                  sao_offset_sign_flag[0..3] (1 each) | sao_band_position
+ *                  (5; band offset) or sao_eo_class (low 2 bits; edge offset)
+ *   33..63  Cb:    the same fields (type and EO class also serve Cr)
+ *   64..92  Cr:    sao_offset_abs[0..3] | sao_offset_sign_flag[0..3] |
+ *                  sao_band_position
+ *   93 alf_ctb_flag[0], 94 alf_use_aps_flag, 95..98 alf_luma_prev_filter_idx
+ *      (use_aps) or alf_luma_fixed_filter_idx, 99 alf_ctb_flag[1],
+ *      100..102 alf_ctb_filter_alt_idx[0], 103 alf_ctb_flag[2],
+ *      104..106 alf_ctb_filter_alt_idx[1], 107..109 alf_ctb_cc_cb_idc,
+ *      110..112 alf_ctb_cc_cr_idc
+ *   113..127 zero
+ * Fields a decoder does not parse (absent syntax) are zero. */
+typedef struct vvclf_ctb_syntax {
+  uint8_t bytes[16];
+} vvclf_ctb_syntax;
+
+typedef struct vvclf_ctb_syntax_config {
+  uint8_t sao_luma_used, sao_chroma_used;  /* sh_sao_luma/chroma_used_flag */
+  uint8_t alf_enabled, alf_cb_enabled, alf_cr_enabled, alf_cc_cb_enabled, alf_cc_cr_enabled, pad0;
+  uint32_t bit_depth, chroma_format_idc;
+  uint32_t num_alf_aps_ids_luma;              /* sh_num_alf_aps_ids_luma */
+  uint32_t alf_chroma_num_alt_filters_minus1; /* of the bound chroma APS */
+  uint32_t alf_cc_cb_filters_signalled_minus1, alf_cc_cr_filters_signalled_minus1;
+} vvclf_ctb_syntax_config;
+
+/* Expands w_ctbs * h_ctbs records into SAO and ALF CTB parameters (either
+ * output may be NULL): merge flags copy the left / upper CTB's result,
+ * Cr takes Cb's type and EO class, edge-offset signs are (+, +, -, -),
+ * SaoOffsetVal = (1 - 2 sign) * (abs << (BitDepth - Min(BitDepth, 10)))
+ * (eq. 153), AlfCtbFiltSetIdxY = 16 + prev_idx with the APS flag, the fixed
+ * index otherwise.  VVCLF_ERR_INVALID for a record the syntax cannot carry
+ * (a merge at the picture edge, an offset above cMax, an index beyond its
+ * range, reserved bits set). */
+int vvclf_ctb_syntax_decode(const vvclf_ctb_syntax* recs, uint32_t w_ctbs, uint32_t h_ctbs,
+                            const vvclf_ctb_syntax_config* cfg, vvclf_sao_ctb_params* sao_out,
+                            vvclf_alf_ctb* alf_out);
+/* The inverse: records that decode to the given parameters, using merge
+ * flags where a CTB equals its left (else upper) neighbour.
+ * VVCLF_ERR_INVALID when a parameter set has no syntax (an offset that is not
+ * abs << shift with abs <= cMax, edge-offset signs other than (+, +, -, -),
+ * Cr type or EO class differing from Cb's, ...). */
+int vvclf_ctb_syntax_encode(const vvclf_sao_ctb_params* sao, const vvclf_alf_ctb* alf, uint32_t w_ctbs,
+                            uint32_t h_ctbs, const vvclf_ctb_syntax_config* cfg, vvclf_ctb_syntax* out);
+
 /* ======================================================================== */
 /* Layer 2: device-resident jobs, batches and serving (no reference          */
 /* counterpart; the reference is single-threaded CPU code).                  */
@@ -543,6 +598,13 @@ int vvclf_server_close_stream(vvclf_server* srv, uint32_t stream_id);
  * until the ticket completes; the result is written into frame->pic.
  * VVCLF_ERR_BUDGET if the job alone exceeds the budget. */
 int vvclf_server_submit(vvclf_server* srv, uint32_t stream_id, vvclf_frame* frame, uint64_t* ticket);
+/* As vvclf_server_submit, with the frame's SAO and ALF CTB parameters given
+ * as per-CTB syntax records (frame->sao / frame->alf give only the CTB grid
+ * sizes; their ctbs pointers are ignored): the records are expanded on a
+ * prep thread into storage the request owns.  `recs` must stay valid until
+ * the ticket completes. */
+int vvclf_server_submit_syntax(vvclf_server* srv, uint32_t stream_id, vvclf_frame* frame,
+                               const vvclf_ctb_syntax* recs, const vvclf_ctb_syntax_config* cfg, uint64_t* ticket);
 /* VVCLF_OK when done (or the job's own error), VVCLF_ERR_AGAIN if pending.
  * A ticket's final status is reported once: the poll or wait that returns it
  * releases the ticket, and later queries of it return VVCLF_ERR_INVALID.

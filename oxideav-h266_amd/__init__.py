@@ -165,6 +165,19 @@ class Crop(C.Structure):
     _fields_ = [("x", u32), ("y", u32), ("w", u32), ("h", u32)]
 
 
+class CtbSyntax(C.Structure):
+    """Per-CTB syntax record (16 bytes, bit layout in include/vvclf.h)."""
+    _fields_ = [("bytes", u8 * 16)]
+
+
+class CtbSyntaxConfig(C.Structure):
+    _fields_ = [("sao_luma_used", u8), ("sao_chroma_used", u8), ("alf_enabled", u8), ("alf_cb_enabled", u8),
+                ("alf_cr_enabled", u8), ("alf_cc_cb_enabled", u8), ("alf_cc_cr_enabled", u8), ("pad0", u8),
+                ("bit_depth", u32), ("chroma_format_idc", u32), ("num_alf_aps_ids_luma", u32),
+                ("alf_chroma_num_alt_filters_minus1", u32), ("alf_cc_cb_filters_signalled_minus1", u32),
+                ("alf_cc_cr_filters_signalled_minus1", u32)]
+
+
 class Frame(C.Structure):
     """Flattened walker state of apply_in_loop_filters_with_alf, src/ctu.rs:11822-12066."""
     _fields_ = [("pic", Picture), ("chroma_format_idc", u32), ("pad0", u32), ("lmcs", LmcsBinding),
@@ -208,7 +221,7 @@ STRUCTS = {"vvclf_plane": Plane, "vvclf_picture": Picture, "vvclf_deblock_cu": D
            "vvclf_alf_picture": AlfPicture, "vvclf_alf_config": AlfConfig, "vvclf_alf_aps": AlfAps,
            "vvclf_alf_aps_binding": AlfApsBinding, "vvclf_lmcs_data": LmcsData, "vvclf_lmcs_derived": LmcsDerived,
            "vvclf_lmcs_binding": LmcsBinding, "vvclf_frame": Frame, "vvclf_rect": Rect, "vvclf_crs_block": CrsBlock,
-           "vvclf_crop": Crop, "vvclf_tune": Tune,
+           "vvclf_crop": Crop, "vvclf_ctb_syntax": CtbSyntax, "vvclf_ctb_syntax_config": CtbSyntaxConfig, "vvclf_tune": Tune,
            "vvclf_server_config": ServerConfig, "vvclf_server_stats": ServerStats}
 
 P = C.POINTER
@@ -231,6 +244,9 @@ SIGNATURES = {
     "vvclf_conformance_crop": (C.c_int, [u32, u32, u32, P(u32), P(Crop)]),
     "vvclf_picture_md5": (C.c_int, [P(Picture), u32, u32, P(Crop), P((u8 * 16) * 3)]),
     "vvclf_job_output_md5": (C.c_int, [VP, u32, P(Crop), P((u8 * 16) * 3)]),
+    "vvclf_ctb_syntax_decode": (C.c_int, [P(CtbSyntax), u32, u32, P(CtbSyntaxConfig), P(SaoCtbParams), P(AlfCtb)]),
+    "vvclf_ctb_syntax_encode": (C.c_int, [P(SaoCtbParams), P(AlfCtb), u32, u32, P(CtbSyntaxConfig), P(CtbSyntax)]),
+    "vvclf_server_submit_syntax": (C.c_int, [VP, u32, P(Frame), P(CtbSyntax), P(CtbSyntaxConfig), P(u64)]),
     "vvclf_frame_check": (C.c_int, [P(Frame), P(u64)]),
     "vvclf_last_error": (C.c_char_p, []),
     "vvclf_abi_sizeof": (C.c_size_t, [C.c_char_p]),

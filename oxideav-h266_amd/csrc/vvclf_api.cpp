@@ -507,6 +507,7 @@ size_t vvclf_abi_sizeof(const char* n) {
   S(vvclf_alf_ctb) S(vvclf_alf_picture) S(vvclf_alf_config) S(vvclf_alf_aps) S(vvclf_alf_aps_binding)
   S(vvclf_lmcs_data) S(vvclf_lmcs_derived) S(vvclf_lmcs_binding) S(vvclf_frame) S(vvclf_tune)
   S(vvclf_server_config) S(vvclf_server_stats) S(vvclf_rect) S(vvclf_crs_block) S(vvclf_crop)
+  S(vvclf_ctb_syntax) S(vvclf_ctb_syntax_config)
 #undef S
   return 0;
 }

@@ -88,7 +88,16 @@ struct res_obj {
 struct request {
   uint64_t ticket = 0;
   uint32_t stream = 0;
-  vvclf_frame* frame = nullptr;
+  vvclf_frame* frame = nullptr;  // the caller's frame: its planes receive the result
+  // submitted as per-CTB syntax records: expanded on a prep thread into the
+  // request's own parameter arrays, which `expanded` (a copy of the caller's
+  // frame) points at; jobs are built from `expanded` then
+  const vvclf_ctb_syntax* recs = nullptr;
+  vvclf_ctb_syntax_config syn_cfg{};
+  std::vector<vvclf_sao_ctb_params> sao_own;
+  std::vector<vvclf_alf_ctb> alf_own;
+  vvclf_frame expanded{};
+  const vvclf_frame* build() const { return recs ? &expanded : frame; }
   clk::time_point t_submit, t_done;
   clk::time_point t_prep, t_ready, t_launch, t_device;  // stage boundaries
   int status = VVCLF_ERR_AGAIN;  // AGAIN while pending
@@ -275,6 +284,26 @@ int ensure_resident(vvclf_server* s, srv_stream& st, const vvclf_frame& f, hipSt
   return VVCLF_OK;
 }
 
+// A request submitted as syntax records: its SAO / ALF CTB parameters.
+int expand_syntax(request& r) {
+  const vvclf_frame& f = *r.frame;
+  uint32_t w = f.sao.pic_width_in_ctbs_y, h = f.sao.pic_height_in_ctbs_y;
+  if (!w || !h) {
+    w = f.alf.pic_width_in_ctbs_y;
+    h = f.alf.pic_height_in_ctbs_y;
+  }
+  const bool sao_grid = f.sao.pic_width_in_ctbs_y != 0, alf_grid = f.alf.pic_width_in_ctbs_y != 0;
+  if ((sao_grid && alf_grid) && (f.alf.pic_width_in_ctbs_y != w || f.alf.pic_height_in_ctbs_y != h))
+    return fail(VVCLF_ERR_INVALID, "syntax records: SAO and ALF CTB grids differ");
+  r.sao_own.assign((size_t)w * h, vvclf_sao_ctb_params{});
+  r.alf_own.assign((size_t)w * h, vvclf_alf_ctb{});
+  if (int rc = vvclf_ctb_syntax_decode(r.recs, w, h, &r.syn_cfg, r.sao_own.data(), r.alf_own.data())) return rc;
+  r.expanded = f;
+  if (sao_grid) r.expanded.sao.ctbs = r.sao_own.data();
+  if (alf_grid) r.expanded.alf.ctbs = r.alf_own.data();
+  return VVCLF_OK;
+}
+
 void prep_loop(vvclf_server* s, int idx) {
   hipSetDevice(s->ctx->device);
   hipStream_t ps = s->prep_streams[idx];
@@ -321,7 +350,8 @@ void prep_loop(vvclf_server* s, int idx) {
       std::lock_guard<std::mutex> g(s->mu);
       st = s->streams[r->stream].get();
     }
-    if (!rc) rc = ensure_resident(s, *st, *r->frame, ps, r->res);
+    if (!rc && r->recs) rc = expand_syntax(*r);
+    if (!rc) rc = ensure_resident(s, *st, *r->build(), ps, r->res);
     if (!rc) {
       job_opts o;
       o.stream = ps;
@@ -329,7 +359,7 @@ void prep_loop(vvclf_server* s, int idx) {
       o.stage = &s->prep_stage[idx];
       o.side_stage = &s->prep_side[idx];
       o.host_threads = host_walk_threads();  // large pictures walk their CUs on several threads
-      rc = job_create_stages(s->ctx, r->frame, STAGE_ALL, &r->job, nullptr, &o);
+      rc = job_create_stages(s->ctx, r->build(), STAGE_ALL, &r->job, nullptr, &o);
     }
     if (!rc && !job_fits_shape(r->job->d, 32, 32)) {  // 32x32 fits every job any shape fits
       vvclf_job_destroy(r->job);
@@ -625,11 +655,14 @@ int vvclf_server_close_stream(vvclf_server* s, uint32_t id) {
   return VVCLF_OK;
 }
 
-int vvclf_server_submit(vvclf_server* s, uint32_t id, vvclf_frame* frame, uint64_t* ticket) {
-  if (!s || !frame || !ticket) return fail(VVCLF_ERR_INVALID, "submit: NULL argument");
+static int submit(vvclf_server* s, uint32_t id, vvclf_frame* frame, const vvclf_ctb_syntax* recs,
+                  const vvclf_ctb_syntax_config* cfg, uint64_t* ticket) {
+  if (!s || !frame || !ticket || (recs && !cfg)) return fail(VVCLF_ERR_INVALID, "submit: NULL argument");
   auto sp = std::make_shared<request>();
   request* r = sp.get();
   r->frame = frame;
+  r->recs = recs;
+  if (cfg) r->syn_cfg = *cfg;
   r->stream = id;
   r->t_submit = clk::now();
   std::lock_guard<std::mutex> g(s->mu);
@@ -646,6 +679,16 @@ int vvclf_server_submit(vvclf_server* s, uint32_t id, vvclf_frame* frame, uint64
   *ticket = r->ticket;
   s->cv_prep.notify_one();
   return VVCLF_OK;
+}
+
+int vvclf_server_submit(vvclf_server* s, uint32_t id, vvclf_frame* frame, uint64_t* ticket) {
+  return submit(s, id, frame, nullptr, nullptr, ticket);
+}
+
+int vvclf_server_submit_syntax(vvclf_server* s, uint32_t id, vvclf_frame* frame, const vvclf_ctb_syntax* recs,
+                               const vvclf_ctb_syntax_config* cfg, uint64_t* ticket) {
+  if (!recs) return fail(VVCLF_ERR_INVALID, "submit_syntax: NULL records");
+  return submit(s, id, frame, recs, cfg, ticket);
 }
 
 int vvclf_server_poll(vvclf_server* s, uint64_t t) {
