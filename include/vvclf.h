@@ -392,6 +392,29 @@ int vvclf_picture_md5(const vvclf_picture* pic, uint32_t chroma_format_idc, uint
                       const vvclf_crop* crop, uint8_t md5[3][16]);
 
 /* ------------------------------------------------------------------------ */
+/* Encoder: per-CTB luma ALF decision                                       */
+/* ------------------------------------------------------------------------ */
+/* Replaces alf_decide_and_apply_clipped (src/alf_enc.rs:185-316; n_aps = 0)
+ * and alf_decide_and_apply_with_aps_clipped (src/alf_enc.rs:352-470; n_aps =
+ * 1), generalised to n_aps APS luma sets (apses[i] is candidate 16 + i, as
+ * AlfApsBinding::luma_apses slot i).  Every CTB of `rec` (pre-ALF luma) is
+ * filtered with each of the 16 fixed sets and each APS set; per CTB the luma
+ * SSE against `src` (the CTB clipped to the picture, ctb_sse_y) is compared
+ * with the SSE of leaving it unfiltered, and the first candidate with a
+ * strictly lower SSE, in the order off, set 0 .. 15, APS 0 .. n_aps - 1,
+ * is chosen.  On return out[i] (raster CTB order, ceil(W / CtbSizeY) x
+ * ceil(H / CtbSizeY) entries) holds luma_on and luma_filt_set_idx of CTB i
+ * (all other fields zero) and rec's luma holds the chosen samples; chroma is
+ * not touched.  `sse` (optional) receives per CTB 1 + 16 + n_aps values:
+ * off, the 16 fixed sets, the APS sets.  An absent or unsignalled APS leaves
+ * the CTB unfiltered in its trial (its SSE equals the off SSE).  tile_bounds
+ * as for the ALF stage (NULL: picture-wide clip).  Host pictures; the trials
+ * run on the GPU, one pass over the picture for all candidates. */
+int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t ctb_log2_size_y, uint32_t bit_depth,
+                          uint32_t chroma_format_idc, const vvclf_alf_aps* const* apses, uint32_t n_aps,
+                          const vvclf_bounds* tile_bounds, vvclf_alf_ctb* out, uint64_t* sse);
+
+/* ------------------------------------------------------------------------ */
 /* Per-CTB side information as a compact record: the syntax elements a      */
 /* slice decoder has parsed for one CTB, from which the SAO and ALF CTB      */
 /* parameters follow (the inference of decode_sao_ctb,                       */

@@ -71,6 +71,8 @@ def load(vv=None, path=None):
         "oracle_apply_alf_clipped": (C.c_int, [P(vv.Picture), P(vv.AlfPicture), P(vv.AlfConfig),
                                                P(vv.AlfApsBinding), P(vv.Bounds)]),
         "oracle_apply_in_loop_filters": (C.c_int, [P(vv.Frame)]),
+        "oracle_alf_decide_luma": (C.c_int, [P(vv.Picture), P(vv.Picture), u32, u32, u32, P(P(vv.AlfAps)), u32,
+                                             P(vv.Bounds), P(vv.AlfCtb), P(C.c_uint64)]),
         "oracle_apply_in_loop_filters_mt": (C.c_int, [P(P(vv.Frame)), u32, u32]),
     }
     for name, (res, args) in sig.items():

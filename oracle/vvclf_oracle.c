@@ -1528,6 +1528,101 @@ int oracle_apply_alf_clipped(vvclf_picture* out, const vvclf_alf_picture* ap, co
 /* ======================================================================== */
 
 /* apply_in_loop_filters_with_alf, src/ctu.rs:11822-12066 */
+/* ctb_sse_y, src/alf_enc.rs:79-105: luma SSE of the w x h block at (x0, y0),
+ * clipped to both planes. */
+static uint64_t ctb_sse_y(const vvclf_plane* a, const vvclf_plane* b, uint32_t x0, uint32_t y0, uint32_t w,
+                          uint32_t h) {
+  uint64_t sse = 0;
+  for (uint32_t y = 0; y < h; y++) {
+    uint32_t yi = y0 + y;
+    if (yi >= a->height || yi >= b->height) break;
+    for (uint32_t x = 0; x < w; x++) {
+      uint32_t xi = x0 + x;
+      if (xi >= a->width || xi >= b->width) break;
+      int64_t d = (int64_t)a->samples[(size_t)yi * a->stride + xi] - (int64_t)b->samples[(size_t)yi * b->stride + xi];
+      sse += (uint64_t)(d * d);
+    }
+  }
+  return sse;
+}
+
+/* alf_decide_and_apply_clipped, src/alf_enc.rs:185-316, and (n_aps = 1)
+ * alf_decide_and_apply_with_aps_clipped, :350-470: per CTB the lowest luma
+ * SSE among off, the 16 fixed sets and the APS sets (trial index 16 + i,
+ * filter set 16 + i), the earlier candidate winning ties; rec's luma takes the
+ * chosen trial's samples.  out: w_ctbs * h_ctbs entries; sse (optional):
+ * [n_ctbs][17 + n_aps - 1 + 1] = off, sets 0..15, APS sets. */
+int oracle_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd, uint32_t cfi,
+                           const vvclf_alf_aps* const* apses, uint32_t n_aps, const vvclf_bounds* tb,
+                           vvclf_alf_ctb* out, uint64_t* sse) {
+  uint32_t ctb = 1u << ctb_log2;
+  uint32_t wc = (rec->luma.width + ctb - 1) / ctb, hc = (rec->luma.height + ctb - 1) / ctb;
+  size_t n = (size_t)wc * hc;
+  uint32_t nt = 16 + n_aps, ncand = 1 + nt;
+  uint64_t* tab = (uint64_t*)calloc(n * ncand, sizeof(uint64_t));
+  for (uint32_t ry = 0; ry < hc; ry++)
+    for (uint32_t rx = 0; rx < wc; rx++)
+      tab[((size_t)ry * wc + rx) * ncand] = ctb_sse_y(&src->luma, &rec->luma, rx * ctb, ry * ctb, ctb, ctb);
+  vvclf_alf_config cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.alf_enabled = 1;
+  cfg.bit_depth = bd;
+  cfg.ctb_log2_size_y = ctb_log2;
+  cfg.chroma_format_idc = cfi;
+  vvclf_alf_aps_binding b;
+  memset(&b, 0, sizeof(b));
+  b.luma_apses = apses;
+  b.n_luma_apses = n_aps;
+  size_t plane = (size_t)rec->luma.stride * rec->luma.height;
+  uint16_t** trial = (uint16_t**)calloc(nt, sizeof(uint16_t*));
+  vvclf_alf_ctb* pic = (vvclf_alf_ctb*)calloc(n, sizeof(vvclf_alf_ctb));
+  int rc = VVCLF_OK;
+  for (uint32_t t = 0; t < nt && rc == VVCLF_OK; t++) {
+    vvclf_picture r = *rec;
+    trial[t] = (uint16_t*)malloc(plane * 2 + 2);
+    memcpy(trial[t], rec->luma.samples, plane * 2);
+    r.luma.samples = trial[t];  /* chroma ALF is off in the trials: chroma is only read */
+    for (size_t i = 0; i < n; i++) {
+      memset(&pic[i], 0, sizeof(pic[i]));
+      pic[i].luma_on = 1;
+      pic[i].luma_filt_set_idx = (uint8_t)t;
+    }
+    vvclf_alf_picture ap = {wc, hc, pic};
+    rc = oracle_apply_alf_clipped(&r, &ap, &cfg, &b, tb);
+    for (uint32_t ry = 0; ry < hc; ry++)
+      for (uint32_t rx = 0; rx < wc; rx++)
+        tab[((size_t)ry * wc + rx) * ncand + 1 + t] = ctb_sse_y(&src->luma, &r.luma, rx * ctb, ry * ctb, ctb, ctb);
+  }
+  for (uint32_t ry = 0; ry < hc && rc == VVCLF_OK; ry++)
+    for (uint32_t rx = 0; rx < wc; rx++) {
+      size_t i = (size_t)ry * wc + rx;
+      uint64_t best = tab[i * ncand];
+      int best_t = -1;
+      for (uint32_t t = 0; t < nt; t++)
+        if (tab[i * ncand + 1 + t] < best) {
+          best = tab[i * ncand + 1 + t];
+          best_t = (int)t;
+        }
+      memset(&out[i], 0, sizeof(out[i]));
+      if (best_t >= 0) {
+        out[i].luma_on = 1;
+        out[i].luma_filt_set_idx = (uint8_t)best_t;
+        uint32_t x0 = rx * ctb, y0 = ry * ctb;
+        for (uint32_t y = y0; y < y0 + ctb && y < rec->luma.height; y++)
+          for (uint32_t x = x0; x < x0 + ctb && x < rec->luma.width; x++) {
+            size_t at = (size_t)y * rec->luma.stride + x;
+            rec->luma.samples[at] = trial[best_t][at];
+          }
+      }
+    }
+  if (sse && rc == VVCLF_OK) memcpy(sse, tab, n * ncand * sizeof(uint64_t));
+  for (uint32_t t = 0; t < nt; t++) free(trial[t]);
+  free(trial);
+  free(pic);
+  free(tab);
+  return rc;
+}
+
 int oracle_apply_in_loop_filters(vvclf_frame* f) {
   static const vvclf_deblock_cu empty_cus[1];
   const vvclf_deblock_cu* ccus = f->has_chroma_cus ? (f->chroma_cus ? f->chroma_cus : empty_cus) : NULL;

@@ -85,6 +85,11 @@ void oracle_alf_classify_ctb(const uint16_t* pre, uint32_t stride, int32_t pw, i
 int oracle_apply_alf_clipped(vvclf_picture* out, const vvclf_alf_picture* alf_pic, const vvclf_alf_config* cfg,
                              const vvclf_alf_aps_binding* binding, const vvclf_bounds* tile_bounds);
 
+/* src/alf_enc.rs:185-316 (and :350-470 with APS sets) */
+int oracle_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd, uint32_t cfi,
+                           const vvclf_alf_aps* const* apses, uint32_t n_aps, const vvclf_bounds* tb,
+                           vvclf_alf_ctb* out, uint64_t* sse);
+
 /* src/ctu.rs:11822-12066 */
 int oracle_apply_in_loop_filters(vvclf_frame* frame);
 

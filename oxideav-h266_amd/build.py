@@ -17,7 +17,7 @@ ARCH = os.environ.get("VVCLF_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["vvclf_plain.hip", "vvclf_fused.hip", "vvclf_derive.hip", "vvclf_host.cpp", "vvclf_api.cpp",
                "vvclf_batch.cpp", "vvclf_server.cpp", "vvclf_noalloc.cpp", "vvclf_recon.hip",
-               "vvclf_syntax.cpp"]
+               "vvclf_syntax.cpp", "vvclf_enc.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              # packed FP32 instructions are disabled for all device code
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1,0 +1,371 @@
+// vvclf_enc.hip — the encoder's per-CTB ALF luma decision as batched GPU
+// filtering: alf_decide_and_apply_clipped (src/alf_enc.rs:185-316) and
+// alf_decide_and_apply_with_aps_clipped (:350-470).
+//
+// The reference filters the whole picture once per candidate (16 fixed
+// sets, then each APS set), measures each CTB's luma SSE against the source
+// after every trial, keeps per CTB the candidate with the lowest SSE (off
+// first, then the candidates in order; ties keep the earlier one) and copies
+// that trial's samples back.  A CTB's ALF output depends only on the pre-ALF
+// picture, so here:
+//   k_alf_enc<TRIAL>   one thread per 4x4 block classifies once and filters
+//                      the block with every candidate's bank; per candidate
+//                      the block's squared error against the source is
+//                      reduced over the wave and added to sse[ctb][cand]
+//                      (a wave lies in one CTB); candidate 0 is "off"
+//   k_alf_pick         per CTB the first minimum
+//   k_alf_enc<COMMIT>  the chosen candidate's output (or the pre-ALF
+//                      samples) written once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/vvclf.h"
+#include "vvclf_filters.h"
+#include "vvclf_host_internal.h"
+
+using namespace vvclf;
+
+namespace {
+
+struct enc_args {
+  const uint16_t* rec;  // pre-ALF luma (pitch)
+  const uint16_t* src;  // source luma (pitch)
+  uint16_t* out;        // commit output (pitch)
+  int pitch, w, h;
+  int ctb_log2, wctb, hctb, bd;
+  const uint64_t* rect;        // per CTB fetch rectangle (pack_rect)
+  const uint32_t* banks;       // [n_cand][ALF_BANK_WORDS]
+  int n_cand;                  // 16 + number of APS sets
+  const uint8_t* ok;           // [n_cand]: 0 for an absent or unsignalled APS
+  unsigned long long* sse;     // [n_ctbs][1 + n_cand]
+  const int32_t* choice;       // [n_ctbs]: chosen trial, -1 off
+};
+
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int vb_kind(int yb, int ctb) { return yb == ctb - 8 ? 1 : (yb == ctb - 4 ? 2 : 0); }
+
+// the class of a block from its 32-bit window (virtual-boundary variant by vb)
+__device__ __forceinline__ int class_w(const int (&W)[10][10], int vb, bool carve, int bd) {
+  if (vb == 0) return alf_block_class<0, false>(W, bd);
+  if (vb == 1) return carve ? alf_block_class<1, true>(W, bd) : alf_block_class<1, false>(W, bd);
+  return carve ? alf_block_class<2, true>(W, bd) : alf_block_class<2, false>(W, bd);
+}
+
+template <bool TRIAL, bool HBD>
+__global__ void __launch_bounds__(256) k_alf_enc(enc_args a, int tile_log2) {
+  const int tile = 1 << tile_log2, tb = tile >> 2;
+  const int bx = threadIdx.x % tb, by = threadIdx.x / tb;
+  const int tx0 = blockIdx.x * tile, ty0 = blockIdx.y * tile;
+  const int ctb = 1 << a.ctb_log2;
+  const int rx = tx0 >> a.ctb_log2, ry = ty0 >> a.ctb_log2;
+  const size_t ci = (size_t)ry * a.wctb + rx;
+  const int x4 = tx0 + bx * 4, y4 = ty0 + by * 4;
+  // lanes past the tile or the picture take part in the wave reductions with
+  // zero error and touch no memory
+  const bool live = by < tb && x4 < a.w && y4 < a.h;
+  const uint64_t r = a.rect[ci];
+  const int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff) - 1,
+            cy1 = (int)(r >> 48) - 1;
+  const int y_ctb = ry * ctb;
+  const bool carve = !(y_ctb + ctb >= a.h) || (a.h - y_ctb) > ctb - 4;
+  const int vb = vb_kind(y4 - y_ctb, ctb);
+  const int amax = (1 << a.bd) - 1;
+  // the block's valid samples (inside the picture)
+  const int nr = live ? min(4, a.h - y4) : 0, nc = live ? min(4, a.w - x4) : 0;
+  // 10x10 window of the block (rect-clamped), packed or 32-bit
+  uint32_t P[10][6];
+  int W[10][10];
+  bool wide = HBD;
+  if (live) {
+    if (!HBD && x4 - 3 >= cx0 && x4 + 6 <= cx1 && y4 - 3 >= cy0 && y4 + 6 <= cy1) {
+      const uint16_t* w0 = a.rec + (size_t)(y4 - 3) * a.pitch + x4 - 4;
+#pragma unroll
+      for (int rr = 0; rr < 10; rr++) {
+        const uint2* q = reinterpret_cast<const uint2*>(w0 + (size_t)rr * a.pitch);
+        const uint2 u = q[0], v = q[1], t = q[2];
+        P[rr][0] = u.x; P[rr][1] = u.y; P[rr][2] = v.x; P[rr][3] = v.y; P[rr][4] = t.x; P[rr][5] = t.y;
+      }
+    } else {
+#pragma unroll
+      for (int rr = 0; rr < 10; rr++) {
+        const uint16_t* row = a.rec + (size_t)clampi(y4 - 3 + rr, cy0, cy1) * a.pitch;
+        uint32_t v[12];
+        v[0] = 0;
+        v[11] = 0;
+#pragma unroll
+        for (int q = 1; q <= 10; q++) v[q] = row[clampi(x4 - 4 + q, cx0, cx1)];
+#pragma unroll
+        for (int k = 0; k < 6; k++) P[rr][k] = v[2 * k] | (v[2 * k + 1] << 16);
+      }
+    }
+    uint32_t any = 0;
+#pragma unroll
+    for (int rr = 0; rr < 10; rr++)
+#pragma unroll
+      for (int k = 0; k < 6; k++) any |= P[rr][k];
+    wide = HBD || (any & 0x80008000u);
+  } else {
+#pragma unroll
+    for (int rr = 0; rr < 10; rr++)
+#pragma unroll
+      for (int k = 0; k < 6; k++) P[rr][k] = 0;
+  }
+  if (wide) {
+#pragma unroll
+    for (int rr = 0; rr < 10; rr++)
+#pragma unroll
+      for (int cc = 0; cc < 10; cc++) W[rr][cc] = (int)((P[rr][(cc + 1) >> 1] >> (16 * ((cc + 1) & 1))) & 0xffffu);
+  }
+  const int cls = wide ? class_w(W, vb, carve, a.bd) : alf_pk_block_class(P, vb, carve, a.bd);
+  const int row_of_bank = ((cls & 31) * 4 + (cls >> 5)) * 12;
+
+  // the block's 16 filtered samples with candidate s (out[r][c])
+  auto filter = [&](int s, int (&o)[4][4]) {
+    const uint32_t* f = a.banks + (size_t)s * ALF_BANK_WORDS + row_of_bank;
+    if (wide) {
+      uint32_t fw[12];
+#pragma unroll
+      for (int k = 0; k < 12; k++) fw[k] = f[k];
+      if (vb == 0) alf_block_filter<0>(W, fw, amax, o);
+      else if (vb == 1) alf_block_filter<1>(W, fw, amax, o);
+      else alf_block_filter<2>(W, fw, amax, o);
+    } else {
+      uint32_t op[4][2];
+      if (vb == 0) alf_pk_filter<0>(P, f, amax, op);
+      else if (vb == 1) alf_pk_filter<1>(P, f, amax, op);
+      else alf_pk_filter<2>(P, f, amax, op);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[rr][c] = (int)((op[rr][c >> 1] >> (16 * (c & 1))) & 0xffffu);
+    }
+  };
+  auto centre = [&](int rr, int c) {
+    return wide ? W[rr + 3][c + 3] : (int)((P[rr + 3][(c + 4) >> 1] >> (16 * ((c + 4) & 1))) & 0xffffu);
+  };
+
+  if (TRIAL) {
+    int S[4][4];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) S[rr][c] = (rr < nr && c < nc) ? a.src[(size_t)(y4 + rr) * a.pitch + x4 + c] : 0;
+    unsigned long long* row = a.sse + ci * (size_t)(1 + a.n_cand);
+    // candidate 0: off (the pre-ALF samples)
+    unsigned long long e = 0;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (rr < nr && c < nc) {
+          const long long d = (long long)S[rr][c] - centre(rr, c);
+          e += (unsigned long long)(d * d);
+        }
+    e = wave_sum64(e);
+    if ((threadIdx.x & 63) == 0 && e) atomicAdd(row, e);
+    for (int s = 0; s < a.n_cand; s++) {
+      if (!a.ok[s]) continue;
+      int o[4][4];
+      filter(s, o);
+      unsigned long long es = 0;
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (rr < nr && c < nc) {
+            const long long d = (long long)S[rr][c] - o[rr][c];
+            es += (unsigned long long)(d * d);
+          }
+      es = wave_sum64(es);
+      if ((threadIdx.x & 63) == 0 && es) atomicAdd(row + 1 + s, es);
+    }
+  } else {
+    if (!live) return;
+    const int s = a.choice[ci];
+    int o[4][4];
+    if (s >= 0) {
+      filter(s, o);
+    } else {
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[rr][c] = centre(rr, c);
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (rr < nr && c < nc) a.out[(size_t)(y4 + rr) * a.pitch + x4 + c] = (uint16_t)o[rr][c];
+  }
+}
+
+// per CTB the first candidate with the lowest SSE (off first); a candidate
+// naming an unusable APS leaves the CTB unfiltered, so its error is the off
+// error
+__global__ void __launch_bounds__(256) k_alf_pick(unsigned long long* sse, const uint8_t* ok, int n_ctbs, int n_cand,
+                                                  int32_t* choice) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_ctbs) return;
+  unsigned long long* row = sse + (size_t)i * (1 + n_cand);
+  unsigned long long best = row[0];
+  int t = -1;
+  for (int s = 0; s < n_cand; s++) {
+    if (!ok[s]) row[1 + s] = row[0];
+    if (row[1 + s] < best) {
+      best = row[1 + s];
+      t = s;
+    }
+  }
+  choice[i] = t;
+}
+
+struct dbuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  hipError_t alloc(size_t n, hipStream_t st) {
+    s = st;
+    return hipMallocAsync(&p, std::max<size_t>(n, 4), st);
+  }
+  ~dbuf() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+};
+
+}  // namespace
+
+extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd,
+                                     uint32_t cfi, const vvclf_alf_aps* const* apses, uint32_t n_aps,
+                                     const vvclf_bounds* tb, vvclf_alf_ctb* out, uint64_t* sse_out) {
+  clear_error();
+  if (!src || !rec || !out || (n_aps && !apses)) return fail(VVCLF_ERR_INVALID, "alf_decide_luma: NULL argument");
+  const vvclf_plane &S = src->luma, &R = rec->luma;
+  const uint32_t W = R.width, H = R.height;
+  if (!S.samples || !R.samples || S.width != W || S.height != H || W == 0 || H == 0)
+    return fail(VVCLF_ERR_INVALID, "alf_decide_luma: source and reconstruction luma differ");
+  if (bd < 8 || bd > 16) return fail(VVCLF_ERR_INVALID, "alf_decide_luma: bit depth %u", bd);
+  if (ctb_log2 < 3 || ctb_log2 > 8) return fail(VVCLF_ERR_UNSUPPORTED, "alf_decide_luma: ctb_log2_size_y %u", ctb_log2);
+  if (W > 65535 || H > 65535) return fail(VVCLF_ERR_UNSUPPORTED, "alf_decide_luma: picture above 65535 samples");
+  if (n_aps > 239) return fail(VVCLF_ERR_INVALID, "alf_decide_luma: %u APS sets (at most 239)", n_aps);
+  for (uint32_t i = 0; i < n_aps; i++)
+    if (apses[i] && apses[i]->alf_luma_filter_signal_flag && apses[i]->n_luma_coeff < 25)
+      return fail(VVCLF_ERR_INVALID, "alf_decide_luma: APS %u carries %u luma filter rows", i, apses[i]->n_luma_coeff);
+  // the filters hold luma coefficients in 16-bit lanes (conformant streams
+  // keep |c| <= 128); larger values are refused rather than filtered
+  // differently
+  for (uint32_t i = 0; i < n_aps; i++) {
+    const vvclf_alf_aps* ap = apses[i];
+    if (!ap || !ap->alf_luma_filter_signal_flag) continue;
+    for (int c = 0; c < 25; c++)
+      for (int k = 0; k < 12; k++)
+        if (ap->luma_coeff[c][k] < -32768 || ap->luma_coeff[c][k] > 32767)
+          return fail(VVCLF_ERR_UNSUPPORTED, "alf_decide_luma: luma coefficient %d outside 16 bits", ap->luma_coeff[c][k]);
+  }
+  const uint32_t ctb = 1u << ctb_log2, wctb = (W + ctb - 1) / ctb, hctb = (H + ctb - 1) / ctb;
+  const uint32_t n_ctbs = wctb * hctb, n_cand = 16 + n_aps;
+  // banks: 16 fixed sets, then the APS sets (an absent or unsignalled APS
+  // leaves its bank zero, i.e. the identity, as the reference skips the CTB)
+  vvclf_alf_config ac;
+  memset(&ac, 0, sizeof(ac));
+  ac.alf_enabled = 1;
+  ac.bit_depth = bd;
+  ac.ctb_log2_size_y = ctb_log2;
+  ac.chroma_format_idc = cfi;
+  vvclf_alf_aps_binding b;
+  memset(&b, 0, sizeof(b));
+  b.luma_apses = apses;
+  b.n_luma_apses = n_aps;
+  host_tables t;
+  build_alf_banks(ac, b, t);
+  std::vector<uint8_t> ok(n_cand, 1);
+  for (uint32_t i = 0; i < n_aps; i++) ok[16 + i] = apses[i] && apses[i]->alf_luma_filter_signal_flag;
+  std::vector<uint64_t> rects(n_ctbs);
+  for (uint32_t ry = 0; ry < hctb; ry++)
+    for (uint32_t rx = 0; rx < wctb; rx++) {
+      rects[(size_t)ry * wctb + rx] = ctb_fetch_rect(tb, rx, ry, ctb, W, H);
+      if (!rects[(size_t)ry * wctb + rx]) return fail(VVCLF_ERR_INVALID, "alf_decide_luma: empty clip rectangle");
+    }
+  vvclf_ctx* ctx = default_ctx();
+  if (!ctx) return VVCLF_ERR_HIP;
+  hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  const uint32_t pitch = (W + 63) / 64 * 64;
+  const size_t plane = (size_t)pitch * H * 2;
+  dbuf drec, dsrc, dout, drect, dbank, dsse, dchoice, dok;
+  hipError_t e = drec.alloc(plane, st);
+  if (e == hipSuccess) e = dsrc.alloc(plane, st);
+  if (e == hipSuccess) e = dout.alloc(plane, st);
+  if (e == hipSuccess) e = drect.alloc(rects.size() * 8, st);
+  if (e == hipSuccess) e = dbank.alloc(t.alf_bank.size() * 4, st);
+  if (e == hipSuccess) e = dsse.alloc((size_t)n_ctbs * (1 + n_cand) * 8, st);
+  if (e == hipSuccess) e = dchoice.alloc((size_t)n_ctbs * 4, st);
+  if (e == hipSuccess) e = dok.alloc(n_cand, st);
+  if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "alf_decide_luma: device buffers: %s", hipGetErrorString(e));
+  e = hipMemcpy2DAsync(drec.p, (size_t)pitch * 2, R.samples, (size_t)R.stride * 2, (size_t)W * 2, H,
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(dsrc.p, (size_t)pitch * 2, S.samples, (size_t)S.stride * 2, (size_t)W * 2, H,
+                         hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(drect.p, rects.data(), rects.size() * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dbank.p, t.alf_bank.data(), t.alf_bank.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dok.p, ok.data(), n_cand, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(dsse.p, 0, (size_t)n_ctbs * (1 + n_cand) * 8, st);
+  enc_args a;
+  a.rec = (const uint16_t*)drec.p;
+  a.src = (const uint16_t*)dsrc.p;
+  a.out = (uint16_t*)dout.p;
+  a.pitch = (int)pitch;
+  a.w = (int)W;
+  a.h = (int)H;
+  a.ctb_log2 = (int)ctb_log2;
+  a.wctb = (int)wctb;
+  a.hctb = (int)hctb;
+  a.bd = (int)bd;
+  a.rect = (const uint64_t*)drect.p;
+  a.banks = (const uint32_t*)dbank.p;
+  a.n_cand = (int)n_cand;
+  a.ok = (const uint8_t*)dok.p;
+  a.sse = (unsigned long long*)dsse.p;
+  a.choice = (const int32_t*)dchoice.p;
+  const int tl = ctb_log2 < 6 ? (int)ctb_log2 : 6, tile = 1 << tl;
+  const dim3 grid((W + tile - 1) / tile, (H + tile - 1) / tile);
+  const unsigned threads = std::max(64u, (unsigned)((tile / 4) * (tile / 4)));
+  const bool hbd = bd > 12;
+  if (e == hipSuccess) {
+    if (hbd) hipLaunchKernelGGL((k_alf_enc<true, true>), grid, dim3(threads), 0, st, a, tl);
+    else hipLaunchKernelGGL((k_alf_enc<true, false>), grid, dim3(threads), 0, st, a, tl);
+    hipLaunchKernelGGL(k_alf_pick, dim3((n_ctbs + 255) / 256), dim3(256), 0, st, (unsigned long long*)dsse.p,
+                       (const uint8_t*)dok.p, (int)n_ctbs, (int)n_cand, (int32_t*)dchoice.p);
+    if (hbd) hipLaunchKernelGGL((k_alf_enc<false, true>), grid, dim3(threads), 0, st, a, tl);
+    else hipLaunchKernelGGL((k_alf_enc<false, false>), grid, dim3(threads), 0, st, a, tl);
+    e = hipGetLastError();
+  }
+  std::vector<int32_t> choice(n_ctbs);
+  std::vector<uint64_t> sse;
+  if (sse_out) sse.resize((size_t)n_ctbs * (1 + n_cand));
+  if (e == hipSuccess) e = hipMemcpyAsync(choice.data(), dchoice.p, (size_t)n_ctbs * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && sse_out)
+    e = hipMemcpyAsync(sse.data(), dsse.p, sse.size() * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(R.samples, (size_t)R.stride * 2, dout.p, (size_t)pitch * 2, (size_t)W * 2, H,
+                         hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "alf_decide_luma");
+  for (uint32_t i = 0; i < n_ctbs; i++) {
+    memset(&out[i], 0, sizeof(out[i]));
+    if (choice[i] >= 0) {
+      out[i].luma_on = 1;
+      out[i].luma_filt_set_idx = (uint8_t)choice[i];
+    }
+  }
+  if (sse_out) memcpy(sse_out, sse.data(), sse.size() * 8);
+  return VVCLF_OK;
+}

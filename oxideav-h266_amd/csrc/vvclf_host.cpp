@@ -540,6 +540,10 @@ void build_luma_bank(const int32_t coeff[25][12], const uint8_t clip[25][12], ui
 }
 }  // namespace
 
+uint64_t ctb_fetch_rect(const vvclf_bounds* tb, uint32_t rx, uint32_t ry, uint32_t ctb, uint32_t W, uint32_t H) {
+  return ctb_rect(tb, rx, ry, ctb, W, H);
+}
+
 // The filter banks of an ALF binding (src/alf.rs:564-644): 16 fixed sets,
 // one per APS slot, the chroma alternatives and the CC filters.
 void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t) {

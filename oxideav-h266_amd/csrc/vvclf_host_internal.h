@@ -95,6 +95,9 @@ int host_walk_threads();
 // have finished.  Concurrent callers share the workers.
 void parallel_for(int n, const std::function<void(int)>& fn);
 void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t);
+// The ALF / SAO fetch rectangle of CTB (rx, ry) (pack_rect), the tile's
+// rectangle with tile bounds, else the picture; 0 when empty.
+uint64_t ctb_fetch_rect(const vvclf_bounds* tb, uint32_t rx, uint32_t ry, uint32_t ctb, uint32_t W, uint32_t H);
 
 static constexpr int PLAIN_STAGES = 6;  // prep, deblock V, deblock H, SAO, ALF luma, ALF chroma
 // Device tables shared by many jobs (a serving stream's resident LMCS / ALF

@@ -105,6 +105,25 @@ def test_alf_coefficients_beyond_the_filter_lanes_are_unsupported(vv):
                 assert b"bits" in vv.lib.vvclf_last_error()
 
 
+def test_alf_decide_rejects_before_any_device_work(vv):
+    """vvclf_alf_decide_luma checks its arguments on the host: a luma APS
+    with fewer than 25 filters is invalid (as the ALF stage), coefficients
+    beyond 16 bits, more than 239 APS sets (luma_filt_set_idx is 8 bits) and
+    CTB sizes outside 8..256 are refused."""
+    from alf_decide_cases import decide, luma_aps, smooth_gradient
+    src = smooth_gradient(vv, 64, 64, 8)
+    short = luma_aps(vv, {6: 8})
+    short.n_luma_coeff = 5
+    wide = luma_aps(vv, {3: 32768})
+    fn = vv.lib.vvclf_alf_decide_luma
+    assert decide(vv, fn, src, src, 6, 8, [short])[0] == vv.ERR_INVALID
+    assert decide(vv, fn, src, src, 6, 8, [wide])[0] == vv.ERR_UNSUPPORTED
+    assert b"16 bits" in vv.lib.vvclf_last_error()
+    assert decide(vv, fn, src, src, 6, 8, [None] * 240)[0] == vv.ERR_INVALID
+    assert decide(vv, fn, src, src, 9, 8)[0] == vv.ERR_UNSUPPORTED
+    assert decide(vv, fn, src, src, 6, 17)[0] == vv.ERR_INVALID
+
+
 def test_derive_lmcs_matches_oracle(vv, orc):
     rng = np.random.default_rng(0)
     for trial in range(300):
