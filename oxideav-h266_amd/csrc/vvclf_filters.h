@@ -1003,4 +1003,55 @@ __device__ __forceinline__ void alf_block_pk(const uint32_t (&P)[10][6], const u
   else alf_pk_filter<2>(P, f, maxv, out);
 }
 
+// ---------------------------------------------------------------------------
+// ALF luma of one block through a sample accessor at(dx, dy) -> the sample at
+// (x4 + dx, y4 + dy) clamped to the fetch rectangle, with run-time loops: the
+// rare 32-bit path (windows holding samples of 2^15 or more), kept light on
+// registers so that the packed path keeps its occupancy.  Same sums, virtual
+// boundary rules and taps as alf_block_class / alf_block_filter
+// (src/alf.rs:822-1010, :652-724); put(c, r, v) receives output sample
+// (x4 + c, y4 + r).
+// ---------------------------------------------------------------------------
+template <class AT>
+__device__ __forceinline__ int alf_class_at(AT at, int vb, bool carve, int bd) {
+  const int jmin = (vb == 2 && carve) ? 0 : -2, jmax = (vb == 1 && carve) ? 3 : 5;
+  const int ac = (vb != 0 && carve) ? 3 : 2;
+  // row offset after the virtual-boundary padding
+  auto rowv = [&](int r) { return vb == 1 ? (r > 3 ? 3 : r) : (vb == 2 ? (r < 0 ? 0 : r) : r); };
+  int sh = 0, sv = 0, sd0 = 0, sd1 = 0;
+#pragma unroll 1
+  for (int jj = jmin; jj <= jmax; jj++) {
+    const int rc = rowv(jj), ru = rowv(jj - 1), rd = rowv(jj + 1);
+#pragma unroll 1
+    for (int ii = -2 + (jj & 1); ii <= 5; ii += 2) {  // positions with ii, jj of one parity
+      const int c2 = 2 * at(ii, rc);
+      sh += abs(c2 - at(ii - 1, rc) - at(ii + 1, rc));
+      sv += abs(c2 - at(ii, ru) - at(ii, rd));
+      sd0 += abs(c2 - at(ii - 1, ru) - at(ii + 1, rd));
+      sd1 += abs(c2 - at(ii + 1, ru) - at(ii - 1, rd));
+    }
+  }
+  return alf_class_from_sums(sh, sv, sd0, sd1, ac, bd);
+}
+
+template <class AT, class PUT>
+__device__ __forceinline__ void alf_filter_at(AT at, const uint32_t* f, int vb, int amax, PUT put) {
+#pragma unroll 1
+  for (int r = 0; r < 4; r++) {
+    const int m = vb == 0 ? 3 : (vb == 1 ? 3 - r : r);
+    const int sft = m == 0 ? 10 : 7;
+#pragma unroll 1
+    for (int c = 0; c < 4; c++) {
+      const int cur = at(c, r);
+      int sum = 0;
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        const int dy = kTapDy[k] < m ? kTapDy[k] : m, dx = kTapDx[k];
+        sum += alf_pair(f[k], at(c + dx, r + dy), at(c - dx, r - dy), cur);
+      }
+      put(c, r, clampi(cur + ((sum + (1 << (sft - 1))) >> sft), 0, amax));
+    }
+  }
+}
+
 }  // namespace vvclf

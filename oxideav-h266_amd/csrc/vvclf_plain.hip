@@ -312,12 +312,12 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
 // ---------------------------------------------------------------------------
 // ALF luma: one thread per 4x4 block.  The block's 10x10 neighbourhood
 // (-3..+6) feeds both the classification window and the 7x7 diamond.
-// A workgroup covers one CTB-aligned tile and stages that CTB's permuted
-// filter bank in LDS.
+// A workgroup covers one CTB-aligned tile; each block reads its class's 12
+// tap words of the CTB's permuted filter bank straight from memory (cached),
+// so that no wave waits on a workgroup barrier.
 // ---------------------------------------------------------------------------
 template <bool HBD>
 __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
-  __shared__ uint32_t bank[ALF_BANK_WORDS];
   const int tile = 1 << tile_log2, tb = tile >> 2;
   const int bx = threadIdx.x % tb, by = threadIdx.x / tb;
   const int tx0 = blockIdx.x * tile, ty0 = blockIdx.y * tile;
@@ -342,9 +342,7 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
     }
     return;
   }
-  for (int i = threadIdx.x; i < ALF_BANK_WORDS; i += blockDim.x)
-    bank[i] = j.alf_bank[(size_t)bank_idx * ALF_BANK_WORDS + i];
-  __syncthreads();
+  const uint32_t* bank = j.alf_bank + (size_t)bank_idx * ALF_BANK_WORDS;
   if (by >= tb) return;
   if (x4 >= src.w || y4 >= src.h) return;
   uint64_t r = j.alf_rect[ci];
@@ -406,18 +404,15 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
 #pragma unroll
       for (int k = 0; k < 6; k++) any |= P[rr][k];
     if (any & 0x80008000u) {
-      int W[10][10];
-#pragma unroll
-      for (int rr = 0; rr < 10; rr++)
-#pragma unroll
-        for (int cc = 0; cc < 10; cc++) W[rr][cc] = (int)((P[rr][(cc + 1) >> 1] >> (16 * ((cc + 1) & 1))) & 0xffffu);
-      int outv[4][4];
-      alf_block(W, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outv);
-#pragma unroll
-      for (int rr = 0; rr < 4; rr++) {
-        outp[rr][0] = (uint32_t)outv[rr][0] | ((uint32_t)outv[rr][1] << 16);
-        outp[rr][1] = (uint32_t)outv[rr][2] | ((uint32_t)outv[rr][3] << 16);
-      }
+      auto at = [&](int dx, int dy) -> int {
+        return src.p[(size_t)clampi(y4 + dy, cy0, cy1) * src.pitch + clampi(x4 + dx, cx0, cx1)];
+      };
+      const int vb = y4 - y_ctb == ctb - 8 ? 1 : (y4 - y_ctb == ctb - 4 ? 2 : 0);
+      const int cls = alf_class_at(at, vb, carve, j.alf_bd);
+      alf_filter_at(at, bank + ((cls & 31) * 4 + (cls >> 5)) * 12, vb, amax, [&](int c, int r, int v) {
+        if (x4 + c < src.w && y4 + r < src.h) dst.p[(size_t)(y4 + r) * dst.pitch + x4 + c] = (uint16_t)v;
+      });
+      return;
     } else {
       alf_block_pk(P, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outp);
     }
