@@ -810,7 +810,12 @@ __device__ constexpr int kTapDx[12] = {0, 1, 0, -1, 2, 1, 0, -1, -2, 3, 2, 1};
 // samples (x4+2h, x4+2h+1) of row y4+r as a u16 pair.  Q[i] holds window row
 // R0 + i (N >= NR + 6 rows).  Taps are the outer loop so that only one tap's
 // constants are live.
-template <int VB, int R0, int NR, int N>
+// NOCLIP: the caller has checked that the block's clip values all exceed
+// maxv and that every window sample is at most maxv, so that no clip acts
+// (|a - cur| <= maxv): the taps are summed unclipped, (a + b) - 2 cur.  This
+// is the case of the fixed filter sets and of APS filters without clipping
+// (AlfClip = 2^BitDepth, src/alf.rs:627-644) on decoded samples.
+template <int VB, int R0, int NR, int N, bool NOCLIP = false>
 __device__ __forceinline__ void alf_pk_filter_rows(const uint32_t (&Q)[N][6], const uint32_t* f, int maxv,
                                                    uint32_t (&out)[4][2]) {
   constexpr int MT[3][4] = {{3, 3, 3, 3}, {3, 2, 1, 0}, {0, 1, 2, 3}};
@@ -823,16 +828,35 @@ __device__ __forceinline__ void alf_pk_filter_rows(const uint32_t (&Q)[N][6], co
   for (int k = 0; k < 12; k++) {
     const uint32_t w = f[k], kk = w >> 16;
     const uint32_t c0 = w & 0xffffu, c1 = w << 16, clip = kk | ((0u - kk) << 16);
+    if (NOCLIP) {
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-      const int m = MT[VB][R0 + r];
-      const int dy = kTapDy[k] < m ? kTapDy[k] : m;
-      const int R = r + 3;  // centre row in Q
+      for (int r = 0; r < NR; r++) {
+        const int m = MT[VB][R0 + r];
+        const int dy = kTapDy[k] < m ? kTapDy[k] : m;
+        const int R = r + 3;
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int C = 3 + 2 * h;
-        alf_pk_tap(c0, c1, clip, as_s16x2(pw_pair(Q, R, C)), pw_pair(Q, R + dy, C + kTapDx[k]),
-                   pw_pair(Q, R - dy, C - kTapDx[k]), acc0[r][h], acc1[r][h]);
+        for (int h = 0; h < 2; h++) {
+          const int C = 3 + 2 * h;
+          const s16x2 cur = as_s16x2(pw_pair(Q, R, C));
+          // samples <= 4095 here: the sums stay inside i16
+          const s16x2 sum = as_s16x2(pw_pair(Q, R + dy, C + kTapDx[k])) +
+                            as_s16x2(pw_pair(Q, R - dy, C - kTapDx[k])) - (cur + cur);
+          acc0[r][h] = __builtin_amdgcn_sdot2(sum, as_s16x2(c0), acc0[r][h], false);
+          acc1[r][h] = __builtin_amdgcn_sdot2(sum, as_s16x2(c1), acc1[r][h], false);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const int m = MT[VB][R0 + r];
+        const int dy = kTapDy[k] < m ? kTapDy[k] : m;
+        const int R = r + 3;  // centre row in Q
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int C = 3 + 2 * h;
+          alf_pk_tap(c0, c1, clip, as_s16x2(pw_pair(Q, R, C)), pw_pair(Q, R + dy, C + kTapDx[k]),
+                     pw_pair(Q, R - dy, C - kTapDx[k]), acc0[r][h], acc1[r][h]);
+        }
       }
     }
   }
@@ -851,10 +875,10 @@ __device__ __forceinline__ void alf_pk_filter_rows(const uint32_t (&Q)[N][6], co
   }
 }
 
-template <int VB>
+template <int VB, bool NOCLIP = false>
 __device__ __forceinline__ void alf_pk_filter(const uint32_t (&P)[10][6], const uint32_t* f, int maxv,
                                               uint32_t (&out)[4][2]) {
-  alf_pk_filter_rows<VB, 0, 4, 10>(P, f, maxv, out);
+  alf_pk_filter_rows<VB, 0, 4, 10, NOCLIP>(P, f, maxv, out);
 }
 
 // Classification reading the window from LDS row by row (three window rows
@@ -991,16 +1015,28 @@ __device__ __forceinline__ int alf_pk_block_class(const uint32_t (&P)[10][6], in
 // Classification + filter of one block from the packed window; `bank` holds
 // the CTB's 25 x 4 x 12 tap words, yb = y4 - yCtb.
 __device__ __forceinline__ void alf_block_pk(const uint32_t (&P)[10][6], const uint32_t* bank, int yb, int ctb,
-                                             bool carve, int bd, int maxv, uint32_t (&out)[4][2]) {
+                                             bool carve, int bd, int maxv, uint32_t (&out)[4][2],
+                                             bool in_range = false) {
   int cls;
   const int vb = yb == ctb - 8 ? 1 : (yb == ctb - 4 ? 2 : 0);
   if (vb == 0) cls = alf_pk_class<0, false>(P, bd);
   else if (vb == 1) cls = carve ? alf_pk_class<1, true>(P, bd) : alf_pk_class<1, false>(P, bd);
   else cls = carve ? alf_pk_class<2, true>(P, bd) : alf_pk_class<2, false>(P, bd);
   const uint32_t* f = bank + ((cls & 31) * 4 + (cls >> 5)) * 12;
-  if (vb == 0) alf_pk_filter<0>(P, f, maxv, out);
-  else if (vb == 1) alf_pk_filter<1>(P, f, maxv, out);
-  else alf_pk_filter<2>(P, f, maxv, out);
+  // the unclipped form where it holds for every lane of the wave (a
+  // wave-uniform branch: fixed-set CTBs and APS sets without clipping)
+  bool noclip = in_range;
+#pragma unroll
+  for (int k = 0; k < 12; k++) noclip = noclip && (f[k] >> 16) > (uint32_t)maxv;
+  if (__all(noclip)) {
+    if (vb == 0) alf_pk_filter<0, true>(P, f, maxv, out);
+    else if (vb == 1) alf_pk_filter<1, true>(P, f, maxv, out);
+    else alf_pk_filter<2, true>(P, f, maxv, out);
+  } else {
+    if (vb == 0) alf_pk_filter<0>(P, f, maxv, out);
+    else if (vb == 1) alf_pk_filter<1>(P, f, maxv, out);
+    else alf_pk_filter<2>(P, f, maxv, out);
+  }
 }
 
 // ---------------------------------------------------------------------------

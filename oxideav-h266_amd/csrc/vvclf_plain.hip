@@ -414,7 +414,10 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
       });
       return;
     } else {
-      alf_block_pk(P, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outp);
+      // every window sample within the bit depth: taps whose clip value
+      // exceeds it take the unclipped form
+      const bool in_range = (((any & 0xffffu) | (any >> 16)) >> j.alf_bd) == 0u;
+      alf_block_pk(P, bank, y4 - y_ctb, ctb, carve, j.alf_bd, amax, outp, in_range);
     }
   }
   if (x4 + 4 <= src.w) {

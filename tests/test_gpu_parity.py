@@ -109,6 +109,46 @@ def test_stages_standalone(vv, orc):
         assert_same(fo.output_arrays(), fg.output_arrays(), "alf")
 
 
+@pytest.mark.parametrize("bd,above", [(10, False), (10, True), (8, True), (12, True)])
+def test_alf_unclipped_taps(vv, orc, bd, above):
+    """ALF luma where the filter sets clip nothing (every fixed set; an APS
+    without clipping): taps whose clip value exceeds the bit depth take the
+    unclipped form only while the window's samples stay within the bit depth.
+    `above` puts samples between 2^BitDepth and 2^15 into the picture, where
+    the clip still acts."""
+    say(f"alf unclipped taps bd={bd} above={above}")
+    rng = np.random.default_rng(bd * 2 + above)
+    W, H, ctb_log2 = 416, 240, 6
+    pic = Pic(vv, W, H, 0, bd=bd)
+    pic.y[:] = rng.integers(0, 1 << bd, (H, W))
+    if above:
+        n = W * H // 100
+        pic.y.ravel()[rng.choice(W * H, n, replace=False)] = rng.integers(1 << bd, 1 << 15, n)
+    aps = vv.AlfAps()
+    aps.alf_luma_filter_signal_flag = 1
+    aps.n_luma_coeff = 25
+    for k in range(25):
+        for t in range(12):
+            aps.luma_coeff[k][t] = int(rng.integers(-20, 21))
+    b = binding_of(vv, luma=[aps])
+    ctb = 1 << ctb_log2
+    wc, hc = (W + ctb - 1) // ctb, (H + ctb - 1) // ctb
+    ent = {}
+    for y in range(hc):
+        for x in range(wc):
+            c = vv.AlfCtb()
+            c.luma_on, c.luma_filt_set_idx = 1, int(rng.integers(0, 17))
+            ent[(x, y)] = c
+    ap, _k = alf_picture(vv, wc, hc, ent)
+    cfg = alf_cfg(vv, bd=bd, ctb_log2=ctb_log2)
+    o, g = pic.copy(), pic.copy()
+    assert orc.oracle_apply_alf_clipped(o.ptr(), C.byref(ap), C.byref(cfg), C.byref(b), None) == 0
+    assert vv.lib.vvclf_apply_alf_clipped(g.ptr(), C.byref(ap), C.byref(cfg), C.byref(b), None) == 0, \
+        vv.lib.vvclf_last_error()
+    assert np.array_equal(o.y, g.y)
+    assert (g.y != pic.y).any()
+
+
 def test_reference_kats_through_the_gpu(vv):
     """The reference's exact-value deblocking tests, run on the GPU path."""
     say("kat: weak p1 clip")
