@@ -337,6 +337,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-same8", action="store_true", help="skip the fused batch of the same 8 pictures")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (a second known-size kernel for the "
+                         "HBM traffic counters) and exit")
     ap.add_argument("--no-mixed64", action="store_true")
     ap.add_argument("--no-8k", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
@@ -348,6 +352,20 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (pictures shard by stream "
               f"across GPUs with no collectives, so the per-GPU figure is the one reported)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy > 0:
+        # device-to-device copies of a known size through torch (HIP's copy
+        # kernel), to scale FETCH_SIZE / WRITE_SIZE: nothing else runs
+        import torch
+        n = args.calibrate_copy << 20
+        a = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)                          # HIP's device-to-device copy
+            torch.bitwise_or(a, 0, out=b)       # an elementwise kernel: n bytes in, n out
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "passes": 4}), flush=True)
+        return
 
     vv = load("vvclf_amd", os.path.join(ROOT, "oxideav-h266_amd", "__init__.py"))
     orc = load_oracle(vv)
@@ -398,7 +416,7 @@ def main():
                      "valu": valu_for(wl, KERNEL_NAMES[dom], stage_ms[dom])},
     }
     out["batched"] = {}
-    if not args.no_batched:
+    if not args.no_batched and not args.no_same8:
         log("batched form, same pictures")
         bt = batched_run(vv, ctx, jobs, frames, refs, "fused batch 8x4K", reps=max(5, args.steps // 2))
         bt["speedup_vs_plain"] = round(bt["value"] / value, 2)
