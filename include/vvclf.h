@@ -414,6 +414,33 @@ int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t
                           uint32_t chroma_format_idc, const vvclf_alf_aps* const* apses, uint32_t n_aps,
                           const vvclf_bounds* tile_bounds, vvclf_alf_ctb* out, uint64_t* sse);
 
+/* Replaces chroma_alf_decide_and_apply_clipped (src/alf_enc.rs:628-806) for
+ * component c_idx (1 Cb, 2 Cr): every CTB's chroma is filtered with each
+ * alternative of `aps` (chroma_coeff / chroma_clip_idx rows 0 ..
+ * n_chroma_coeff - 1); per CTB the first alternative whose SSE against `src`
+ * (the chroma CTB clipped to the plane) is strictly below the SSE of leaving
+ * it unfiltered is kept: its samples are written into rec's plane and
+ * apply[i] gets cb_on / cb_alt_idx (cr_on / cr_alt_idx); the records of other
+ * CTBs, and every other field, keep what they held (apply: raster CTB order,
+ * ceil(W / CtbSizeY) x ceil(H / CtbSizeY)).  `sse` (optional) receives per CTB
+ * 1 + n_chroma_coeff values: off, then each alternative.  A no-op returning
+ * VVCLF_OK where the reference returns early (chroma_format_idc 0, no
+ * signalled chroma filter); 4:2:0 only on the GPU.  Host pictures. */
+int vvclf_alf_decide_chroma(const vvclf_picture* src, vvclf_picture* rec, vvclf_alf_ctb* apply,
+                            const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2_size_y, uint32_t bit_depth,
+                            uint32_t chroma_format_idc, const vvclf_bounds* tile_bounds, uint64_t* sse);
+
+/* Replaces cc_alf_decide_and_apply_clipped (src/alf_enc.rs:935-1140): the
+ * same decision over the CC-ALF filters of `aps` for component c_idx
+ * (cc_cb_coeff / cc_cr_coeff rows 0 .. n - 1; idc k + 1), the correction
+ * taken from `pre_luma`, the luma plane before luma ALF (as the reference's
+ * pre_luma_alf_samples) and added to rec's chroma; apply[i].cc_cb_idc /
+ * cc_cr_idc of the CTBs that take a filter.  `sse`: off, then each filter. */
+int vvclf_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plane* pre_luma,
+                        vvclf_alf_ctb* apply, const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2_size_y,
+                        uint32_t bit_depth, uint32_t chroma_format_idc, const vvclf_bounds* tile_bounds,
+                        uint64_t* sse);
+
 /* ------------------------------------------------------------------------ */
 /* Per-CTB side information as a compact record: the syntax elements a      */
 /* slice decoder has parsed for one CTB, from which the SAO and ALF CTB      */

@@ -73,6 +73,10 @@ def load(vv=None, path=None):
         "oracle_apply_in_loop_filters": (C.c_int, [P(vv.Frame)]),
         "oracle_alf_decide_luma": (C.c_int, [P(vv.Picture), P(vv.Picture), u32, u32, u32, P(P(vv.AlfAps)), u32,
                                              P(vv.Bounds), P(vv.AlfCtb), P(C.c_uint64)]),
+        "oracle_alf_decide_chroma": (C.c_int, [P(vv.Picture), P(vv.Picture), P(vv.AlfCtb), P(vv.AlfAps), u32, u32,
+                                               u32, u32, P(vv.Bounds), P(C.c_uint64)]),
+        "oracle_cc_alf_decide": (C.c_int, [P(vv.Picture), P(vv.Picture), P(vv.Plane), P(vv.AlfCtb), P(vv.AlfAps),
+                                           u32, u32, u32, u32, P(vv.Bounds), P(C.c_uint64)]),
         "oracle_apply_in_loop_filters_mt": (C.c_int, [P(P(vv.Frame)), u32, u32]),
     }
     for name, (res, args) in sig.items():

@@ -1623,6 +1623,138 @@ int oracle_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_
   return rc;
 }
 
+/* Shared tail of the chroma decisions (src/alf_enc.rs:766-806, :1092-1139):
+ * per CTB the first trial strictly below the off SSE; its samples are copied
+ * into the component plane and mark(ctb, k) records the choice. */
+static void chroma_pick_commit(vvclf_plane* dst, uint16_t* const* trial, const uint64_t* tab, uint32_t ncand,
+                               uint32_t wc, uint32_t hc, uint32_t cw, uint32_t ch, vvclf_alf_ctb* apply, uint32_t c_idx,
+                               int cc) {
+  for (uint32_t ry = 0; ry < hc; ry++)
+    for (uint32_t rx = 0; rx < wc; rx++) {
+      size_t i = (size_t)ry * wc + rx;
+      uint64_t best = tab[i * ncand];
+      int best_k = -1;
+      for (uint32_t k = 0; k + 1 < ncand; k++)
+        if (tab[i * ncand + 1 + k] < best) {
+          best = tab[i * ncand + 1 + k];
+          best_k = (int)k;
+        }
+      if (best_k < 0) continue;
+      uint32_t x0 = rx * cw, y0 = ry * ch;
+      for (uint32_t y = y0; y < y0 + ch && y < dst->height; y++)
+        for (uint32_t x = x0; x < x0 + cw && x < dst->width; x++) {
+          size_t at = (size_t)y * dst->stride + x;
+          dst->samples[at] = trial[best_k][at];
+        }
+      vvclf_alf_ctb* e = &apply[i];
+      if (cc) {
+        if (c_idx == 1) e->cc_cb_idc = (uint8_t)(best_k + 1);
+        else e->cc_cr_idc = (uint8_t)(best_k + 1);
+      } else if (c_idx == 1) {
+        e->cb_on = 1;
+        e->cb_alt_idx = (uint8_t)best_k;
+      } else {
+        e->cr_on = 1;
+        e->cr_alt_idx = (uint8_t)best_k;
+      }
+    }
+}
+
+/* chroma_alf_decide_and_apply_clipped, src/alf_enc.rs:628-806 (c_idx 1 Cb,
+ * 2 Cr), and cc_alf_decide_and_apply_clipped, :935-1140 (cc = 1, with the
+ * pre-luma-ALF plane pre_luma).  A no-op returning VVCLF_OK where the
+ * reference returns early (4:0:0, no signalled filter). */
+static int chroma_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plane* pre_luma,
+                         vvclf_alf_ctb* apply, const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2,
+                         uint32_t bd, uint32_t cfi, const vvclf_bounds* tb, uint64_t* sse, int cc) {
+  if (cfi == 0) return VVCLF_OK;
+  uint32_t n;
+  if (cc) {
+    n = c_idx == 1 ? aps->n_cc_cb_coeff : aps->n_cc_cr_coeff;
+  } else {
+    if (!aps->alf_chroma_filter_signal_flag) return VVCLF_OK;
+    n = aps->n_chroma_coeff;
+  }
+  if (n == 0) return VVCLF_OK;
+  uint32_t ctb = 1u << ctb_log2, sw, sh;
+  sub_of(cfi, &sw, &sh);
+  uint32_t wc = (rec->luma.width + ctb - 1) / ctb, hc = (rec->luma.height + ctb - 1) / ctb;
+  uint32_t cw = ctb / sw, ch = ctb / sh;
+  size_t nc = (size_t)wc * hc;
+  uint32_t ncand = 1 + n;
+  const vvclf_plane* sp = c_idx == 1 ? &src->cb : &src->cr;
+  vvclf_plane* rp = c_idx == 1 ? &rec->cb : &rec->cr;
+  uint64_t* tab = (uint64_t*)calloc(nc * ncand, sizeof(uint64_t));
+  for (uint32_t ry = 0; ry < hc; ry++)
+    for (uint32_t rx = 0; rx < wc; rx++)
+      tab[((size_t)ry * wc + rx) * ncand] = ctb_sse_y(sp, rp, rx * cw, ry * ch, cw, ch);
+  vvclf_alf_config cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.alf_enabled = 1;
+  cfg.cb_enabled = cfg.cc_cb_enabled = c_idx == 1;
+  cfg.cr_enabled = cfg.cc_cr_enabled = c_idx == 2;
+  cfg.bit_depth = bd;
+  cfg.ctb_log2_size_y = ctb_log2;
+  cfg.chroma_format_idc = cfi;
+  vvclf_alf_aps_binding b;
+  memset(&b, 0, sizeof(b));
+  if (!cc) b.chroma_aps = aps;
+  else if (c_idx == 1) b.cc_cb_aps = aps;
+  else b.cc_cr_aps = aps;
+  size_t plane = (size_t)rp->stride * rp->height;
+  uint16_t** trial = (uint16_t**)calloc(n, sizeof(uint16_t*));
+  vvclf_alf_ctb* pic = (vvclf_alf_ctb*)calloc(nc, sizeof(vvclf_alf_ctb));
+  int rc = VVCLF_OK;
+  for (uint32_t k = 0; k < n && rc == VVCLF_OK; k++) {
+    vvclf_picture t = *rec;
+    if (cc) t.luma = *pre_luma; /* the CC-ALF source is the pre-luma-ALF plane */
+    trial[k] = (uint16_t*)malloc(plane * 2 + 2);
+    memcpy(trial[k], rp->samples, plane * 2);
+    if (c_idx == 1) t.cb.samples = trial[k];
+    else t.cr.samples = trial[k];
+    for (size_t i = 0; i < nc; i++) {
+      memset(&pic[i], 0, sizeof(pic[i]));
+      if (cc) {
+        if (c_idx == 1) pic[i].cc_cb_idc = (uint8_t)(k + 1);
+        else pic[i].cc_cr_idc = (uint8_t)(k + 1);
+      } else if (c_idx == 1) {
+        pic[i].cb_on = 1;
+        pic[i].cb_alt_idx = (uint8_t)k;
+      } else {
+        pic[i].cr_on = 1;
+        pic[i].cr_alt_idx = (uint8_t)k;
+      }
+    }
+    vvclf_alf_picture ap = {wc, hc, pic};
+    rc = oracle_apply_alf_clipped(&t, &ap, &cfg, &b, tb);
+    const vvclf_plane* tp = c_idx == 1 ? &t.cb : &t.cr;
+    for (uint32_t ry = 0; ry < hc; ry++)
+      for (uint32_t rx = 0; rx < wc; rx++)
+        tab[((size_t)ry * wc + rx) * ncand + 1 + k] = ctb_sse_y(sp, tp, rx * cw, ry * ch, cw, ch);
+  }
+  if (rc == VVCLF_OK) {
+    chroma_pick_commit(rp, trial, tab, ncand, wc, hc, cw, ch, apply, c_idx, cc);
+    if (sse) memcpy(sse, tab, nc * ncand * sizeof(uint64_t));
+  }
+  for (uint32_t k = 0; k < n; k++) free(trial[k]);
+  free(trial);
+  free(pic);
+  free(tab);
+  return rc;
+}
+
+int oracle_alf_decide_chroma(const vvclf_picture* src, vvclf_picture* rec, vvclf_alf_ctb* apply,
+                             const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2, uint32_t bd, uint32_t cfi,
+                             const vvclf_bounds* tb, uint64_t* sse) {
+  return chroma_decide(src, rec, NULL, apply, aps, c_idx, ctb_log2, bd, cfi, tb, sse, 0);
+}
+
+int oracle_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plane* pre_luma,
+                         vvclf_alf_ctb* apply, const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2,
+                         uint32_t bd, uint32_t cfi, const vvclf_bounds* tb, uint64_t* sse) {
+  return chroma_decide(src, rec, pre_luma, apply, aps, c_idx, ctb_log2, bd, cfi, tb, sse, 1);
+}
+
 int oracle_apply_in_loop_filters(vvclf_frame* f) {
   static const vvclf_deblock_cu empty_cus[1];
   const vvclf_deblock_cu* ccus = f->has_chroma_cus ? (f->chroma_cus ? f->chroma_cus : empty_cus) : NULL;

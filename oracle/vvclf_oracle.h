@@ -89,6 +89,16 @@ int oracle_apply_alf_clipped(vvclf_picture* out, const vvclf_alf_picture* alf_pi
 int oracle_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd, uint32_t cfi,
                            const vvclf_alf_aps* const* apses, uint32_t n_aps, const vvclf_bounds* tb,
                            vvclf_alf_ctb* out, uint64_t* sse);
+/* src/alf_enc.rs:628-806: chroma ALF alternative per CTB for component c_idx
+ * (1 Cb, 2 Cr); apply is the CTB array (in / out), sse [n_ctbs][1 + n_alts] */
+int oracle_alf_decide_chroma(const vvclf_picture* src, vvclf_picture* rec, vvclf_alf_ctb* apply,
+                             const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2, uint32_t bd, uint32_t cfi,
+                             const vvclf_bounds* tb, uint64_t* sse);
+/* src/alf_enc.rs:935-1140: CC-ALF filter per CTB from the pre-luma-ALF plane;
+ * sse [n_ctbs][1 + n_filters] */
+int oracle_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plane* pre_luma,
+                         vvclf_alf_ctb* apply, const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2,
+                         uint32_t bd, uint32_t cfi, const vvclf_bounds* tb, uint64_t* sse);
 
 /* src/ctu.rs:11822-12066 */
 int oracle_apply_in_loop_filters(vvclf_frame* frame);

@@ -246,6 +246,10 @@ SIGNATURES = {
     "vvclf_job_output_md5": (C.c_int, [VP, u32, P(Crop), P((u8 * 16) * 3)]),
     "vvclf_alf_decide_luma": (C.c_int, [P(Picture), P(Picture), u32, u32, u32, P(P(AlfAps)), u32, P(Bounds),
                                          P(AlfCtb), P(u64)]),
+    "vvclf_alf_decide_chroma": (C.c_int, [P(Picture), P(Picture), P(AlfCtb), P(AlfAps), u32, u32, u32, u32,
+                                           P(Bounds), P(u64)]),
+    "vvclf_cc_alf_decide": (C.c_int, [P(Picture), P(Picture), P(Plane), P(AlfCtb), P(AlfAps), u32, u32, u32, u32,
+                                       P(Bounds), P(u64)]),
     "vvclf_ctb_syntax_decode": (C.c_int, [P(CtbSyntax), u32, u32, P(CtbSyntaxConfig), P(SaoCtbParams), P(AlfCtb)]),
     "vvclf_ctb_syntax_encode": (C.c_int, [P(SaoCtbParams), P(AlfCtb), u32, u32, P(CtbSyntaxConfig), P(CtbSyntax)]),
     "vvclf_server_submit_syntax": (C.c_int, [VP, u32, P(Frame), P(CtbSyntax), P(CtbSyntaxConfig), P(u64)]),

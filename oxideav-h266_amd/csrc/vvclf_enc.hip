@@ -239,6 +239,112 @@ struct dbuf {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Chroma decisions (4:2:0): chroma ALF alternatives
+// (chroma_alf_decide_and_apply_clipped, src/alf_enc.rs:628-806) and CC-ALF
+// filters (cc_alf_decide_and_apply_clipped, :935-1140).  One thread per chroma
+// sample over T x T tiles, T = min(chroma CTB, 16): a tile, and so a wave,
+// lies in one CTB.  Per candidate the sample's squared error against the
+// source is reduced over the wave into sse[ctb][1 + cand]; candidate 0 is off
+// (the input sample).
+// ---------------------------------------------------------------------------
+struct cenc_args {
+  const uint16_t* rec;   // the component before the decision (pitch cp)
+  const uint16_t* src;   // source component (pitch cp)
+  uint16_t* out;         // commit output (pitch cp)
+  const uint16_t* luma;  // CC-ALF: the luma plane before luma ALF (pitch lp)
+  int cp, lp, wc, hc, w, h;
+  int ctb_log2, wctb, bd;
+  const uint64_t* rect;   // per CTB luma fetch rectangle (pack_rect)
+  const uint32_t* cbank;  // chroma alternatives, 6 tap words each
+  const int32_t* ccbank;  // CC filters of the component, 7 taps each
+  int n_cand;
+  unsigned long long* sse;  // [n_ctbs][1 + n_cand]
+  const int32_t* choice;    // [n_ctbs]
+};
+
+// Chroma ALF of sample (xs, y) with alternative f (the per-sample form of
+// k_alf_chroma, src/alf.rs:1014-1065): taps of the 5x5 diamond read clamped
+// to the CTB's chroma rectangle, Table 46 rows near the virtual boundary.
+__device__ __forceinline__ int chroma_alf_at(const cenc_args& a, int xs, int y, int cx0, int cy0, int cx1, int cy1,
+                                             const uint32_t* f, int sft, int y1, int y2, int maxv) {
+  auto S = [&](int dx, int dy) -> int {
+    return a.rec[(size_t)clampi(y + dy, cy0, cy1) * a.cp + clampi(xs + dx, cx0, cx1)];
+  };
+  const int c0 = S(0, 0);
+  int sum = alf_pair(f[0], S(0, y2), S(0, -y2), c0);
+  sum += alf_pair(f[1], S(1, y1), S(-1, -y1), c0);
+  sum += alf_pair(f[2], S(0, y1), S(0, -y1), c0);
+  sum += alf_pair(f[3], S(-1, y1), S(1, -y1), c0);
+  sum += alf_pair(f[4], S(2, 0), S(-2, 0), c0);
+  sum += alf_pair(f[5], S(1, 0), S(-1, 0), c0);
+  return clampi(c0 + ((sum + (1 << (sft - 1))) >> sft), 0, maxv);
+}
+
+// CC-ALF correction of chroma sample (xs, y) (src/alf.rs:447-523): the seven
+// luma taps around (2 xs, 2 y), rows by Table 47, clamped to the CTB's luma
+// rectangle [lx0, lx1) x [ly0, ly1).
+__device__ __forceinline__ int cc_delta_at(const cenc_args& a, int xs, int y, int lx0, int ly0, int lx1, int ly1,
+                                           const cc_taps& k, int yp1, int yp2, int half) {
+  const int yl = 2 * y, xl = 2 * xs;
+  auto row = [&](int dy) { return a.luma + (size_t)clampi(yl + dy, ly0, ly1 - 1) * a.lp; };
+  const uint16_t *la = row(-yp1), *lc = row(0), *lb = row(yp1), *ld = row(yp2);
+  auto cx = [&](int dx) { return clampi(xl + dx, lx0, lx1 - 1); };
+  const int sum = k.sum(la[cx(0)], lc[cx(-1)], lc[cx(1)], lb[cx(-1)], lb[cx(0)], lb[cx(1)], ld[cx(0)], lc[cx(0)]);
+  return clampi((sum + 64) >> 7, -half, half - 1);
+}
+
+template <bool TRIAL, bool CC>
+__global__ void __launch_bounds__(256) k_chroma_enc(cenc_args a, int tile_log2) {
+  const int T = 1 << tile_log2;
+  const int tx = threadIdx.x % T, ty = threadIdx.x / T;
+  const int x = blockIdx.x * T + tx, y = blockIdx.y * T + ty;
+  const bool live = ty < T && x < a.wc && y < a.hc;
+  const int ctb = 1 << a.ctb_log2, ctbc = ctb >> 1;
+  // the tile's CTB (the tile lies in one)
+  const int rx = (2 * (int)(blockIdx.x * T)) >> a.ctb_log2, ry = (2 * (int)(blockIdx.y * T)) >> a.ctb_log2;
+  const size_t ci = (size_t)ry * a.wctb + rx;
+  const uint64_t r = a.rect[ci];
+  const int lx0 = (int)(r & 0xffff), ly0 = (int)((r >> 16) & 0xffff), lx1 = (int)((r >> 32) & 0xffff),
+            ly1 = (int)(r >> 48);
+  const int maxv = (1 << a.bd) - 1;
+  const int cur = live ? (int)a.rec[(size_t)y * a.cp + x] : 0;
+  // per-candidate geometry
+  int sft = 7, y1 = 1, y2 = 2, yp1 = 1, yp2 = 2;
+  const int cx0 = lx0 >> 1, cy0 = ly0 >> 1, cx1 = ((lx1 + 1) >> 1) - 1, cy1 = ((ly1 + 1) >> 1) - 1;
+  if (CC) {
+    const int yl = 2 * y, y_ctb_c = ry * ctbc;
+    const bool lbb = !(yl + ctb >= a.h && (a.h - y_ctb_c * 2) <= ctb - 4);
+    alf_t47((y - y_ctb_c) * 2, ctb, lbb, yp1, yp2);
+  } else {
+    alf_t46(y - ry * ctbc, ctbc, sft, y1, y2);
+  }
+  const int half = 1 << (a.bd - 1);
+  auto candidate = [&](int c) -> int {
+    if (CC) {
+      const cc_taps k(a.ccbank + c * 7);
+      return clampi(cur + cc_delta_at(a, x, y, lx0, ly0, lx1, ly1, k, yp1, yp2, half), 0, maxv);
+    }
+    return chroma_alf_at(a, x, y, cx0, cy0, cx1, cy1, a.cbank + c * 6, sft, y1, y2, maxv);
+  };
+  if (TRIAL) {
+    const int s = live ? (int)a.src[(size_t)y * a.cp + x] : 0;
+    unsigned long long* row = a.sse + ci * (size_t)(1 + a.n_cand);
+    long long d = s - cur;
+    unsigned long long e = wave_sum64((unsigned long long)(d * d));
+    if ((threadIdx.x & 63) == 0 && e) atomicAdd(row, e);
+    for (int c = 0; c < a.n_cand; c++) {
+      d = live ? (long long)s - candidate(c) : 0;
+      e = wave_sum64((unsigned long long)(d * d));
+      if ((threadIdx.x & 63) == 0 && e) atomicAdd(row + 1 + c, e);
+    }
+  } else {
+    if (!live) return;
+    const int c = a.choice[ci];
+    a.out[(size_t)y * a.cp + x] = (uint16_t)(c >= 0 ? candidate(c) : cur);
+  }
+}
+
 }  // namespace
 
 extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd,
@@ -369,3 +475,173 @@ extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* re
   if (sse_out) memcpy(sse_out, sse.data(), sse.size() * 8);
   return VVCLF_OK;
 }
+
+namespace {
+
+// Shared host part of the two chroma decisions.
+int chroma_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plane* pre_luma, vvclf_alf_ctb* apply,
+                  const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2, uint32_t bd, uint32_t cfi,
+                  const vvclf_bounds* tb, uint64_t* sse_out, bool cc) {
+  const char* name = cc ? "cc_alf_decide" : "alf_decide_chroma";
+  if (!src || !rec || !apply || !aps || (cc && !pre_luma)) return fail(VVCLF_ERR_INVALID, "%s: NULL argument", name);
+  if (c_idx != 1 && c_idx != 2) return fail(VVCLF_ERR_INVALID, "%s: component %u (1 Cb, 2 Cr)", name, c_idx);
+  if (cfi == 0) return VVCLF_OK;  // monochrome: nothing to decide (as the reference)
+  if (cfi != 1) return fail(VVCLF_ERR_UNSUPPORTED, "%s: chroma_format_idc %u (4:2:0 only)", name, cfi);
+  uint32_t n;
+  if (cc) {
+    n = c_idx == 1 ? aps->n_cc_cb_coeff : aps->n_cc_cr_coeff;
+    if (n > VVCLF_ALF_MAX_CC_FILTERS) return fail(VVCLF_ERR_INVALID, "%s: %u CC-ALF filters", name, n);
+  } else {
+    if (!aps->alf_chroma_filter_signal_flag) return VVCLF_OK;
+    n = aps->n_chroma_coeff;
+    if (n > VVCLF_ALF_MAX_CHROMA_ALT) return fail(VVCLF_ERR_INVALID, "%s: %u chroma alternatives", name, n);
+  }
+  if (n == 0) return VVCLF_OK;
+  if (bd < 8 || bd > 16) return fail(VVCLF_ERR_INVALID, "%s: bit depth %u", name, bd);
+  if (ctb_log2 < 3 || ctb_log2 > 8) return fail(VVCLF_ERR_UNSUPPORTED, "%s: ctb_log2_size_y %u", name, ctb_log2);
+  const vvclf_plane &SP = c_idx == 1 ? src->cb : src->cr, &RP = c_idx == 1 ? rec->cb : rec->cr;
+  const uint32_t W = rec->luma.width, H = rec->luma.height, Wc = RP.width, Hc = RP.height;
+  if (!SP.samples || !RP.samples || SP.width != Wc || SP.height != Hc || Wc != (W + 1) / 2 || Hc != (H + 1) / 2 ||
+      Wc == 0 || Hc == 0)
+    return fail(VVCLF_ERR_INVALID, "%s: chroma planes do not match a 4:2:0 %ux%u picture", name, W, H);
+  if (W > 65535 || H > 65535) return fail(VVCLF_ERR_UNSUPPORTED, "%s: picture above 65535 samples", name);
+  if (cc && (!pre_luma->samples || pre_luma->width != W || pre_luma->height != H || pre_luma->stride < W))
+    return fail(VVCLF_ERR_INVALID, "%s: the pre-ALF luma plane does not match the picture", name);
+  // coefficient ranges of the filter lanes (as the ALF stage)
+  for (uint32_t k = 0; k < n; k++)
+    for (int t = 0; t < (cc ? VVCLF_ALF_CC_NUM_COEFF : 6); t++) {
+      const int32_t v = cc ? (c_idx == 1 ? aps->cc_cb_coeff[k][t] : aps->cc_cr_coeff[k][t]) : aps->chroma_coeff[k][t];
+      if (cc ? (v < -(1 << 20) || v >= (1 << 20)) : (v < -32768 || v > 32767))
+        return fail(VVCLF_ERR_UNSUPPORTED, "%s: coefficient %d outside the filter lanes", name, v);
+    }
+  const uint32_t ctb = 1u << ctb_log2, wctb = (W + ctb - 1) / ctb, hctb = (H + ctb - 1) / ctb;
+  const uint32_t n_ctbs = wctb * hctb;
+  vvclf_alf_config ac;
+  memset(&ac, 0, sizeof(ac));
+  ac.alf_enabled = 1;
+  ac.bit_depth = bd;
+  ac.ctb_log2_size_y = ctb_log2;
+  ac.chroma_format_idc = cfi;
+  vvclf_alf_aps_binding b;
+  memset(&b, 0, sizeof(b));
+  if (!cc) b.chroma_aps = aps;
+  else if (c_idx == 1) b.cc_cb_aps = aps;
+  else b.cc_cr_aps = aps;
+  host_tables t;
+  build_alf_banks(ac, b, t);
+  std::vector<uint64_t> rects(n_ctbs);
+  for (uint32_t ry = 0; ry < hctb; ry++)
+    for (uint32_t rx = 0; rx < wctb; rx++) {
+      rects[(size_t)ry * wctb + rx] = ctb_fetch_rect(tb, rx, ry, ctb, W, H);
+      if (!rects[(size_t)ry * wctb + rx]) return fail(VVCLF_ERR_INVALID, "%s: empty clip rectangle", name);
+    }
+  std::vector<uint8_t> ok(n, 1);
+  vvclf_ctx* ctx = default_ctx();
+  if (!ctx) return VVCLF_ERR_HIP;
+  hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  const uint32_t cpitch = (Wc + 63) / 64 * 64, lpitch = (W + 63) / 64 * 64;
+  const size_t cplane = (size_t)cpitch * Hc * 2, lplane = cc ? (size_t)lpitch * H * 2 : 0;
+  dbuf drec, dsrc, dout, dl, drect, dcb, dccb, dsse, dchoice, dok;
+  hipError_t e = drec.alloc(cplane, st);
+  if (e == hipSuccess) e = dsrc.alloc(cplane, st);
+  if (e == hipSuccess) e = dout.alloc(cplane, st);
+  if (e == hipSuccess) e = dl.alloc(lplane, st);
+  if (e == hipSuccess) e = drect.alloc(rects.size() * 8, st);
+  if (e == hipSuccess) e = dcb.alloc(t.alf_cbank.size() * 4, st);
+  if (e == hipSuccess) e = dccb.alloc(t.alf_ccbank.size() * 4, st);
+  if (e == hipSuccess) e = dsse.alloc((size_t)n_ctbs * (1 + n) * 8, st);
+  if (e == hipSuccess) e = dchoice.alloc((size_t)n_ctbs * 4, st);
+  if (e == hipSuccess) e = dok.alloc(n, st);
+  if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "%s: device buffers: %s", name, hipGetErrorString(e));
+  e = hipMemcpy2DAsync(drec.p, (size_t)cpitch * 2, RP.samples, (size_t)RP.stride * 2, (size_t)Wc * 2, Hc,
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(dsrc.p, (size_t)cpitch * 2, SP.samples, (size_t)SP.stride * 2, (size_t)Wc * 2, Hc,
+                         hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && cc)
+    e = hipMemcpy2DAsync(dl.p, (size_t)lpitch * 2, pre_luma->samples, (size_t)pre_luma->stride * 2, (size_t)W * 2, H,
+                         hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(drect.p, rects.data(), rects.size() * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dcb.p, t.alf_cbank.data(), t.alf_cbank.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dccb.p, t.alf_ccbank.data(), t.alf_ccbank.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dok.p, ok.data(), n, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(dsse.p, 0, (size_t)n_ctbs * (1 + n) * 8, st);
+  cenc_args a;
+  a.rec = (const uint16_t*)drec.p;
+  a.src = (const uint16_t*)dsrc.p;
+  a.out = (uint16_t*)dout.p;
+  a.luma = (const uint16_t*)dl.p;
+  a.cp = (int)cpitch;
+  a.lp = (int)lpitch;
+  a.wc = (int)Wc;
+  a.hc = (int)Hc;
+  a.w = (int)W;
+  a.h = (int)H;
+  a.ctb_log2 = (int)ctb_log2;
+  a.wctb = (int)wctb;
+  a.bd = (int)bd;
+  a.rect = (const uint64_t*)drect.p;
+  a.cbank = (const uint32_t*)dcb.p;
+  a.ccbank = (const int32_t*)dccb.p + (c_idx - 1) * ALF_CCBANK_WORDS;
+  a.n_cand = (int)n;
+  a.sse = (unsigned long long*)dsse.p;
+  a.choice = (const int32_t*)dchoice.p;
+  const int tl = std::min<int>((int)ctb_log2 - 1, 4), T = 1 << tl;
+  const dim3 grid((Wc + T - 1) / T, (Hc + T - 1) / T);
+  const unsigned threads = std::max(64u, (unsigned)(T * T));
+  if (e == hipSuccess) {
+    if (cc) hipLaunchKernelGGL((k_chroma_enc<true, true>), grid, dim3(threads), 0, st, a, tl);
+    else hipLaunchKernelGGL((k_chroma_enc<true, false>), grid, dim3(threads), 0, st, a, tl);
+    hipLaunchKernelGGL(k_alf_pick, dim3((n_ctbs + 255) / 256), dim3(256), 0, st, (unsigned long long*)dsse.p,
+                       (const uint8_t*)dok.p, (int)n_ctbs, (int)n, (int32_t*)dchoice.p);
+    if (cc) hipLaunchKernelGGL((k_chroma_enc<false, true>), grid, dim3(threads), 0, st, a, tl);
+    else hipLaunchKernelGGL((k_chroma_enc<false, false>), grid, dim3(threads), 0, st, a, tl);
+    e = hipGetLastError();
+  }
+  std::vector<int32_t> choice(n_ctbs);
+  std::vector<uint64_t> sse;
+  if (sse_out) sse.resize((size_t)n_ctbs * (1 + n));
+  if (e == hipSuccess) e = hipMemcpyAsync(choice.data(), dchoice.p, (size_t)n_ctbs * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && sse_out) e = hipMemcpyAsync(sse.data(), dsse.p, sse.size() * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(RP.samples, (size_t)RP.stride * 2, dout.p, (size_t)cpitch * 2, (size_t)Wc * 2, Hc,
+                         hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, name);
+  for (uint32_t i = 0; i < n_ctbs; i++) {
+    const int k = choice[i];
+    if (k < 0) continue;  // the CTB's record keeps what it held
+    if (cc) {
+      if (c_idx == 1) apply[i].cc_cb_idc = (uint8_t)(k + 1);
+      else apply[i].cc_cr_idc = (uint8_t)(k + 1);
+    } else if (c_idx == 1) {
+      apply[i].cb_on = 1;
+      apply[i].cb_alt_idx = (uint8_t)k;
+    } else {
+      apply[i].cr_on = 1;
+      apply[i].cr_alt_idx = (uint8_t)k;
+    }
+  }
+  if (sse_out) memcpy(sse_out, sse.data(), sse.size() * 8);
+  return VVCLF_OK;
+}
+
+}  // namespace
+
+extern "C" int vvclf_alf_decide_chroma(const vvclf_picture* src, vvclf_picture* rec, vvclf_alf_ctb* apply,
+                                       const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2, uint32_t bd,
+                                       uint32_t cfi, const vvclf_bounds* tb, uint64_t* sse) {
+  clear_error();
+  return chroma_decide(src, rec, nullptr, apply, aps, c_idx, ctb_log2, bd, cfi, tb, sse, false);
+}
+
+extern "C" int vvclf_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plane* pre_luma,
+                                   vvclf_alf_ctb* apply, const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2,
+                                   uint32_t bd, uint32_t cfi, const vvclf_bounds* tb, uint64_t* sse) {
+  clear_error();
+  return chroma_decide(src, rec, pre_luma, apply, aps, c_idx, ctb_log2, bd, cfi, tb, sse, true);
+}
+

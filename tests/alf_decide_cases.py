@@ -92,3 +92,64 @@ def decide(vv, fn, src, rec, ctb_log2, bd, apses=(), tb=None):
     if rc == 0:
         assert all(e == (0, 0, 0, 0, 0, 0) for e in extra)
     return rc, choice, sse, out
+
+
+def chroma_aps(vv, rng, n_alts, scale=24, clip=True):
+    """A chroma APS with n_alts random alternatives."""
+    a = vv.AlfAps()
+    a.alf_chroma_filter_signal_flag = 1
+    a.alf_chroma_clip_flag = int(clip)
+    a.n_chroma_coeff = n_alts
+    a.alf_chroma_num_alt_filters_minus1 = max(0, n_alts - 1)
+    for k in range(n_alts):
+        for t in range(6):
+            a.chroma_coeff[k][t] = int(rng.integers(-scale, scale + 1))
+            a.chroma_clip_idx[k][t] = int(rng.integers(0, 4)) if clip else 0
+    return a
+
+
+def cc_aps(vv, rng, c_idx, n, scale=32, rows=None):
+    """A CC-ALF APS with n random filters for component c_idx (or `rows`)."""
+    a = vv.AlfAps()
+    rows = rows if rows is not None else [[int(rng.integers(-scale, scale + 1)) for _ in range(7)] for _ in range(n)]
+    coeff = a.cc_cb_coeff if c_idx == 1 else a.cc_cr_coeff
+    for k, r in enumerate(rows):
+        for t in range(7):
+            coeff[k][t] = r[t]
+    if c_idx == 1:
+        a.alf_cc_cb_filter_signal_flag, a.n_cc_cb_coeff = 1, len(rows)
+    else:
+        a.alf_cc_cr_filter_signal_flag, a.n_cc_cr_coeff = 1, len(rows)
+    return a
+
+
+def ctb_fields(c):
+    return (c.luma_on, c.luma_filt_set_idx, c.cb_on, c.cb_alt_idx, c.cr_on, c.cr_alt_idx, c.cc_cb_idc, c.cc_cr_idc)
+
+
+def decide_chroma(vv, fn, src, rec, aps, c_idx, ctb_log2, bd, tb=None, pre_luma=None, init=None, cfi=1):
+    """Runs a chroma decision (oracle_alf_decide_chroma / vvclf_alf_decide_chroma,
+    or with pre_luma the CC-ALF forms) on a copy of rec and of the CTB records
+    `init` (list of field tuples, default all zero); returns (rc, records,
+    sse, picture)."""
+    out = rec.copy()
+    h, w = out.y.shape
+    ctb = 1 << ctb_log2
+    n = ((w + ctb - 1) // ctb) * ((h + ctb - 1) // ctb)
+    apply = (vv.AlfCtb * n)()
+    for i, f in enumerate(init or []):
+        (apply[i].luma_on, apply[i].luma_filt_set_idx, apply[i].cb_on, apply[i].cb_alt_idx, apply[i].cr_on,
+         apply[i].cr_alt_idx, apply[i].cc_cb_idc, apply[i].cc_cr_idc) = f
+    if pre_luma is None:
+        ncand = aps.n_chroma_coeff
+    else:
+        ncand = aps.n_cc_cb_coeff if c_idx == 1 else aps.n_cc_cr_coeff
+    sse = np.zeros((n, 1 + ncand), np.uint64)
+    args = [src.ptr(), out.ptr()]
+    if pre_luma is not None:
+        pl = vv.Plane(pre_luma.ctypes.data_as(C.POINTER(C.c_uint16)), pre_luma.shape[1], pre_luma.shape[1],
+                      pre_luma.shape[0], bd)
+        args.append(C.byref(pl))
+    rc = fn(*args, apply, C.byref(aps), c_idx, ctb_log2, bd, cfi, C.byref(tb) if tb is not None else None,
+            sse.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return rc, [ctb_fields(apply[i]) for i in range(n)], sse, out

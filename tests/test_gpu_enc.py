@@ -7,7 +7,8 @@ import sys
 import numpy as np
 import pytest
 
-from alf_decide_cases import decide, luma_aps, noisy, random_aps, smooth_gradient, tile_bounds
+from alf_decide_cases import decide, decide_chroma, luma_aps, noisy, random_aps, smooth_gradient, tile_bounds
+from test_alf_decide_oracle import cc_fixture, chroma_fixture, lowpass_chroma_aps
 
 pytestmark = pytest.mark.gpu
 
@@ -80,3 +81,62 @@ def test_decide_rejects_short_aps(vv):
     short.n_luma_coeff = 5
     rc, *_ = decide(vv, vv.lib.vvclf_alf_decide_luma, src, src, 6, 8, [short])
     assert rc == vv.ERR_INVALID
+
+
+CHROMA_CASES = [
+    # w, h, ctb_log2, bd, c_idx, n candidates, tile bounds, wide samples
+    (200, 136, 6, 10, 1, 4, None, False),
+    (256, 128, 7, 8, 2, 8, None, False),
+    (160, 96, 5, 12, 1, 2, ([0, 96, 160], [0, 64, 96]), False),
+    (96, 72, 4, 10, 2, 3, ([0, 48, 96], [0, 32, 72]), False),
+    (72, 40, 3, 8, 1, 2, None, False),
+    (416, 240, 8, 16, 2, 3, None, False),
+    (256, 192, 6, 10, 1, 3, None, True),
+    (1920, 1080, 7, 10, 2, 4, ([0, 640, 1280, 1920], [0, 512, 1080]), False),
+]
+
+
+@pytest.mark.parametrize("cc", [False, True])
+@pytest.mark.parametrize("w,h,ctb_log2,bd,c_idx,n,tbs,wide", CHROMA_CASES)
+def test_chroma_decisions_match_oracle(vv, orc, w, h, ctb_log2, bd, c_idx, n, tbs, wide, cc):
+    say(f"{'cc-alf' if cc else 'chroma alf'} decide {w}x{h} ctb_log2={ctb_log2} bd={bd} c={c_idx} n={n} "
+        f"tiles={tbs is not None} wide={wide}")
+    rng, src, rec, pre = chroma_fixture(vv, w, h, bd, w + 7 * h + c_idx)
+    tb = tile_bounds(vv, *tbs) if tbs else None
+    if cc:
+        aps, rec = cc_fixture(orc, vv, rng, src, rec, pre, c_idx, ctb_log2, bd, min(n, 4), tb)
+    else:
+        aps = lowpass_chroma_aps(vv, rng, n)
+    if wide:  # samples of 2^15 and more in the right half
+        for pl in (rec.cb, rec.cr, pre):
+            r = pl[:, pl.shape[1] // 2:]
+            idx = rng.choice(r.size, r.size // 50, replace=False)
+            r.reshape(-1)[idx] = rng.integers(1 << 15, 1 << 16, idx.size)
+            pl[:, pl.shape[1] // 2:] = r
+    ctb = 1 << ctb_log2
+    nct = ((w + ctb - 1) // ctb) * ((h + ctb - 1) // ctb)
+    init = [(1, 2, 0, 0, 0, 0, 0, 0)] * nct  # luma fields the decision keeps
+    fg, fo = (vv.lib.vvclf_cc_alf_decide, orc.oracle_cc_alf_decide) if cc else \
+        (vv.lib.vvclf_alf_decide_chroma, orc.oracle_alf_decide_chroma)
+    p = pre if cc else None
+    rc, recs, sse, out = decide_chroma(vv, fg, src, rec, aps, c_idx, ctb_log2, bd, tb, p, init)
+    assert rc == 0, vv.lib.vvclf_last_error()
+    rc_o, recs_o, sse_o, out_o = decide_chroma(vv, fo, src, rec, aps, c_idx, ctb_log2, bd, tb, p, init)
+    assert rc_o == 0
+    assert np.array_equal(sse, sse_o)
+    assert recs == recs_o
+    assert np.array_equal(out.cb, out_o.cb) and np.array_equal(out.cr, out_o.cr) and np.array_equal(out.y, rec.y)
+    assert recs != init  # some CTBs take a filter
+    if 2 * ctb <= w:  # and the exact left half stays off
+        assert any(r == init[0] for r in recs)
+
+
+def test_chroma_decisions_reject_on_the_host(vv):
+    say("chroma decisions: refusals")
+    rng, src, rec, pre = chroma_fixture(vv, 64, 64, 8, 3)
+    aps = lowpass_chroma_aps(vv, rng, 2)
+    fn = vv.lib.vvclf_alf_decide_chroma
+    assert decide_chroma(vv, fn, src, rec, aps, 1, 6, 8, cfi=2)[0] == vv.ERR_UNSUPPORTED
+    assert decide_chroma(vv, fn, src, rec, aps, 3, 6, 8)[0] == vv.ERR_INVALID
+    aps.n_chroma_coeff = 9
+    assert decide_chroma(vv, fn, src, rec, aps, 1, 6, 8)[0] == vv.ERR_INVALID
