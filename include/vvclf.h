@@ -441,6 +441,22 @@ int vvclf_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvcl
                         uint32_t bit_depth, uint32_t chroma_format_idc, const vvclf_bounds* tile_bounds,
                         uint64_t* sse);
 
+/* Replaces sao_decide_picture (src/sao_enc.rs:385-462): per CTB (raster
+ * order, out has ceil(W / CtbSizeY) x ceil(H / CtbSizeY) entries) and
+ * component, the SAO parameters the reference's encoder picks from the band
+ * and edge statistics of `rec` (the pre-SAO reconstruction) against `src`
+ * over the CTB (chroma: half the luma CTB clipped to the picture, rounded
+ * down): edge offset (first of the four classes with the largest gain) when
+ * one improves, else band offset (the first of the 4-band windows starting
+ * at 0..27 with the largest gain), else not applied; Cr keeps Cb's type and
+ * edge class (zero offsets when nothing improves).  sao_luma / sao_chroma
+ * as the slice flags; 4:2:0 or 4:0:0 (empty chroma planes).  A
+ * reconstruction sample of 2^BitDepth or more that reaches a band statistic
+ * returns VVCLF_ERR_INVALID (the reference indexes past its 32 bands).
+ * Host pictures; the statistics and the choice run on the GPU. */
+int vvclf_sao_decide_picture(const vvclf_picture* src, const vvclf_picture* rec, uint32_t ctb_log2_size_y,
+                             uint32_t bit_depth, uint32_t sao_luma, uint32_t sao_chroma, vvclf_sao_ctb_params* out);
+
 /* ------------------------------------------------------------------------ */
 /* Per-CTB side information as a compact record: the syntax elements a      */
 /* slice decoder has parsed for one CTB, from which the SAO and ALF CTB      */

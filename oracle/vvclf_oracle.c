@@ -1755,6 +1755,186 @@ int oracle_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvc
   return chroma_decide(src, rec, pre_luma, apply, aps, c_idx, ctb_log2, bd, cfi, tb, sse, 1);
 }
 
+/* ======================================================================== */
+/* SAO encoder decision (src/sao_enc.rs)                                     */
+/* ======================================================================== */
+
+/* eo_category, src/sao_enc.rs:349-383 */
+static int sao_eo_category(int r, int n0, int n1) {
+  int s0 = r > n0 ? 1 : (r < n0 ? -1 : 0), s1 = r > n1 ? 1 : (r < n1 ? -1 : 0);
+  switch (s0 + s1) {
+    case 2: return 4;
+    case 1: return 3;
+    case -1: return 2;
+    case -2: return 1;
+    default: return 0;
+  }
+}
+
+static const int SAO_EO_DIRS[4][4] = {{-1, 0, 1, 0}, {0, -1, 0, 1}, {-1, -1, 1, 1}, {1, -1, -1, 1}};
+
+/* try_band_offset, src/sao_enc.rs:77-170; returns 0 when no band window
+ * improves (the reference's None), -1 for a sample of 2^BitDepth or more
+ * (its band index runs past the 32 bands: the reference panics) */
+static int sao_try_band(const vvclf_plane* src, const vvclf_plane* rec, uint32_t x0, uint32_t y0, uint32_t w,
+                        uint32_t h, uint32_t bd, vvclf_sao_ctb* out) {
+  int64_t cnt[32] = {0}, dif[32] = {0};
+  uint32_t shift = bd - 5;
+  uint32_t xe = x0 + w < rec->width ? x0 + w : rec->width, ye = y0 + h < rec->height ? y0 + h : rec->height;
+  for (uint32_t y = y0; y < ye; y++)
+    for (uint32_t x = x0; x < xe; x++) {
+      if (x >= src->width || y >= src->height) continue;
+      uint32_t r = rec->samples[(size_t)y * rec->stride + x], sv = src->samples[(size_t)y * src->stride + x];
+      uint32_t b = r >> shift;
+      if (b >= 32) return -1;
+      cnt[b] += 1;
+      dif[b] += (int64_t)sv - (int64_t)r;
+    }
+  int64_t best_delta = 0;
+  int best_start = 0;
+  int32_t best_off[4] = {0, 0, 0, 0};
+  int64_t cmax = ((int64_t)1 << ((bd < 10 ? bd : 10) - 5)) - 1;
+  for (int start = 0; start < 28; start++) {
+    int64_t delta = 0;
+    int32_t off[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+      int b = start + i;
+      if (cnt[b] > 0) {
+        int64_t opt = (dif[b] + cnt[b] / 2) / cnt[b];
+        int64_t oc = opt < -cmax ? -cmax : (opt > cmax ? cmax : opt);
+        off[i] = (int32_t)oc;
+        delta -= dif[b] * oc;
+      }
+    }
+    if (delta < best_delta) {
+      best_delta = delta;
+      best_start = start;
+      memcpy(best_off, off, sizeof(off));
+    }
+  }
+  if (best_delta >= 0) return 0;
+  uint32_t ab[4], sg[4];
+  for (int i = 0; i < 4; i++) {
+    ab[i] = (uint32_t)(best_off[i] < 0 ? -best_off[i] : best_off[i]);
+    sg[i] = best_off[i] < 0;
+  }
+  memset(out, 0, sizeof(*out));
+  out->sao_type_idx = VVCLF_SAO_BAND;
+  out->band_position = (uint8_t)(best_start & 31);
+  oracle_sao_derive_offset_val(ab, sg, bd, out->offset_val);
+  return 1;
+}
+
+/* try_edge_offset_class, src/sao_enc.rs:211-284: the class's candidate and
+ * its distortion delta */
+static int64_t sao_try_eo_class(const vvclf_plane* src, const vvclf_plane* rec, uint32_t x0, uint32_t y0, uint32_t w,
+                                uint32_t h, uint32_t bd, int cls, vvclf_sao_ctb* out) {
+  int64_t cnt[5] = {0}, dif[5] = {0};
+  uint32_t xe = x0 + w < rec->width ? x0 + w : rec->width, ye = y0 + h < rec->height ? y0 + h : rec->height;
+  for (uint32_t y = y0; y < ye; y++)
+    for (uint32_t x = x0; x < xe; x++) {
+      int nx0 = (int)x + SAO_EO_DIRS[cls][0], ny0 = (int)y + SAO_EO_DIRS[cls][1];
+      int nx1 = (int)x + SAO_EO_DIRS[cls][2], ny1 = (int)y + SAO_EO_DIRS[cls][3];
+      if (nx0 < 0 || ny0 < 0 || nx0 >= (int)rec->width || ny0 >= (int)rec->height) continue;
+      if (nx1 < 0 || ny1 < 0 || nx1 >= (int)rec->width || ny1 >= (int)rec->height) continue;
+      int r = rec->samples[(size_t)y * rec->stride + x];
+      int c = sao_eo_category(r, rec->samples[(size_t)ny0 * rec->stride + nx0],
+                              rec->samples[(size_t)ny1 * rec->stride + nx1]);
+      if (x >= src->width || y >= src->height) continue;
+      cnt[c] += 1;
+      dif[c] += (int64_t)src->samples[(size_t)y * src->stride + x] - r;
+    }
+  uint32_t off[4] = {0, 0, 0, 0};
+  int64_t delta = 0;
+  for (int i = 0; i < 4; i++) {
+    int c = i + 1;
+    if (cnt[c] > 0) {
+      int64_t a = dif[c] < 0 ? -dif[c] : dif[c];
+      int64_t opt = (a + cnt[c] / 2) / cnt[c];
+      int64_t oc = opt < 0 ? 0 : (opt > 7 ? 7 : opt);
+      off[i] = (uint32_t)oc;
+      delta -= a * oc;
+    }
+  }
+  static const uint32_t eo_sign[4] = {0, 0, 1, 1};
+  memset(out, 0, sizeof(*out));
+  out->sao_type_idx = VVCLF_SAO_EDGE;
+  out->eo_class = (uint8_t)cls;
+  oracle_sao_derive_offset_val(off, eo_sign, bd, out->offset_val);
+  return delta;
+}
+
+/* sao_decide_ctb, src/sao_enc.rs:44-71 (with try_best_edge_offset, :175-207):
+ * edge offset when some class improves, else band offset when some band
+ * window improves, else not applied */
+static int sao_decide_ctb_c(const vvclf_plane* src, const vvclf_plane* rec, uint32_t x0, uint32_t y0, uint32_t w,
+                            uint32_t h, uint32_t bd, vvclf_sao_ctb* out) {
+  vvclf_sao_ctb bo, eo, t;
+  int has_bo = sao_try_band(src, rec, x0, y0, w, h, bd, &bo);
+  if (has_bo < 0) return VVCLF_ERR_INVALID;
+  int64_t best = 0;
+  int has_eo = 0;
+  for (int c = 0; c < 4; c++) {
+    int64_t d = sao_try_eo_class(src, rec, x0, y0, w, h, bd, c, &t);
+    if (d < best) {
+      best = d;
+      eo = t;
+      has_eo = 1;
+    }
+  }
+  memset(out, 0, sizeof(*out));
+  if (has_eo) *out = eo;
+  else if (has_bo) *out = bo;
+  return VVCLF_OK;
+}
+
+/* sao_decide_ctb_constrained, src/sao_enc.rs:286-347: Cr within Cb's type
+ * (and edge class); zero offsets of that type when nothing improves */
+static int sao_decide_ctb_cr(const vvclf_plane* src, const vvclf_plane* rec, uint32_t x0, uint32_t y0, uint32_t w,
+                             uint32_t h, uint32_t bd, const vvclf_sao_ctb* shared, vvclf_sao_ctb* out) {
+  memset(out, 0, sizeof(*out));
+  if (shared->sao_type_idx == VVCLF_SAO_BAND) {
+    int r = sao_try_band(src, rec, x0, y0, w, h, bd, out);
+    if (r < 0) return VVCLF_ERR_INVALID;
+    if (!r) {
+      memset(out, 0, sizeof(*out));
+      out->sao_type_idx = VVCLF_SAO_BAND;
+    }
+  } else if (shared->sao_type_idx == VVCLF_SAO_EDGE) {
+    vvclf_sao_ctb t;
+    int64_t d = sao_try_eo_class(src, rec, x0, y0, w, h, bd, shared->eo_class, &t);
+    if (d < 0) {
+      *out = t;
+    } else {
+      out->sao_type_idx = VVCLF_SAO_EDGE;
+      out->eo_class = shared->eo_class;
+    }
+  }
+  return VVCLF_OK;
+}
+
+/* sao_decide_picture, src/sao_enc.rs:385-462 (4:2:0 chroma at half the luma
+ * CTB, rounded down) */
+int oracle_sao_decide_picture(const vvclf_picture* src, const vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd,
+                              uint32_t sao_luma, uint32_t sao_chroma, vvclf_sao_ctb_params* out) {
+  uint32_t ctb = 1u << ctb_log2, W = rec->luma.width, H = rec->luma.height;
+  uint32_t wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb;
+  for (uint32_t ry = 0; ry < hc; ry++)
+    for (uint32_t rx = 0; rx < wc; rx++) {
+      vvclf_sao_ctb_params* p = &out[(size_t)ry * wc + rx];
+      memset(p, 0, sizeof(*p));
+      uint32_t x0 = rx * ctb, y0 = ry * ctb;
+      uint32_t w = ctb < W - x0 ? ctb : W - x0, h = ctb < H - y0 ? ctb : H - y0;
+      if (sao_luma && sao_decide_ctb_c(&src->luma, &rec->luma, x0, y0, w, h, bd, &p->luma)) return VVCLF_ERR_INVALID;
+      uint32_t cx = x0 / 2, cy = y0 / 2, cw = w / 2, ch = h / 2;
+      if (sao_chroma && cw > 0 && ch > 0) {
+        if (sao_decide_ctb_c(&src->cb, &rec->cb, cx, cy, cw, ch, bd, &p->cb)) return VVCLF_ERR_INVALID;
+        if (sao_decide_ctb_cr(&src->cr, &rec->cr, cx, cy, cw, ch, bd, &p->cb, &p->cr)) return VVCLF_ERR_INVALID;
+      }
+    }
+  return VVCLF_OK;
+}
+
 int oracle_apply_in_loop_filters(vvclf_frame* f) {
   static const vvclf_deblock_cu empty_cus[1];
   const vvclf_deblock_cu* ccus = f->has_chroma_cus ? (f->chroma_cus ? f->chroma_cus : empty_cus) : NULL;

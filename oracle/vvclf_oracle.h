@@ -99,6 +99,10 @@ int oracle_alf_decide_chroma(const vvclf_picture* src, vvclf_picture* rec, vvclf
 int oracle_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plane* pre_luma,
                          vvclf_alf_ctb* apply, const vvclf_alf_aps* aps, uint32_t c_idx, uint32_t ctb_log2,
                          uint32_t bd, uint32_t cfi, const vvclf_bounds* tb, uint64_t* sse);
+/* src/sao_enc.rs:385-462: per-CTB SAO parameters of a picture (raster CTB
+ * order), from per-CTB band and edge statistics of rec against src */
+int oracle_sao_decide_picture(const vvclf_picture* src, const vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd,
+                              uint32_t sao_luma, uint32_t sao_chroma, vvclf_sao_ctb_params* out);
 
 /* src/ctu.rs:11822-12066 */
 int oracle_apply_in_loop_filters(vvclf_frame* frame);

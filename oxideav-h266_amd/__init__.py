@@ -250,6 +250,7 @@ SIGNATURES = {
                                            P(Bounds), P(u64)]),
     "vvclf_cc_alf_decide": (C.c_int, [P(Picture), P(Picture), P(Plane), P(AlfCtb), P(AlfAps), u32, u32, u32, u32,
                                        P(Bounds), P(u64)]),
+    "vvclf_sao_decide_picture": (C.c_int, [P(Picture), P(Picture), u32, u32, u32, u32, P(SaoCtbParams)]),
     "vvclf_ctb_syntax_decode": (C.c_int, [P(CtbSyntax), u32, u32, P(CtbSyntaxConfig), P(SaoCtbParams), P(AlfCtb)]),
     "vvclf_ctb_syntax_encode": (C.c_int, [P(SaoCtbParams), P(AlfCtb), u32, u32, P(CtbSyntaxConfig), P(CtbSyntax)]),
     "vvclf_server_submit_syntax": (C.c_int, [VP, u32, P(Frame), P(CtbSyntax), P(CtbSyntaxConfig), P(u64)]),

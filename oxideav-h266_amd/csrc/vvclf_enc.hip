@@ -345,6 +345,220 @@ __global__ void __launch_bounds__(256) k_chroma_enc(cenc_args a, int tile_log2) 
   }
 }
 
+// ---------------------------------------------------------------------------
+// SAO decision (sao_decide_picture, src/sao_enc.rs:385-462): per CTB and
+// component, band statistics (count and sum of src - rec per band,
+// band = rec >> (BitDepth - 5)) and edge statistics (per edge class and
+// category 1..4, neighbours taken from rec inside the plane) over the CTB's
+// region, then the reference's per-CTB choice from them.
+// k_sao_stats: a workgroup per T x T tile of one component (inside one CTB),
+// counters in LDS, one 64-bit atomic per counter and tile into the CTB's row.
+// ---------------------------------------------------------------------------
+constexpr int SAO_STAT_WORDS = 100;  // 32 band counts, 32 band sums, 16 edge counts, 16 edge sums, overflow
+enum { SB_CNT = 0, SB_DIF = 32, SE_CNT = 64, SE_DIF = 80, S_OVER = 96 };
+
+struct sao_args {
+  const uint16_t* rec[3];
+  const uint16_t* src[3];
+  int pitch[3], pw[3], ph[3];  // plane pitch and size
+  int w, h, ctb_log2, wctb, bd, luma, chroma;
+  long long* stats;  // [n_ctbs][3][SAO_STAT_WORDS]
+};
+
+__device__ __forceinline__ int eo_cat(int r, int n0, int n1) {
+  const int e = (r > n0) - (r < n0) + (r > n1) - (r < n1);
+  return e == 2 ? 4 : (e == 1 ? 3 : (e == -1 ? 2 : (e == -2 ? 1 : 0)));
+}
+
+__global__ void __launch_bounds__(256) k_sao_stats(sao_args a, int tile_log2_luma) {
+  __shared__ int cnt[SAO_STAT_WORDS];
+  const int c = blockIdx.z;
+  if (c == 0 ? !a.luma : !a.chroma) return;
+  for (int i = threadIdx.x; i < SAO_STAT_WORDS; i += 256) cnt[i] = 0;
+  __syncthreads();
+  const int tl = c ? tile_log2_luma - 1 : tile_log2_luma, T = 1 << tl;
+  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+  const int pw = a.pw[c], ph = a.ph[c];
+  if (x0 < pw && y0 < ph) {  // (workgroup-uniform)
+    const int ctb = 1 << a.ctb_log2, sh = c ? 1 : 0;
+    const int rx = (x0 << sh) >> a.ctb_log2, ry = (y0 << sh) >> a.ctb_log2;
+    // the CTB's region: luma CTB clipped to the picture; chroma half of it,
+    // rounded down (src/sao_enc.rs:404-432)
+    const int lx = rx * ctb, ly = ry * ctb;
+    const int rw = min(ctb, a.w - lx) >> sh, rh = min(ctb, a.h - ly) >> sh;
+    const int ex = min((lx >> sh) + rw, pw), ey = min((ly >> sh) + rh, ph);
+    const uint16_t* R = a.rec[c];
+    const uint16_t* S = a.src[c];
+    const int P = a.pitch[c], shift = a.bd - 5;
+    for (int i = threadIdx.x; i < T * T; i += 256) {
+      const int x = x0 + (i & (T - 1)), y = y0 + (i >> tl);
+      if (x >= ex || y >= ey) continue;
+      const int r = R[(size_t)y * P + x], d = (int)S[(size_t)y * P + x] - r;
+      const int b = r >> shift;
+      if (b >= 32) {
+        atomicOr(&cnt[S_OVER], 1);
+      } else {
+        atomicAdd(&cnt[SB_CNT + b], 1);
+        atomicAdd(&cnt[SB_DIF + b], d);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int dx = k == 1 ? 0 : (k == 3 ? 1 : -1), dy = k == 0 ? 0 : -1;  // n0 = (x+dx, y+dy), n1 = mirror
+        const int x_0 = x + dx, y_0 = y + dy, x_1 = x - dx, y_1 = y - dy;
+        if (x_0 < 0 || y_0 < 0 || x_0 >= pw || y_0 >= ph || x_1 < 0 || y_1 < 0 || x_1 >= pw || y_1 >= ph) continue;
+        const int cat = eo_cat(r, R[(size_t)y_0 * P + x_0], R[(size_t)y_1 * P + x_1]);
+        if (cat) {
+          atomicAdd(&cnt[SE_CNT + k * 4 + cat - 1], 1);
+          atomicAdd(&cnt[SE_DIF + k * 4 + cat - 1], d);
+        }
+      }
+    }
+    __syncthreads();
+    long long* row = a.stats + (((size_t)ry * a.wctb + rx) * 3 + c) * SAO_STAT_WORDS;
+    for (int i = threadIdx.x; i < SAO_STAT_WORDS; i += 256)
+      if (cnt[i]) atomicAdd(reinterpret_cast<unsigned long long*>(row + i), (unsigned long long)(long long)cnt[i]);
+  }
+}
+
+__device__ void sao_offsets(const int32_t (&absv)[4], const int (&sign)[4], int bd, int32_t (&ov)[5]) {
+  const int sh = bd - min(bd, 10);
+  ov[0] = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) ov[i + 1] = (1 - 2 * (sign[i] & 1)) * (absv[i] << sh);
+}
+
+// try_band_offset (src/sao_enc.rs:77-170) from the statistics; false when no
+// band window improves
+__device__ bool sao_band_pick(const long long* st, int bd, vvclf_sao_ctb& o) {
+  long long best_delta = 0;
+  int best_start = 0;
+  int32_t best_off[4] = {0, 0, 0, 0};
+  const long long cmax = (1ll << (min(bd, 10) - 5)) - 1;
+  for (int start = 0; start < 28; start++) {
+    long long delta = 0;
+    int32_t off[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+      const long long n = st[SB_CNT + start + i], dsum = st[SB_DIF + start + i];
+      if (n > 0) {
+        long long opt = (dsum + n / 2) / n;
+        opt = opt < -cmax ? -cmax : (opt > cmax ? cmax : opt);
+        off[i] = (int32_t)opt;
+        delta -= dsum * opt;
+      }
+    }
+    if (delta < best_delta) {
+      best_delta = delta;
+      best_start = start;
+      for (int i = 0; i < 4; i++) best_off[i] = off[i];
+    }
+  }
+  if (best_delta >= 0) return false;
+  int32_t ab[4];
+  int sg[4];
+  for (int i = 0; i < 4; i++) {
+    ab[i] = best_off[i] < 0 ? -best_off[i] : best_off[i];
+    sg[i] = best_off[i] < 0;
+  }
+  o.sao_type_idx = VVCLF_SAO_BAND;
+  o.eo_class = 0;
+  o.band_position = (uint8_t)(best_start & 31);
+  o.pad = 0;
+  sao_offsets(ab, sg, bd, o.offset_val);
+  return true;
+}
+
+// try_edge_offset_class (src/sao_enc.rs:211-284): the class's candidate and
+// its delta
+__device__ long long sao_eo_pick(const long long* st, int cls, int bd, vvclf_sao_ctb& o) {
+  int32_t off[4] = {0, 0, 0, 0};
+  long long delta = 0;
+  for (int i = 0; i < 4; i++) {
+    const long long n = st[SE_CNT + cls * 4 + i], dsum = st[SE_DIF + cls * 4 + i];
+    if (n > 0) {
+      const long long av = dsum < 0 ? -dsum : dsum;
+      long long opt = (av + n / 2) / n;
+      opt = opt < 0 ? 0 : (opt > 7 ? 7 : opt);
+      off[i] = (int32_t)opt;
+      delta -= av * opt;
+    }
+  }
+  const int sg[4] = {0, 0, 1, 1};
+  o.sao_type_idx = VVCLF_SAO_EDGE;
+  o.eo_class = (uint8_t)cls;
+  o.band_position = 0;
+  o.pad = 0;
+  sao_offsets(off, sg, bd, o.offset_val);
+  return delta;
+}
+
+__device__ void sao_clear(vvclf_sao_ctb& o) {
+  o.sao_type_idx = 0;
+  o.eo_class = 0;
+  o.band_position = 0;
+  o.pad = 0;
+  for (int i = 0; i < 5; i++) o.offset_val[i] = 0;
+}
+
+// sao_decide_ctb (src/sao_enc.rs:44-71): edge offset when a class improves,
+// else band offset when a window improves, else not applied
+__device__ void sao_decide_one(const long long* st, int bd, vvclf_sao_ctb& o) {
+  vvclf_sao_ctb bo, t;
+  const bool has_bo = sao_band_pick(st, bd, bo);
+  long long best = 0;
+  int best_cls = -1;
+  for (int k = 0; k < 4; k++) {
+    const long long d = sao_eo_pick(st, k, bd, t);
+    if (d < best) {
+      best = d;
+      best_cls = k;
+    }
+  }
+  if (best_cls >= 0) sao_eo_pick(st, best_cls, bd, o);
+  else if (has_bo) o = bo;
+  else sao_clear(o);
+}
+
+// one thread per CTB; *bad is set when a sample of 2^BitDepth or more would
+// reach a band statistic the decision reads (the reference panics there)
+__global__ void __launch_bounds__(256) k_sao_decide(sao_args a, int n_ctbs, vvclf_sao_ctb_params* out,
+                                                    unsigned* bad) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_ctbs) return;
+  const long long* st = a.stats + (size_t)i * 3 * SAO_STAT_WORDS;
+  vvclf_sao_ctb_params p;
+  sao_clear(p.luma);
+  sao_clear(p.cb);
+  sao_clear(p.cr);
+  if (a.luma) {
+    if (st[S_OVER]) atomicOr(bad, 1u);
+    sao_decide_one(st, a.bd, p.luma);
+  }
+  const int ctb = 1 << a.ctb_log2, rx = i % a.wctb, ry = i / a.wctb;
+  const int cw = min(ctb, a.w - rx * ctb) / 2, ch = min(ctb, a.h - ry * ctb) / 2;
+  if (a.chroma && cw > 0 && ch > 0) {
+    const long long *sb = st + SAO_STAT_WORDS, *sr = st + 2 * SAO_STAT_WORDS;
+    if (sb[S_OVER]) atomicOr(bad, 1u);
+    sao_decide_one(sb, a.bd, p.cb);
+    // sao_decide_ctb_constrained (src/sao_enc.rs:286-347)
+    if (p.cb.sao_type_idx == VVCLF_SAO_BAND) {
+      if (sr[S_OVER]) atomicOr(bad, 1u);
+      if (!sao_band_pick(sr, a.bd, p.cr)) {
+        sao_clear(p.cr);
+        p.cr.sao_type_idx = VVCLF_SAO_BAND;
+      }
+    } else if (p.cb.sao_type_idx == VVCLF_SAO_EDGE) {
+      vvclf_sao_ctb t;
+      if (sao_eo_pick(sr, p.cb.eo_class, a.bd, t) < 0) {
+        p.cr = t;
+      } else {
+        p.cr.sao_type_idx = VVCLF_SAO_EDGE;
+        p.cr.eo_class = p.cb.eo_class;
+      }
+    }
+  }
+  out[i] = p;
+}
+
 }  // namespace
 
 extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd,
@@ -501,8 +715,9 @@ int chroma_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plan
   if (ctb_log2 < 3 || ctb_log2 > 8) return fail(VVCLF_ERR_UNSUPPORTED, "%s: ctb_log2_size_y %u", name, ctb_log2);
   const vvclf_plane &SP = c_idx == 1 ? src->cb : src->cr, &RP = c_idx == 1 ? rec->cb : rec->cr;
   const uint32_t W = rec->luma.width, H = rec->luma.height, Wc = RP.width, Hc = RP.height;
-  if (!SP.samples || !RP.samples || SP.width != Wc || SP.height != Hc || Wc != (W + 1) / 2 || Hc != (H + 1) / 2 ||
-      Wc == 0 || Hc == 0)
+  // 4:2:0 chroma planes of half the luma size, rounded either way
+  if (!SP.samples || !RP.samples || SP.width != Wc || SP.height != Hc || (Wc != W / 2 && Wc != (W + 1) / 2) ||
+      (Hc != H / 2 && Hc != (H + 1) / 2) || Wc == 0 || Hc == 0)
     return fail(VVCLF_ERR_INVALID, "%s: chroma planes do not match a 4:2:0 %ux%u picture", name, W, H);
   if (W > 65535 || H > 65535) return fail(VVCLF_ERR_UNSUPPORTED, "%s: picture above 65535 samples", name);
   if (cc && (!pre_luma->samples || pre_luma->width != W || pre_luma->height != H || pre_luma->stride < W))
@@ -643,5 +858,89 @@ extern "C" int vvclf_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec,
                                    uint32_t bd, uint32_t cfi, const vvclf_bounds* tb, uint64_t* sse) {
   clear_error();
   return chroma_decide(src, rec, pre_luma, apply, aps, c_idx, ctb_log2, bd, cfi, tb, sse, true);
+}
+
+extern "C" int vvclf_sao_decide_picture(const vvclf_picture* src, const vvclf_picture* rec, uint32_t ctb_log2,
+                                        uint32_t bd, uint32_t sao_luma, uint32_t sao_chroma,
+                                        vvclf_sao_ctb_params* out) {
+  clear_error();
+  if (!src || !rec || !out) return fail(VVCLF_ERR_INVALID, "sao_decide_picture: NULL argument");
+  if (bd < 8 || bd > 16) return fail(VVCLF_ERR_INVALID, "sao_decide_picture: bit depth %u", bd);
+  if (ctb_log2 < 3 || ctb_log2 > 8)
+    return fail(VVCLF_ERR_UNSUPPORTED, "sao_decide_picture: ctb_log2_size_y %u", ctb_log2);
+  const uint32_t W = rec->luma.width, H = rec->luma.height;
+  if (W == 0 || H == 0 || W > 65535 || H > 65535)
+    return fail(VVCLF_ERR_UNSUPPORTED, "sao_decide_picture: picture %ux%u", W, H);
+  const vvclf_plane* rp[3] = {&rec->luma, &rec->cb, &rec->cr};
+  const vvclf_plane* sp[3] = {&src->luma, &src->cb, &src->cr};
+  const bool use[3] = {sao_luma != 0, sao_chroma != 0, sao_chroma != 0};
+  for (int c = 0; c < 3; c++) {
+    if (!use[c]) continue;
+    if (sp[c]->width != rp[c]->width || sp[c]->height != rp[c]->height ||
+        (rp[c]->width && (!rp[c]->samples || !sp[c]->samples)))
+      return fail(VVCLF_ERR_INVALID, "sao_decide_picture: plane %d of source and reconstruction differ", c);
+    if (c && rp[c]->width && ((rp[c]->width != W / 2 && rp[c]->width != (W + 1) / 2) ||
+                              (rp[c]->height != H / 2 && rp[c]->height != (H + 1) / 2)))
+      return fail(VVCLF_ERR_UNSUPPORTED, "sao_decide_picture: chroma planes other than 4:2:0");
+  }
+  const uint32_t ctb = 1u << ctb_log2, wctb = (W + ctb - 1) / ctb, hctb = (H + ctb - 1) / ctb;
+  const uint32_t n_ctbs = wctb * hctb;
+  vvclf_ctx* ctx = default_ctx();
+  if (!ctx) return VVCLF_ERR_HIP;
+  hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  sao_args a;
+  memset(&a, 0, sizeof(a));
+  dbuf dr[3], ds[3], dst, dout, dbad;
+  hipError_t e = hipSuccess;
+  for (int c = 0; c < 3 && e == hipSuccess; c++) {
+    const uint32_t pw = use[c] ? rp[c]->width : 0, ph = use[c] ? rp[c]->height : 0;
+    const uint32_t pitch = (pw + 63) / 64 * 64;
+    a.pitch[c] = (int)pitch;
+    a.pw[c] = (int)pw;
+    a.ph[c] = (int)ph;
+    e = dr[c].alloc((size_t)pitch * ph * 2, st);
+    if (e == hipSuccess) e = ds[c].alloc((size_t)pitch * ph * 2, st);
+    if (e == hipSuccess && pw && ph)
+      e = hipMemcpy2DAsync(dr[c].p, (size_t)pitch * 2, rp[c]->samples, (size_t)rp[c]->stride * 2, (size_t)pw * 2, ph,
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && pw && ph)
+      e = hipMemcpy2DAsync(ds[c].p, (size_t)pitch * 2, sp[c]->samples, (size_t)sp[c]->stride * 2, (size_t)pw * 2, ph,
+                           hipMemcpyHostToDevice, st);
+    a.rec[c] = (const uint16_t*)dr[c].p;
+    a.src[c] = (const uint16_t*)ds[c].p;
+  }
+  const size_t stat_bytes = (size_t)n_ctbs * 3 * SAO_STAT_WORDS * 8;
+  if (e == hipSuccess) e = dst.alloc(stat_bytes, st);
+  if (e == hipSuccess) e = dout.alloc((size_t)n_ctbs * sizeof(vvclf_sao_ctb_params), st);
+  if (e == hipSuccess) e = dbad.alloc(4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(dst.p, 0, stat_bytes, st);
+  if (e == hipSuccess) e = hipMemsetAsync(dbad.p, 0, 4, st);
+  if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "sao_decide_picture: device buffers: %s", hipGetErrorString(e));
+  a.w = (int)W;
+  a.h = (int)H;
+  a.ctb_log2 = (int)ctb_log2;
+  a.wctb = (int)wctb;
+  a.bd = (int)bd;
+  a.luma = use[0];
+  a.chroma = use[1] && rp[1]->width > 0;
+  a.stats = (long long*)dst.p;
+  // luma tiles of min(CTB, 32), chroma tiles of half that (inside one CTB)
+  const int tl = std::min<int>((int)ctb_log2, 5), T = 1 << tl;
+  const dim3 grid((W + T - 1) / T, (H + T - 1) / T, 3);
+  hipLaunchKernelGGL(k_sao_stats, grid, dim3(256), 0, st, a, tl);
+  hipLaunchKernelGGL(k_sao_decide, dim3((n_ctbs + 255) / 256), dim3(256), 0, st, a, (int)n_ctbs,
+                     (vvclf_sao_ctb_params*)dout.p, (unsigned*)dbad.p);
+  e = hipGetLastError();
+  unsigned bad = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, dout.p, (size_t)n_ctbs * sizeof(vvclf_sao_ctb_params), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, dbad.p, 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "sao_decide_picture");
+  if (bad)
+    return fail(VVCLF_ERR_INVALID, "sao_decide_picture: a reconstruction sample of 2^%u or more in a band statistic",
+                bd);
+  return VVCLF_OK;
 }
 

@@ -16,6 +16,7 @@ import tempfile
 import pytest
 
 import binding
+import sao_decide_cases
 import test_oracle_pins as pins
 
 ORACLE_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
@@ -93,6 +94,21 @@ MUTANTS = [
      ["pin_tile_clip_equals_separate_pictures", "pin_sao_tile_edges"]),
     ("alf-tile-chroma-x1", "cc_[2] = (cy_[2] + (int32_t)sw - 1) / (int32_t)sw;",
      "cc_[2] = (cy_[2] + (int32_t)sw + 1) / (int32_t)sw;", 1, ["pin_tile_clip_equals_separate_pictures"]),
+    # SAO encoder decision (src/sao_enc.rs), caught by tests/sao_decide_cases.py
+    ("sao-band-window", "for (int start = 0; start < 28; start++) {", "for (int start = 0; start < 29; start++) {", 0,
+     ["sao:case_band_31_is_never_in_a_window"]),
+    ("sao-round-floor", "int64_t opt = (dif[b] + cnt[b] / 2) / cnt[b];",
+     "int64_t opt = (dif[b] + cnt[b] / 2) / cnt[b] - (((dif[b] + cnt[b] / 2) % cnt[b]) < 0);", 0,
+     ["sao:case_offset_rounds_toward_zero", "sao:case_band_offset_closed_form"]),
+    ("sao-eo-tie", "if (d < best) {\n      best = d;\n      eo = t;", "if (d <= best) {\n      best = d;\n      eo = t;", 0,
+     ["sao:case_band_offset_closed_form"]),
+    ("sao-eo-over-bo", "if (has_eo) *out = eo;\n  else if (has_bo) *out = bo;",
+     "if (has_bo) *out = bo;\n  else if (has_eo) *out = eo;", 0, ["sao:case_edge_offset_class_and_counts"]),
+    ("sao-cr-free", "sao_decide_ctb_cr(&src->cr, &rec->cr, cx, cy, cw, ch, bd, &p->cb, &p->cr)",
+     "sao_decide_ctb_c(&src->cr, &rec->cr, cx, cy, cw, ch, bd, &p->cr)", 0, ["sao:case_cr_follows_cb"]),
+    ("sao-chroma-ceil", "uint32_t cx = x0 / 2, cy = y0 / 2, cw = w / 2, ch = h / 2;",
+     "uint32_t cx = x0 / 2, cy = y0 / 2, cw = (w + 1) / 2, ch = (h + 1) / 2;", 0,
+     ["sao:case_chroma_region_rounds_down"]),
 ]
 
 
@@ -141,7 +157,10 @@ def test_pins_catch_mutant(mutant, mutant_libs, vv):
     failures = []
     for pin in catchers:
         try:
-            pins.ALL_PINS[pin](lib, vv)
+            if pin.startswith("sao:"):
+                sao_decide_cases.ALL_CASES[pin[4:]](vv, lib.oracle_sao_decide_picture)
+            else:
+                pins.ALL_PINS[pin](lib, vv)
         except AssertionError as e:
             failures.append((pin, str(e)[:80]))
     assert failures, f"no pin caught mutant {name}"
