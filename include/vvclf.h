@@ -570,6 +570,18 @@ int vvclf_job_run(vvclf_job* job, void* stream);
 /* The same per-picture kernel sequence, captured into a HIP graph on the
  * first call and replayed from then on (fewer launch gaps). */
 int vvclf_job_graph_launch(vvclf_job* job, void* stream);
+/* Many pictures' plain kernel sequences in one HIP graph of `lanes` parallel
+ * branches (job i on branch i % lanes, in order within a branch), so that
+ * independent pictures overlap on the device: one picture's latency-bound
+ * stages run beside another's VALU-bound ALF.  The jobs must outlive the
+ * graph.  Launches are asynchronous on `stream`; each job's result is what
+ * vvclf_job_run gives.  No counterpart in the reference (its driver filters
+ * one picture at a time). */
+typedef struct vvclf_jobs_graph vvclf_jobs_graph;
+int vvclf_jobs_graph_create(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n_jobs, uint32_t lanes,
+                            vvclf_jobs_graph** out);
+int vvclf_jobs_graph_launch(vvclf_jobs_graph* g, void* stream);
+int vvclf_jobs_graph_destroy(vvclf_jobs_graph* g);
 /* Stage timing for the plain form: a pool of HIP events for up to
  * `max_runs` runs.  vvclf_job_run_prof records events between the stage
  * kernels on the run's stream (no synchronisation); vvclf_prof_stage_ms waits

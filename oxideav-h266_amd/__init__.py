@@ -251,6 +251,9 @@ SIGNATURES = {
     "vvclf_cc_alf_decide": (C.c_int, [P(Picture), P(Picture), P(Plane), P(AlfCtb), P(AlfAps), u32, u32, u32, u32,
                                        P(Bounds), P(u64)]),
     "vvclf_sao_decide_picture": (C.c_int, [P(Picture), P(Picture), u32, u32, u32, u32, P(SaoCtbParams)]),
+    "vvclf_jobs_graph_create": (C.c_int, [VP, P(VP), u32, u32, P(VP)]),
+    "vvclf_jobs_graph_launch": (C.c_int, [VP, VP]),
+    "vvclf_jobs_graph_destroy": (C.c_int, [VP]),
     "vvclf_ctb_syntax_decode": (C.c_int, [P(CtbSyntax), u32, u32, P(CtbSyntaxConfig), P(SaoCtbParams), P(AlfCtb)]),
     "vvclf_ctb_syntax_encode": (C.c_int, [P(SaoCtbParams), P(AlfCtb), u32, u32, P(CtbSyntaxConfig), P(CtbSyntax)]),
     "vvclf_server_submit_syntax": (C.c_int, [VP, u32, P(Frame), P(CtbSyntax), P(CtbSyntaxConfig), P(u64)]),
@@ -474,4 +477,23 @@ class Job:
     def close(self):
         if self.h:
             lib.vvclf_job_destroy(self.h)
+            self.h = None
+
+
+class JobsGraph:
+    """Many jobs' plain sequences in one HIP graph of `lanes` parallel
+    branches (vvclf_jobs_graph_*); the jobs must outlive it."""
+
+    def __init__(self, ctx, jobs, lanes=2):
+        self.h = VP()
+        self._jobs = list(jobs)
+        arr = (VP * max(1, len(jobs)))(*[j.h for j in jobs])
+        check(lib.vvclf_jobs_graph_create(ctx.h, arr, len(jobs), lanes, C.byref(self.h)))
+
+    def launch(self, stream=None):
+        check(lib.vvclf_jobs_graph_launch(self.h, stream))
+
+    def close(self):
+        if self.h:
+            lib.vvclf_jobs_graph_destroy(self.h)
             self.h = None

@@ -674,6 +674,79 @@ int vvclf_job_graph_launch(vvclf_job* j, void* stream) {
   return VVCLF_OK;
 }
 
+struct vvclf_jobs_graph {
+  vvclf_ctx* ctx;
+  std::vector<vvclf_job*> jobs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int vvclf_jobs_graph_create(vvclf_ctx* ctx, vvclf_job* const* jobs, uint32_t n, uint32_t lanes,
+                            vvclf_jobs_graph** out) {
+  clear_error();
+  if (!ctx || !out || (n && !jobs)) return fail(VVCLF_ERR_INVALID, "jobs_graph_create: NULL argument");
+  if (lanes < 1 || lanes > 16) return fail(VVCLF_ERR_INVALID, "jobs_graph_create: %u lanes (1..16)", lanes);
+  for (uint32_t i = 0; i < n; i++) {
+    if (!jobs[i]) return fail(VVCLF_ERR_INVALID, "jobs_graph_create: NULL job %u", i);
+    if (jobs[i]->ctx->device != ctx->device) return fail(VVCLF_ERR_INVALID, "jobs_graph_create: job %u on another device", i);
+  }
+  hipSetDevice(ctx->device);
+  lanes = std::max(1u, std::min(lanes, std::max(n, 1u)));
+  std::vector<hipStream_t> cs(lanes, nullptr);
+  std::vector<hipEvent_t> ev(lanes, nullptr);
+  hipError_t e = hipSuccess;
+  for (uint32_t k = 0; k < lanes && e == hipSuccess; k++) {
+    e = hipStreamCreateWithFlags(&cs[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
+  }
+  auto g = new vvclf_jobs_graph{ctx, std::vector<vvclf_job*>(jobs, jobs + n)};
+  if (e == hipSuccess) {
+    // capture on lane 0; the other lanes join through an event and are
+    // joined back before the capture ends
+    e = hipStreamBeginCapture(cs[0], hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipError_t le = hipEventRecord(ev[0], cs[0]);
+      for (uint32_t k = 1; k < lanes && le == hipSuccess; k++) le = hipStreamWaitEvent(cs[k], ev[0], 0);
+      for (uint32_t i = 0; i < n && le == hipSuccess; i++) le = launch_plain(jobs[i]->d, cs[i % lanes]);
+      for (uint32_t k = 1; k < lanes && le == hipSuccess; k++) {
+        le = hipEventRecord(ev[k], cs[k]);
+        if (le == hipSuccess) le = hipStreamWaitEvent(cs[0], ev[k], 0);
+      }
+      e = hipStreamEndCapture(cs[0], &g->graph);
+      if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  }
+  for (uint32_t k = 0; k < lanes; k++) {
+    if (ev[k]) hipEventDestroy(ev[k]);
+    if (cs[k]) hipStreamDestroy(cs[k]);
+  }
+  if (e != hipSuccess) {
+    vvclf_jobs_graph_destroy(g);
+    return hip_fail(e, "jobs graph capture");
+  }
+  *out = g;
+  return VVCLF_OK;
+}
+
+int vvclf_jobs_graph_launch(vvclf_jobs_graph* g, void* stream) {
+  if (!g) return fail(VVCLF_ERR_INVALID, "jobs_graph_launch: NULL graph");
+  hipSetDevice(g->ctx->device);
+  hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  for (vvclf_job* j : g->jobs) j->fused_out = false;
+  return VVCLF_OK;
+}
+
+int vvclf_jobs_graph_destroy(vvclf_jobs_graph* g) {
+  if (!g) return VVCLF_OK;
+  hipSetDevice(g->ctx->device);
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
+  return VVCLF_OK;
+}
+
 int vvclf_job_download(vvclf_job* j, vvclf_picture* out) {
   if (!j || !out) return fail(VVCLF_ERR_INVALID, "job_download: NULL argument");
   hipSetDevice(j->ctx->device);

@@ -171,3 +171,35 @@ def test_autotune_picks_a_valid_shape(vv, orc):
     for j in jobs:
         j.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 3])
+def test_jobs_graph_lanes_match_oracle(vv, orc, lanes):
+    """The plain sequences of many jobs in one graph of parallel lanes: every
+    picture equals the oracle on each of three replays, with eager plain runs
+    of one job between replays."""
+    say(f"jobs graph lanes={lanes}")
+    gpu, ref = make(vv, orc, MIX)
+    ctx = vv.Context()
+    jobs = [vv.Job(ctx, f.desc) for f in gpu]
+    g = vv.JobsGraph(ctx, jobs, lanes)
+    for rep in range(3):
+        g.launch(ctx.stream)
+        ctx.synchronize()
+        check_all(vv, jobs, gpu, ref, f"jobs graph lanes {lanes} replay {rep}")
+        jobs[rep].run(ctx.stream)
+        ctx.synchronize()
+    g.close()
+    for j in jobs:
+        j.close()
+    ctx.close()
+
+
+def test_jobs_graph_rejects_bad_arguments(vv):
+    say("jobs graph: bad arguments")
+    ctx = vv.Context()
+    h = C.c_void_p()
+    assert vv.lib.vvclf_jobs_graph_create(ctx.h, None, 1, 2, C.byref(h)) == vv.ERR_INVALID
+    assert vv.lib.vvclf_jobs_graph_create(ctx.h, (C.c_void_p * 1)(None), 1, 2, C.byref(h)) == vv.ERR_INVALID
+    assert vv.lib.vvclf_jobs_graph_create(ctx.h, (C.c_void_p * 1)(None), 0, 0, C.byref(h)) == vv.ERR_INVALID
+    ctx.close()
