@@ -457,6 +457,20 @@ int vvclf_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvcl
 int vvclf_sao_decide_picture(const vvclf_picture* src, const vvclf_picture* rec, uint32_t ctb_log2_size_y,
                              uint32_t bit_depth, uint32_t sao_luma, uint32_t sao_chroma, vvclf_sao_ctb_params* out);
 
+/* Replaces apply_chroma_sao_merge (src/sao_enc.rs:525-660): per CTB in
+ * raster order, keep the independent chroma parameters (`independent`, as
+ * vvclf_sao_decide_picture gives them) or take those already decided for the
+ * left or the above neighbour, by the smallest Cb + Cr SSE (SAO of the
+ * candidate on `rec_pre_sao`'s chroma CTB against `src`) + 8 * bits (the
+ * reference's bit estimates: 1 + per component 1 / 16 / 10 for not applied /
+ * band / edge independently, 1 for merge left, 2 for merge above; ties keep
+ * the earlier option).  merged: luma as given, chroma as decided;
+ * merge_map[i]: 0 independent, 1 merge left, 2 merge above.  On the GPU the
+ * CTBs are decided in anti-diagonal waves. */
+int vvclf_sao_chroma_merge(const vvclf_picture* src, const vvclf_picture* rec_pre_sao,
+                           const vvclf_sao_ctb_params* independent, uint32_t bit_depth, uint32_t ctb_log2_size_y,
+                           vvclf_sao_ctb_params* merged, uint8_t* merge_map);
+
 /* ------------------------------------------------------------------------ */
 /* Per-CTB side information as a compact record: the syntax elements a      */
 /* slice decoder has parsed for one CTB, from which the SAO and ALF CTB      */

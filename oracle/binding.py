@@ -79,6 +79,8 @@ def load(vv=None, path=None):
                                            u32, u32, u32, u32, P(vv.Bounds), P(C.c_uint64)]),
         "oracle_sao_decide_picture": (C.c_int, [P(vv.Picture), P(vv.Picture), u32, u32, u32, u32,
                                                 P(vv.SaoCtbParams)]),
+        "oracle_sao_chroma_merge": (C.c_int, [P(vv.Picture), P(vv.Picture), P(vv.SaoCtbParams), u32, u32,
+                                              P(vv.SaoCtbParams), P(C.c_uint8)]),
         "oracle_apply_in_loop_filters_mt": (C.c_int, [P(P(vv.Frame)), u32, u32]),
     }
     for name, (res, args) in sig.items():

@@ -1935,6 +1935,91 @@ int oracle_sao_decide_picture(const vvclf_picture* src, const vvclf_picture* rec
   return VVCLF_OK;
 }
 
+/* chroma_independent_bit_cost, src/sao_enc.rs:670-696 */
+static int64_t sao_chroma_indep_bits(const vvclf_sao_ctb_params* p) {
+  int64_t bits = 1;
+  const vvclf_sao_ctb* c[2] = {&p->cb, &p->cr};
+  for (int k = 0; k < 2; k++)
+    bits += c[k]->sao_type_idx == VVCLF_SAO_BAND ? 16 : (c[k]->sao_type_idx == VVCLF_SAO_EDGE ? 10 : 1);
+  return bits;
+}
+
+/* chroma_ctb_sse_with, src/sao_enc.rs:698-749: Cb + Cr SSE of chroma CTB
+ * (rx, ry) after SAO with (cb, cr) on the pre-SAO planes; `tmp` holds copies
+ * of those planes and is restored on return */
+static uint64_t sao_chroma_sse_with(const vvclf_picture* src, const vvclf_picture* rec, vvclf_plane* tmp,
+                                    uint32_t rx, uint32_t ry, uint32_t cctb, uint32_t bd, const vvclf_sao_ctb* cb,
+                                    const vvclf_sao_ctb* cr) {
+  const vvclf_sao_ctb* prm[2] = {cb, cr};
+  const vvclf_plane* pre[2] = {&rec->cb, &rec->cr};
+  for (int k = 0; k < 2; k++)
+    if (prm[k]->sao_type_idx) sao_ctb(&tmp[k], pre[k]->samples, NULL, rx, ry, cctb, cctb, bd, prm[k]);
+  uint32_t cx = rx * cctb, cy = ry * cctb, W = tmp[0].width, H = tmp[0].height;
+  uint32_t cw = cx < W ? (W - cx < cctb ? W - cx : cctb) : 0, ch = cy < H ? (H - cy < cctb ? H - cy : cctb) : 0;
+  uint64_t sse = 0;
+  for (uint32_t y = cy; y < cy + ch; y++)
+    for (uint32_t x = cx; x < cx + cw; x++) {
+      int64_t d0 = (int64_t)src->cb.samples[(size_t)y * src->cb.stride + x] - tmp[0].samples[(size_t)y * tmp[0].stride + x];
+      int64_t d1 = (int64_t)src->cr.samples[(size_t)y * src->cr.stride + x] - tmp[1].samples[(size_t)y * tmp[1].stride + x];
+      sse += (uint64_t)(d0 * d0 + d1 * d1);
+    }
+  for (int k = 0; k < 2; k++) /* restore the CTB region */
+    for (uint32_t y = cy; y < cy + ch; y++)
+      memcpy(tmp[k].samples + (size_t)y * tmp[k].stride + cx, pre[k]->samples + (size_t)y * pre[k]->stride + cx,
+             (size_t)cw * 2);
+  return sse;
+}
+
+/* apply_chroma_sao_merge, src/sao_enc.rs:525-660: per CTB in raster order,
+ * keep the independent chroma parameters or take the (already decided) left
+ * or above neighbour's, by SSE + 8 * bits (merge left 1 bit, above 2; ties
+ * keep the earlier option); map: 0 independent, 1 merge left, 2 merge above */
+int oracle_sao_chroma_merge(const vvclf_picture* src, const vvclf_picture* rec, const vvclf_sao_ctb_params* indep,
+                            uint32_t bd, uint32_t ctb_log2, vvclf_sao_ctb_params* merged, uint8_t* map) {
+  uint32_t ctb = 1u << ctb_log2, W = rec->luma.width, H = rec->luma.height;
+  uint32_t wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb, cctb = ctb / 2;
+  size_t n = (size_t)wc * hc;
+  memcpy(merged, indep, n * sizeof(*merged));
+  vvclf_plane tmp[2] = {rec->cb, rec->cr};
+  tmp[0].samples = dup_plane(&rec->cb);
+  tmp[1].samples = dup_plane(&rec->cr);
+  for (uint32_t ry = 0; ry < hc; ry++)
+    for (uint32_t rx = 0; rx < wc; rx++) {
+      vvclf_sao_ctb_params* cur = &merged[(size_t)ry * wc + rx];
+      vvclf_sao_ctb_params in = *cur;
+      int64_t best = (int64_t)sao_chroma_sse_with(src, rec, tmp, rx, ry, cctb, bd, &in.cb, &in.cr) +
+                     8 * sao_chroma_indep_bits(&in);
+      int choice = 0;
+      vvclf_sao_ctb bcb = in.cb, bcr = in.cr;
+      if (rx > 0) {
+        const vvclf_sao_ctb_params* l = &merged[(size_t)ry * wc + rx - 1];
+        int64_t c = (int64_t)sao_chroma_sse_with(src, rec, tmp, rx, ry, cctb, bd, &l->cb, &l->cr) + 8 * 1;
+        if (c < best) {
+          best = c;
+          choice = 1;
+          bcb = l->cb;
+          bcr = l->cr;
+        }
+      }
+      if (ry > 0) {
+        const vvclf_sao_ctb_params* a = &merged[(size_t)(ry - 1) * wc + rx];
+        int64_t c = (int64_t)sao_chroma_sse_with(src, rec, tmp, rx, ry, cctb, bd, &a->cb, &a->cr) + 8 * 2;
+        if (c < best) {
+          best = c;
+          choice = 2;
+          bcb = a->cb;
+          bcr = a->cr;
+        }
+      }
+      map[(size_t)ry * wc + rx] = (uint8_t)choice;
+      cur->cb = bcb;
+      cur->cr = bcr;
+    }
+  free(tmp[0].samples);
+  free(tmp[1].samples);
+  return VVCLF_OK;
+}
+
 int oracle_apply_in_loop_filters(vvclf_frame* f) {
   static const vvclf_deblock_cu empty_cus[1];
   const vvclf_deblock_cu* ccus = f->has_chroma_cus ? (f->chroma_cus ? f->chroma_cus : empty_cus) : NULL;

@@ -103,6 +103,10 @@ int oracle_cc_alf_decide(const vvclf_picture* src, vvclf_picture* rec, const vvc
  * order), from per-CTB band and edge statistics of rec against src */
 int oracle_sao_decide_picture(const vvclf_picture* src, const vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd,
                               uint32_t sao_luma, uint32_t sao_chroma, vvclf_sao_ctb_params* out);
+/* src/sao_enc.rs:525-660: chroma SAO merge decision over independent
+ * parameters (map: 0 independent, 1 merge left, 2 merge above) */
+int oracle_sao_chroma_merge(const vvclf_picture* src, const vvclf_picture* rec, const vvclf_sao_ctb_params* indep,
+                            uint32_t bd, uint32_t ctb_log2, vvclf_sao_ctb_params* merged, uint8_t* map);
 
 /* src/ctu.rs:11822-12066 */
 int oracle_apply_in_loop_filters(vvclf_frame* frame);

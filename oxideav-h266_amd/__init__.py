@@ -251,6 +251,8 @@ SIGNATURES = {
     "vvclf_cc_alf_decide": (C.c_int, [P(Picture), P(Picture), P(Plane), P(AlfCtb), P(AlfAps), u32, u32, u32, u32,
                                        P(Bounds), P(u64)]),
     "vvclf_sao_decide_picture": (C.c_int, [P(Picture), P(Picture), u32, u32, u32, u32, P(SaoCtbParams)]),
+    "vvclf_sao_chroma_merge": (C.c_int, [P(Picture), P(Picture), P(SaoCtbParams), u32, u32, P(SaoCtbParams),
+                                          P(u8)]),
     "vvclf_jobs_graph_create": (C.c_int, [VP, P(VP), u32, u32, P(VP)]),
     "vvclf_jobs_graph_launch": (C.c_int, [VP, VP]),
     "vvclf_jobs_graph_destroy": (C.c_int, [VP]),

@@ -559,6 +559,109 @@ __global__ void __launch_bounds__(256) k_sao_decide(sao_args a, int n_ctbs, vvcl
   out[i] = p;
 }
 
+// ---------------------------------------------------------------------------
+// Chroma SAO merge decision (apply_chroma_sao_merge, src/sao_enc.rs:525-660).
+// A CTB's choice reads its left and above neighbours' final parameters, so
+// the CTBs are decided in anti-diagonal waves (one launch per diagonal, every
+// CTB of a diagonal in parallel); a workgroup per CTB evaluates the three
+// candidates (independent, merge left, merge above) over the chroma CTB in
+// one pass: SAO of each candidate on the pre-SAO samples (as apply_sao_ctb,
+// src/sao.rs:504-626, without tile clips), Cb + Cr SSE against the source.
+// ---------------------------------------------------------------------------
+struct merge_args {
+  const uint16_t* rec[2];  // pre-SAO Cb, Cr (pitch)
+  const uint16_t* src[2];
+  int pitch, cw, ch;        // chroma planes (both the same size)
+  int wctb, hctb, cctb, bd;
+  vvclf_sao_ctb_params* merged;  // in: independent parameters; out: merged
+  uint8_t* map;                  // 0 independent, 1 merge left, 2 merge above
+};
+
+// SAO of chroma sample (x, y) with parameters p, reading the pre-SAO plane
+__device__ __forceinline__ int sao_sample(const uint16_t* pre, int pitch, int pw, int ph, int x, int y,
+                                          const vvclf_sao_ctb& p, int bd) {
+  const int maxv = (1 << bd) - 1;
+  const int v = pre[(size_t)y * pitch + x];
+  if (p.sao_type_idx == VVCLF_SAO_EDGE) {
+    const int c = p.eo_class & 3;
+    const int dx = c == 1 ? 0 : (c == 3 ? 1 : -1), dy = c == 0 ? 0 : -1;  // n0; n1 mirrored
+    const int x0 = x + dx, y0 = y + dy, x1 = x - dx, y1 = y - dy;
+    if (x0 < 0 || y0 < 0 || x1 < 0 || y1 < 0 || x0 >= pw || y0 >= ph || x1 >= pw || y1 >= ph) return min(v, maxv);
+    const int n0 = pre[(size_t)y0 * pitch + x0], n1 = pre[(size_t)y1 * pitch + x1];
+    int e = 2 + (v > n0) - (v < n0) + (v > n1) - (v < n1);
+    if (e <= 2) e = e == 2 ? 0 : e + 1;
+    return clampi(v + (e ? p.offset_val[e] : 0), 0, maxv);
+  }
+  if (p.sao_type_idx == VVCLF_SAO_BAND) {
+    const int k = (((v >> (bd - 5)) & 31) - p.band_position) & 31;
+    return clampi(v + (k < 4 ? p.offset_val[k + 1] : 0), 0, maxv);
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_sao_merge_step(merge_args a, int d, int rx_lo) {
+  __shared__ unsigned long long part[4][3];
+  const int rx = rx_lo + blockIdx.x, ry = d - rx;
+  const size_t i = (size_t)ry * a.wctb + rx;
+  const vvclf_sao_ctb_params in = a.merged[i];
+  const vvclf_sao_ctb_params lf = rx > 0 ? a.merged[i - 1] : in;
+  const vvclf_sao_ctb_params ab = ry > 0 ? a.merged[i - a.wctb] : in;
+  const int x0 = rx * a.cctb, y0 = ry * a.cctb;
+  const int cw = max(0, min(a.cctb, a.cw - x0)), ch = max(0, min(a.cctb, a.ch - y0));
+  unsigned long long s0 = 0, s1 = 0, s2 = 0;
+  for (int t = threadIdx.x; t < cw * ch; t += 256) {
+    const int x = x0 + t % cw, y = y0 + t / cw;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int sv = a.src[c][(size_t)y * a.pitch + x];
+      const vvclf_sao_ctb &pi = c ? in.cr : in.cb, &pl = c ? lf.cr : lf.cb, &pa = c ? ab.cr : ab.cb;
+      long long dd = sv - sao_sample(a.rec[c], a.pitch, a.cw, a.ch, x, y, pi, a.bd);
+      s0 += (unsigned long long)(dd * dd);
+      dd = sv - sao_sample(a.rec[c], a.pitch, a.cw, a.ch, x, y, pl, a.bd);
+      s1 += (unsigned long long)(dd * dd);
+      dd = sv - sao_sample(a.rec[c], a.pitch, a.cw, a.ch, x, y, pa, a.bd);
+      s2 += (unsigned long long)(dd * dd);
+    }
+  }
+  s0 = wave_sum64(s0);
+  s1 = wave_sum64(s1);
+  s2 = wave_sum64(s2);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part[w][0] = s0;
+    part[w][1] = s1;
+    part[w][2] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  unsigned long long t0 = 0, t1 = 0, t2 = 0;
+  for (int k = 0; k < 4; k++) {
+    t0 += part[k][0];
+    t1 += part[k][1];
+    t2 += part[k][2];
+  }
+  auto bits = [](const vvclf_sao_ctb& c) -> long long {
+    return c.sao_type_idx == VVCLF_SAO_BAND ? 16 : (c.sao_type_idx == VVCLF_SAO_EDGE ? 10 : 1);
+  };
+  long long best = (long long)t0 + 8 * (1 + bits(in.cb) + bits(in.cr));
+  int choice = 0;
+  vvclf_sao_ctb bcb = in.cb, bcr = in.cr;
+  if (rx > 0 && (long long)t1 + 8 < best) {
+    best = (long long)t1 + 8;
+    choice = 1;
+    bcb = lf.cb;
+    bcr = lf.cr;
+  }
+  if (ry > 0 && (long long)t2 + 16 < best) {
+    choice = 2;
+    bcb = ab.cb;
+    bcr = ab.cr;
+  }
+  a.map[i] = (uint8_t)choice;
+  a.merged[i].cb = bcb;
+  a.merged[i].cr = bcr;
+}
+
 }  // namespace
 
 extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* rec, uint32_t ctb_log2, uint32_t bd,
@@ -941,6 +1044,76 @@ extern "C" int vvclf_sao_decide_picture(const vvclf_picture* src, const vvclf_pi
   if (bad)
     return fail(VVCLF_ERR_INVALID, "sao_decide_picture: a reconstruction sample of 2^%u or more in a band statistic",
                 bd);
+  return VVCLF_OK;
+}
+
+extern "C" int vvclf_sao_chroma_merge(const vvclf_picture* src, const vvclf_picture* rec,
+                                      const vvclf_sao_ctb_params* independent, uint32_t bd, uint32_t ctb_log2,
+                                      vvclf_sao_ctb_params* merged, uint8_t* merge_map) {
+  clear_error();
+  if (!src || !rec || !independent || !merged || !merge_map)
+    return fail(VVCLF_ERR_INVALID, "sao_chroma_merge: NULL argument");
+  if (bd < 8 || bd > 16) return fail(VVCLF_ERR_INVALID, "sao_chroma_merge: bit depth %u", bd);
+  if (ctb_log2 < 3 || ctb_log2 > 8) return fail(VVCLF_ERR_UNSUPPORTED, "sao_chroma_merge: ctb_log2_size_y %u", ctb_log2);
+  const uint32_t W = rec->luma.width, H = rec->luma.height;
+  if (W == 0 || H == 0 || W > 65535 || H > 65535)
+    return fail(VVCLF_ERR_UNSUPPORTED, "sao_chroma_merge: picture %ux%u", W, H);
+  const vvclf_plane* rp[2] = {&rec->cb, &rec->cr};
+  const vvclf_plane* sp[2] = {&src->cb, &src->cr};
+  const uint32_t cw = rp[0]->width, ch = rp[0]->height;
+  for (int c = 0; c < 2; c++)
+    if (rp[c]->width != cw || rp[c]->height != ch || sp[c]->width != cw || sp[c]->height != ch ||
+        (cw && (!rp[c]->samples || !sp[c]->samples)))
+      return fail(VVCLF_ERR_INVALID, "sao_chroma_merge: chroma planes of source and reconstruction differ");
+  if (cw && ((cw != W / 2 && cw != (W + 1) / 2) || (ch != H / 2 && ch != (H + 1) / 2)))
+    return fail(VVCLF_ERR_UNSUPPORTED, "sao_chroma_merge: chroma planes other than 4:2:0");
+  const uint32_t ctb = 1u << ctb_log2, wctb = (W + ctb - 1) / ctb, hctb = (H + ctb - 1) / ctb;
+  const size_t n = (size_t)wctb * hctb;
+  vvclf_ctx* ctx = default_ctx();
+  if (!ctx) return VVCLF_ERR_HIP;
+  hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  const uint32_t pitch = (cw + 63) / 64 * 64;
+  dbuf dr[2], ds[2], dp, dm;
+  hipError_t e = hipSuccess;
+  merge_args a;
+  memset(&a, 0, sizeof(a));
+  for (int c = 0; c < 2 && e == hipSuccess; c++) {
+    e = dr[c].alloc((size_t)pitch * ch * 2, st);
+    if (e == hipSuccess) e = ds[c].alloc((size_t)pitch * ch * 2, st);
+    if (e == hipSuccess && cw && ch)
+      e = hipMemcpy2DAsync(dr[c].p, (size_t)pitch * 2, rp[c]->samples, (size_t)rp[c]->stride * 2, (size_t)cw * 2, ch,
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && cw && ch)
+      e = hipMemcpy2DAsync(ds[c].p, (size_t)pitch * 2, sp[c]->samples, (size_t)sp[c]->stride * 2, (size_t)cw * 2, ch,
+                           hipMemcpyHostToDevice, st);
+    a.rec[c] = (const uint16_t*)dr[c].p;
+    a.src[c] = (const uint16_t*)ds[c].p;
+  }
+  if (e == hipSuccess) e = dp.alloc(n * sizeof(vvclf_sao_ctb_params), st);
+  if (e == hipSuccess) e = dm.alloc(n, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dp.p, independent, n * sizeof(vvclf_sao_ctb_params), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "sao_chroma_merge: device buffers: %s", hipGetErrorString(e));
+  a.pitch = (int)pitch;
+  a.cw = (int)cw;
+  a.ch = (int)ch;
+  a.wctb = (int)wctb;
+  a.hctb = (int)hctb;
+  a.cctb = (int)(ctb / 2);
+  a.bd = (int)bd;
+  a.merged = (vvclf_sao_ctb_params*)dp.p;
+  a.map = (uint8_t*)dm.p;
+  // anti-diagonals d = rx + ry in order: each CTB's neighbours are decided
+  for (int d = 0; d <= (int)(wctb + hctb) - 2; d++) {
+    const int lo = std::max(0, d - ((int)hctb - 1)), hi = std::min(d, (int)wctb - 1);
+    hipLaunchKernelGGL(k_sao_merge_step, dim3(hi - lo + 1), dim3(256), 0, st, a, d, lo);
+  }
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(merged, dp.p, n * sizeof(vvclf_sao_ctb_params), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(merge_map, dm.p, n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "sao_chroma_merge");
   return VVCLF_OK;
 }
 

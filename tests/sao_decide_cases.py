@@ -148,3 +148,89 @@ def case_samples_beyond_the_bit_depth(vv, fn):
 
 
 ALL_CASES = {k: v for k, v in globals().items() if k.startswith("case_")}
+
+
+# ------------------------------------------------ chroma SAO merge decision
+def merge(vv, fn, src, rec, indep, ctb_log2, bd):
+    n = len(indep)
+    arr = (vv.SaoCtbParams * n)(*indep)
+    out = (vv.SaoCtbParams * n)()
+    m = (C.c_uint8 * n)()
+    rc = fn(src.ptr(), rec.ptr(), arr, bd, ctb_log2, out, m)
+    return rc, [out[i] for i in range(n)], list(m)
+
+
+def merge_case_single_ctb_keeps_independent(vv, fn, decide_fn):
+    # src/sao_enc.rs:845-859
+    src = pic(vv, 64, 64, 8, 100, 128)
+    rec = src.copy()
+    rec.cb.ravel()[0] ^= 0x10
+    rc, indep = run(vv, decide_fn, src, rec, 7, 8)
+    assert rc == 0
+    rc, out, m = merge(vv, fn, src, rec, indep, 7, 8)
+    assert rc == 0 and m == [0] and fields(out[0].cb) == fields(indep[0].cb)
+
+
+def merge_case_flat_picture_merges_everywhere(vv, fn, decide_fn):
+    # src/sao_enc.rs:863-897, worked out: identical CTBs give every candidate
+    # the same SSE, so a neighbour's parameters cost 8 (left) or 16 (above)
+    # against 8 * 33 for two independent band offsets: every CTB after the
+    # first merges left, the first of each other row merges up
+    src = pic(vv, 128, 128, 8, 100, 100)
+    rec = src.copy()
+    rec.cb[:] = 110
+    rec.cr[:] = 90
+    rc, indep = run(vv, decide_fn, src, rec, 5, 8, luma=False)
+    assert rc == 0 and all(fields(o.cb)[0] == 1 for o in indep)
+    rc, out, m = merge(vv, fn, src, rec, indep, 5, 8)
+    assert rc == 0
+    assert m == [0 if i == 0 else (1 if i % 4 else 2) for i in range(16)]
+
+
+def merge_case_rate_weight(vv, fn, decide_fn):
+    # picture 24x16 (chroma 12x8), CTB 16: CTB 0 exact (both chroma planes
+    # not applied), CTB 1 (chroma columns 8..11, 32 samples per plane) two
+    # levels high in one band: its independent pick is band offset -1 on Cb
+    # and Cr, SSE 2 * 32 * 1 = 64, bits 1 + 16 + 16; merging left (not
+    # applied) leaves SSE 2 * 32 * 4 = 256: 256 + 8 < 64 + 8 * 33, merge.
+    # (A weight of 4, or 10 bits for a band offset, would keep it
+    # independent.)  The same pair stacked merges up: 256 + 16 < 328.
+    for w, h, idx in ((24, 16, 1), (16, 24, 1)):
+        src, rec = pic(vv, w, h, 8, 100, 102), pic(vv, w, h, 8, 100, 102)
+        if w > h:
+            src.cb[:, 8:] = 100
+            src.cr[:, 8:] = 100
+        else:
+            src.cb[8:, :] = 100
+            src.cr[8:, :] = 100
+        rc, indep = run(vv, decide_fn, src, rec, 4, 8, luma=False)
+        assert rc == 0 and fields(indep[0].cb) == NA and fields(indep[idx].cb) == (1, 0, 9, (0, 0, 0, 0, -1))
+        assert fields(indep[idx].cr) == (1, 0, 9, (0, 0, 0, 0, -1))
+        rc, out, m = merge(vv, fn, src, rec, indep, 4, 8)
+        assert rc == 0 and m == ([0, 1] if w > h else [0, 2])
+        assert fields(out[idx].cb) == NA and fields(out[idx].cr) == NA
+        # with a source three levels low the band offset (-2) gains more
+        # than the merge saves: 2 * 32 * 1 + 264 against 2 * 32 * 9 + 8
+        src.cb[src.cb == 100] = 99
+        src.cr[src.cr == 100] = 99
+        rc, indep = run(vv, decide_fn, src, rec, 4, 8, luma=False)
+        rc, out, m = merge(vv, fn, src, rec, indep, 4, 8)
+        assert rc == 0 and m == [0, 0]
+
+
+def merge_case_above_costs_two_bits(vv, fn, decide_fn):
+    # picture 12x30 (chroma 6x15), CTB 16: the lower CTB's chroma region is
+    # 6 x 7 = 42 samples per plane, two levels high: independent band offset
+    # -1 gives 2 * 42 * 1 + 8 * 33 = 348, merging up (not applied)
+    # 2 * 42 * 4 + 8 * 2 = 352: independent by 4 (a 1-bit merge-up would
+    # merge: 344)
+    src, rec = pic(vv, 12, 30, 8, 100, 102), pic(vv, 12, 30, 8, 100, 102)
+    src.cb[8:, :] = 100
+    src.cr[8:, :] = 100
+    rc, indep = run(vv, decide_fn, src, rec, 4, 8, luma=False)
+    assert rc == 0 and fields(indep[1].cb) == (1, 0, 9, (0, 0, 0, 0, -1)) and fields(indep[0].cb) == NA
+    rc, out, m = merge(vv, fn, src, rec, indep, 4, 8)
+    assert rc == 0 and m == [0, 0]
+
+
+ALL_MERGE_CASES = {k: v for k, v in globals().items() if k.startswith("merge_case_")}

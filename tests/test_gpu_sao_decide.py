@@ -81,3 +81,27 @@ def test_decision_matches_oracle(vv, orc, w, h, bd, ctb_log2, luma, chroma, mono
     kinds = {cases.fields(getattr(o, c))[0] for o in want for c in ("luma", "cb", "cr")}
     if w * h >= 100000:
         assert {1, 2} <= kinds, kinds  # band and edge offsets both chosen somewhere
+
+
+@pytest.mark.parametrize("name", sorted(cases.ALL_MERGE_CASES))
+def test_merge_case_through_the_gpu(vv, name):
+    say(f"sao merge case {name}")
+    cases.ALL_MERGE_CASES[name](vv, vv.lib.vvclf_sao_chroma_merge, vv.lib.vvclf_sao_decide_picture)
+
+
+@pytest.mark.parametrize("w,h,bd,ctb_log2", [(416, 240, 10, 6), (333, 199, 8, 5), (1920, 1080, 10, 7),
+                                            (256, 256, 12, 4), (96, 72, 10, 3)])
+def test_merge_matches_oracle(vv, orc, w, h, bd, ctb_log2):
+    say(f"sao chroma merge {w}x{h} bd={bd} ctb_log2={ctb_log2}")
+    rng = np.random.default_rng(w + h * 5 + bd)
+    src, rec = deblocked_like(vv, rng, w, h, bd)
+    rc, indep = cases.run(vv, orc.oracle_sao_decide_picture, src, rec, ctb_log2, bd)
+    assert rc == 0
+    rc, got, gm = cases.merge(vv, vv.lib.vvclf_sao_chroma_merge, src, rec, indep, ctb_log2, bd)
+    assert rc == 0, vv.lib.vvclf_last_error()
+    rc, want, wm = cases.merge(vv, orc.oracle_sao_chroma_merge, src, rec, indep, ctb_log2, bd)
+    assert rc == 0 and gm == wm
+    for g, o in zip(got, want):
+        for comp in ("luma", "cb", "cr"):
+            assert cases.fields(getattr(g, comp)) == cases.fields(getattr(o, comp))
+    assert set(wm) >= {0, 1}, set(wm)  # independent and merged CTBs both occur

@@ -109,6 +109,12 @@ MUTANTS = [
     ("sao-chroma-ceil", "uint32_t cx = x0 / 2, cy = y0 / 2, cw = w / 2, ch = h / 2;",
      "uint32_t cx = x0 / 2, cy = y0 / 2, cw = (w + 1) / 2, ch = (h + 1) / 2;", 0,
      ["sao:case_chroma_region_rounds_down"]),
+    ("sao-merge-lambda", "8 * sao_chroma_indep_bits(&in);", "4 * sao_chroma_indep_bits(&in);", 0,
+     ["merge:merge_case_rate_weight"]),
+    ("sao-merge-band-bits", "bits += c[k]->sao_type_idx == VVCLF_SAO_BAND ? 16 :",
+     "bits += c[k]->sao_type_idx == VVCLF_SAO_BAND ? 10 :", 0, ["merge:merge_case_rate_weight"]),
+    ("sao-merge-above-bits", "cctb, bd, &a->cb, &a->cr) + 8 * 2;", "cctb, bd, &a->cb, &a->cr) + 8 * 1;", 0,
+     ["merge:merge_case_above_costs_two_bits"]),
 ]
 
 
@@ -159,6 +165,9 @@ def test_pins_catch_mutant(mutant, mutant_libs, vv):
         try:
             if pin.startswith("sao:"):
                 sao_decide_cases.ALL_CASES[pin[4:]](vv, lib.oracle_sao_decide_picture)
+            elif pin.startswith("merge:"):
+                sao_decide_cases.ALL_MERGE_CASES[pin[6:]](vv, lib.oracle_sao_chroma_merge,
+                                                          lib.oracle_sao_decide_picture)
             else:
                 pins.ALL_PINS[pin](lib, vv)
         except AssertionError as e:
