@@ -471,6 +471,35 @@ int vvclf_sao_chroma_merge(const vvclf_picture* src, const vvclf_picture* rec_pr
                            const vvclf_sao_ctb_params* independent, uint32_t bit_depth, uint32_t ctb_log2_size_y,
                            vvclf_sao_ctb_params* merged, uint8_t* merge_map);
 
+/* DesignedLumaAlfClipRdo, src/alf_aps_design.rs:552-597. */
+typedef struct vvclf_alf_clip_rdo {
+  int32_t coeff[VVCLF_ALF_NUM_FILTERS][VVCLF_ALF_LUMA_NUM_COEFF]; /* the input rows, carried over */
+  uint8_t clip_idx[VVCLF_ALF_NUM_FILTERS][VVCLF_ALF_LUMA_NUM_COEFF];
+  uint8_t alf_luma_clip_flag, pad[3];
+  uint64_t baseline_sse_y;  /* luma SSE with clip index 0 everywhere */
+  uint64_t post_clip_sse_y; /* luma SSE with clip_idx */
+} vvclf_alf_clip_rdo;
+
+/* Replaces design_clip_rdo_for_luma_aps (src/alf_aps_design.rs:697-768): a
+ * greedy search of alf_luma_clip_idx for an APS whose 25 luma rows are
+ * `coeff`.  From clip index 0 everywhere, for filtIdx 0 .. 24 and tap
+ * j 0 .. 11 in that order, indices 1, 2, 3 are tried; the one giving the
+ * lowest luma SSE against `src` (ALF on `rec_pre_alf` with every CTB on the
+ * APS, luma only, picture-wide clip) is kept if it is strictly below the
+ * current SSE (ties keep the earlier index), else the tap stays 0.
+ * alf_luma_clip_flag: any index nonzero.  On the GPU: a 4x4 block's ALF
+ * output depends only on its class's row, so the SSE splits over classes
+ * and the 25 rows are searched together: per tap one pass over the picture
+ * filters each block with its row's three trial indices and sums the
+ * squared error per class (13 passes against the reference's 901 picture
+ * replays); the picks and the next trial banks are made on the device.
+ * Coefficients outside 16 bits: VVCLF_ERR_UNSUPPORTED.  Host pictures,
+ * luma read. */
+int vvclf_alf_clip_rdo_luma(const vvclf_picture* src, const vvclf_picture* rec_pre_alf,
+                            const int32_t coeff[VVCLF_ALF_NUM_FILTERS][VVCLF_ALF_LUMA_NUM_COEFF],
+                            uint32_t ctb_log2_size_y, uint32_t bit_depth, uint32_t chroma_format_idc,
+                            vvclf_alf_clip_rdo* out);
+
 /* ------------------------------------------------------------------------ */
 /* Per-CTB side information as a compact record: the syntax elements a      */
 /* slice decoder has parsed for one CTB, from which the SAO and ALF CTB      */

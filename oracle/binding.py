@@ -81,6 +81,8 @@ def load(vv=None, path=None):
                                                 P(vv.SaoCtbParams)]),
         "oracle_sao_chroma_merge": (C.c_int, [P(vv.Picture), P(vv.Picture), P(vv.SaoCtbParams), u32, u32,
                                               P(vv.SaoCtbParams), P(C.c_uint8)]),
+        "oracle_alf_clip_rdo_luma": (C.c_int, [P(vv.Picture), P(vv.Picture), P(C.c_int32), u32, u32, u32,
+                                               P(vv.AlfClipRdo)]),
         "oracle_apply_in_loop_filters_mt": (C.c_int, [P(P(vv.Frame)), u32, u32]),
     }
     for name, (res, args) in sig.items():

@@ -2020,6 +2020,93 @@ int oracle_sao_chroma_merge(const vvclf_picture* src, const vvclf_picture* rec, 
   return VVCLF_OK;
 }
 
+/* total_sse_luma, src/alf_aps_design.rs:599-612 (planes of equal size) */
+static uint64_t total_sse_luma(const vvclf_plane* a, const vvclf_plane* b) {
+  uint32_t w = minu(a->width, b->width), h = minu(a->height, b->height);
+  uint64_t sse = 0;
+  for (uint32_t y = 0; y < h; y++)
+    for (uint32_t x = 0; x < w; x++) {
+      int64_t d = (int64_t)a->samples[(size_t)y * a->stride + x] - b->samples[(size_t)y * b->stride + x];
+      sse += (uint64_t)(d * d);
+    }
+  return sse;
+}
+
+/* replay_aps_full_picture_sse, src/alf_aps_design.rs:618-664: ALF of a copy
+ * of rec with every CTB on APS slot 0 (set 16), luma only, picture-wide
+ * clip; the luma SSE against src */
+static uint64_t clip_rdo_replay(const vvclf_picture* src, const vvclf_picture* rec, const vvclf_alf_aps* aps,
+                                uint32_t ctb_log2, uint32_t bd, uint32_t cfi, uint16_t* buf, vvclf_alf_ctb* ctbs,
+                                uint32_t wc, uint32_t hc) {
+  size_t plane = (size_t)rec->luma.stride * rec->luma.height;
+  memcpy(buf, rec->luma.samples, plane * 2);
+  vvclf_picture r = *rec;
+  r.luma.samples = buf;
+  vvclf_alf_config cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.alf_enabled = 1;
+  cfg.bit_depth = bd;
+  cfg.ctb_log2_size_y = ctb_log2;
+  cfg.chroma_format_idc = cfi;
+  const vvclf_alf_aps* slot[1] = {aps};
+  vvclf_alf_aps_binding b;
+  memset(&b, 0, sizeof(b));
+  b.luma_apses = slot;
+  b.n_luma_apses = 1;
+  for (size_t i = 0; i < (size_t)wc * hc; i++) {
+    memset(&ctbs[i], 0, sizeof(ctbs[i]));
+    ctbs[i].luma_on = 1;
+    ctbs[i].luma_filt_set_idx = 16;
+  }
+  vvclf_alf_picture ap = {wc, hc, ctbs};
+  oracle_apply_alf_clipped(&r, &ap, &cfg, &b, NULL);
+  return total_sse_luma(&src->luma, &r.luma);
+}
+
+/* design_clip_rdo_for_luma_aps, src/alf_aps_design.rs:697-768: greedy
+ * per-class, per-tap descent over clip indices 1..3, each trial a full
+ * picture replay; a trial is kept only when it strictly lowers the SSE */
+int oracle_alf_clip_rdo_luma(const vvclf_picture* src, const vvclf_picture* rec, const int32_t coeff[25][12],
+                             uint32_t ctb_log2, uint32_t bd, uint32_t cfi, vvclf_alf_clip_rdo* out) {
+  uint32_t ctb = 1u << ctb_log2;
+  uint32_t wc = (rec->luma.width + ctb - 1) / ctb, hc = (rec->luma.height + ctb - 1) / ctb;
+  memset(out, 0, sizeof(*out));
+  memcpy(out->coeff, coeff, sizeof(out->coeff));
+  vvclf_alf_aps* aps = (vvclf_alf_aps*)calloc(1, sizeof(vvclf_alf_aps));
+  aps->alf_luma_filter_signal_flag = 1;
+  aps->alf_luma_clip_flag = 1;
+  aps->n_luma_coeff = 25;
+  memcpy(aps->luma_coeff, coeff, sizeof(aps->luma_coeff));
+  uint16_t* buf = (uint16_t*)malloc((size_t)rec->luma.stride * rec->luma.height * 2 + 2);
+  vvclf_alf_ctb* ctbs = (vvclf_alf_ctb*)calloc((size_t)wc * hc, sizeof(vvclf_alf_ctb));
+  uint64_t running = clip_rdo_replay(src, rec, aps, ctb_log2, bd, cfi, buf, ctbs, wc, hc);
+  out->baseline_sse_y = running;
+  for (int k = 0; k < 25; k++)
+    for (int j = 0; j < 12; j++) {
+      uint8_t best_clip = 0;
+      uint64_t best = running;
+      for (uint8_t c = 1; c <= 3; c++) {
+        aps->luma_clip_idx[k][j] = c;
+        uint64_t t = clip_rdo_replay(src, rec, aps, ctb_log2, bd, cfi, buf, ctbs, wc, hc);
+        if (t < best) {
+          best = t;
+          best_clip = c;
+        }
+      }
+      aps->luma_clip_idx[k][j] = best_clip;
+      out->clip_idx[k][j] = best_clip;
+      running = best;
+    }
+  out->post_clip_sse_y = running;
+  for (int k = 0; k < 25; k++)
+    for (int j = 0; j < 12; j++)
+      if (out->clip_idx[k][j]) out->alf_luma_clip_flag = 1;
+  free(ctbs);
+  free(buf);
+  free(aps);
+  return VVCLF_OK;
+}
+
 int oracle_apply_in_loop_filters(vvclf_frame* f) {
   static const vvclf_deblock_cu empty_cus[1];
   const vvclf_deblock_cu* ccus = f->has_chroma_cus ? (f->chroma_cus ? f->chroma_cus : empty_cus) : NULL;

@@ -107,6 +107,10 @@ int oracle_sao_decide_picture(const vvclf_picture* src, const vvclf_picture* rec
  * parameters (map: 0 independent, 1 merge left, 2 merge above) */
 int oracle_sao_chroma_merge(const vvclf_picture* src, const vvclf_picture* rec, const vvclf_sao_ctb_params* indep,
                             uint32_t bd, uint32_t ctb_log2, vvclf_sao_ctb_params* merged, uint8_t* map);
+/* src/alf_aps_design.rs:697-768: greedy clip-index search of a 25-row luma
+ * APS by full-picture replays */
+int oracle_alf_clip_rdo_luma(const vvclf_picture* src, const vvclf_picture* rec, const int32_t coeff[25][12],
+                             uint32_t ctb_log2, uint32_t bd, uint32_t cfi, vvclf_alf_clip_rdo* out);
 
 /* src/ctu.rs:11822-12066 */
 int oracle_apply_in_loop_filters(vvclf_frame* frame);

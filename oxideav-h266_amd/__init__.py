@@ -124,6 +124,12 @@ class AlfAps(C.Structure):
                 ("cc_cb_coeff", (i32 * 7) * 4), ("cc_cr_coeff", (i32 * 7) * 4)]
 
 
+class AlfClipRdo(C.Structure):
+    """DesignedLumaAlfClipRdo, src/alf_aps_design.rs:552-597."""
+    _fields_ = [("coeff", (i32 * 12) * 25), ("clip_idx", (u8 * 12) * 25), ("alf_luma_clip_flag", u8),
+                ("pad", u8 * 3), ("baseline_sse_y", u64), ("post_clip_sse_y", u64)]
+
+
 class AlfApsBinding(C.Structure):
     """AlfApsBinding, src/alf.rs:180-195."""
     _fields_ = [("luma_apses", C.POINTER(C.POINTER(AlfAps))), ("n_luma_apses", u32), ("pad", u32),
@@ -218,7 +224,7 @@ STRUCTS = {"vvclf_plane": Plane, "vvclf_picture": Picture, "vvclf_deblock_cu": D
            "vvclf_mv_cell": MvCell, "vvclf_mv_grid": MvGrid, "vvclf_deblock_params": DeblockParams,
            "vvclf_sao_ctb": SaoCtb, "vvclf_sao_ctb_params": SaoCtbParams, "vvclf_sao_picture": SaoPicture,
            "vvclf_sao_config": SaoConfig, "vvclf_bounds": Bounds, "vvclf_alf_ctb": AlfCtb,
-           "vvclf_alf_picture": AlfPicture, "vvclf_alf_config": AlfConfig, "vvclf_alf_aps": AlfAps,
+           "vvclf_alf_picture": AlfPicture, "vvclf_alf_config": AlfConfig, "vvclf_alf_aps": AlfAps, "vvclf_alf_clip_rdo": AlfClipRdo,
            "vvclf_alf_aps_binding": AlfApsBinding, "vvclf_lmcs_data": LmcsData, "vvclf_lmcs_derived": LmcsDerived,
            "vvclf_lmcs_binding": LmcsBinding, "vvclf_frame": Frame, "vvclf_rect": Rect, "vvclf_crs_block": CrsBlock,
            "vvclf_crop": Crop, "vvclf_ctb_syntax": CtbSyntax, "vvclf_ctb_syntax_config": CtbSyntaxConfig, "vvclf_tune": Tune,
@@ -253,6 +259,7 @@ SIGNATURES = {
     "vvclf_sao_decide_picture": (C.c_int, [P(Picture), P(Picture), u32, u32, u32, u32, P(SaoCtbParams)]),
     "vvclf_sao_chroma_merge": (C.c_int, [P(Picture), P(Picture), P(SaoCtbParams), u32, u32, P(SaoCtbParams),
                                           P(u8)]),
+    "vvclf_alf_clip_rdo_luma": (C.c_int, [P(Picture), P(Picture), P(i32), u32, u32, u32, P(AlfClipRdo)]),
     "vvclf_jobs_graph_create": (C.c_int, [VP, P(VP), u32, u32, P(VP)]),
     "vvclf_jobs_graph_launch": (C.c_int, [VP, VP]),
     "vvclf_jobs_graph_destroy": (C.c_int, [VP]),

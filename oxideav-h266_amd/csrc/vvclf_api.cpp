@@ -504,7 +504,7 @@ size_t vvclf_abi_sizeof(const char* n) {
   if (!strcmp(n, #T)) return sizeof(T);
   S(vvclf_plane) S(vvclf_picture) S(vvclf_deblock_cu) S(vvclf_mv_cell) S(vvclf_mv_grid) S(vvclf_deblock_params)
   S(vvclf_sao_ctb) S(vvclf_sao_ctb_params) S(vvclf_sao_picture) S(vvclf_sao_config) S(vvclf_bounds)
-  S(vvclf_alf_ctb) S(vvclf_alf_picture) S(vvclf_alf_config) S(vvclf_alf_aps) S(vvclf_alf_aps_binding)
+  S(vvclf_alf_ctb) S(vvclf_alf_picture) S(vvclf_alf_config) S(vvclf_alf_aps) S(vvclf_alf_clip_rdo) S(vvclf_alf_aps_binding)
   S(vvclf_lmcs_data) S(vvclf_lmcs_derived) S(vvclf_lmcs_binding) S(vvclf_frame) S(vvclf_tune)
   S(vvclf_server_config) S(vvclf_server_stats) S(vvclf_rect) S(vvclf_crs_block) S(vvclf_crop)
   S(vvclf_ctb_syntax) S(vvclf_ctb_syntax_config)
@@ -621,16 +621,16 @@ int vvclf_job_set_input(vvclf_job* j, const vvclf_picture* pic) {
     return fail(VVCLF_ERR_INVALID, "job_set_input: luma geometry differs");
   hipSetDevice(j->ctx->device);
   hipStream_t ps = j->pool_stream;
-  hipError_t e = hipMemcpy2DAsync(j->d.in.y, (size_t)j->d.py * 2, pic->luma.samples, (size_t)pic->luma.stride * 2,
+  hipError_t e = copy_rows(j->d.in.y, (size_t)j->d.py * 2, pic->luma.samples, (size_t)pic->luma.stride * 2,
                                   (size_t)j->w * 2, j->h, hipMemcpyHostToDevice, ps);
   if (e != hipSuccess) return hip_fail(e, "upload luma");
   if (j->wc) {
     if (!pic->cb.samples || !pic->cr.samples || pic->cb.width != j->wc || pic->cb.height != j->hc)
       return fail(VVCLF_ERR_INVALID, "job_set_input: chroma geometry differs");
-    e = hipMemcpy2DAsync(j->d.in.cb, (size_t)j->d.pc * 2, pic->cb.samples, (size_t)pic->cb.stride * 2,
+    e = copy_rows(j->d.in.cb, (size_t)j->d.pc * 2, pic->cb.samples, (size_t)pic->cb.stride * 2,
                          (size_t)j->wc * 2, j->hc, hipMemcpyHostToDevice, ps);
     if (e == hipSuccess)
-      e = hipMemcpy2DAsync(j->d.in.cr, (size_t)j->d.pc * 2, pic->cr.samples, (size_t)pic->cr.stride * 2,
+      e = copy_rows(j->d.in.cr, (size_t)j->d.pc * 2, pic->cr.samples, (size_t)pic->cr.stride * 2,
                            (size_t)j->wc * 2, j->hc, hipMemcpyHostToDevice, ps);
     if (e != hipSuccess) return hip_fail(e, "upload chroma");
   }
@@ -754,14 +754,14 @@ int vvclf_job_download(vvclf_job* j, vvclf_picture* out) {
   if (e != hipSuccess) return hip_fail(e, "synchronize");
   if (out->luma.width != j->w || out->luma.height != j->h || !out->luma.samples)
     return fail(VVCLF_ERR_INVALID, "job_download: luma geometry differs");
-  e = hipMemcpy2D(out->luma.samples, (size_t)out->luma.stride * 2, job_final(j, 0), (size_t)j->d.py * 2,
+  e = copy_rows_sync(out->luma.samples, (size_t)out->luma.stride * 2, job_final(j, 0), (size_t)j->d.py * 2,
                   (size_t)j->w * 2, j->h, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(e, "download luma");
   if (j->wc) {
-    e = hipMemcpy2D(out->cb.samples, (size_t)out->cb.stride * 2, job_final(j, 1), (size_t)j->d.pc * 2,
+    e = copy_rows_sync(out->cb.samples, (size_t)out->cb.stride * 2, job_final(j, 1), (size_t)j->d.pc * 2,
                     (size_t)j->wc * 2, j->hc, hipMemcpyDeviceToHost);
     if (e == hipSuccess)
-      e = hipMemcpy2D(out->cr.samples, (size_t)out->cr.stride * 2, job_final(j, 2), (size_t)j->d.pc * 2,
+      e = copy_rows_sync(out->cr.samples, (size_t)out->cr.stride * 2, job_final(j, 2), (size_t)j->d.pc * 2,
                       (size_t)j->wc * 2, j->hc, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "download chroma");
   }

@@ -16,6 +16,12 @@
 //   k_alf_pick         per CTB the first minimum
 //   k_alf_enc<COMMIT>  the chosen candidate's output (or the pre-ALF
 //                      samples) written once
+// and the clip-index search of an APS (design_clip_rdo_for_luma_aps,
+// src/alf_aps_design.rs:697-768) on the same kernel:
+//   k_alf_enc<CLASS>   the block filtered with each candidate bank, the
+//                      squared error summed per class (the SSE splits over
+//                      classes, since a block's output depends only on its
+//                      class's row)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,6 +50,9 @@ struct enc_args {
   const int32_t* choice;       // [n_ctbs]: chosen trial, -1 off
 };
 
+enum { ENC_TRIAL = 0, ENC_COMMIT = 1, ENC_CLASS = 2 };
+constexpr int CLASS_MAX_CAND = 4;  // candidate banks of one ENC_CLASS launch
+
 __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -59,7 +68,7 @@ __device__ __forceinline__ int class_w(const int (&W)[10][10], int vb, bool carv
   return carve ? alf_block_class<2, true>(W, bd) : alf_block_class<2, false>(W, bd);
 }
 
-template <bool TRIAL, bool HBD>
+template <int MODE, bool HBD>
 __global__ void __launch_bounds__(256) k_alf_enc(enc_args a, int tile_log2) {
   const int tile = 1 << tile_log2, tb = tile >> 2;
   const int bx = threadIdx.x % tb, by = threadIdx.x / tb;
@@ -152,7 +161,32 @@ __global__ void __launch_bounds__(256) k_alf_enc(enc_args a, int tile_log2) {
     return wide ? W[rr + 3][c + 3] : (int)((P[rr + 3][(c + 4) >> 1] >> (16 * ((c + 4) & 1))) & 0xffffu);
   };
 
-  if (TRIAL) {
+  if constexpr (MODE == ENC_CLASS) {
+    // sse: [n_cand][25], per class; ok and the off candidate unused
+    __shared__ unsigned long long acc[CLASS_MAX_CAND * 25];
+    for (int i = threadIdx.x; i < CLASS_MAX_CAND * 25; i += blockDim.x) acc[i] = 0;
+    __syncthreads();
+    if (live) {
+      const int k = cls & 31;
+      for (int s = 0; s < a.n_cand; s++) {
+        int o[4][4];
+        filter(s, o);
+        unsigned long long es = 0;
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            if (rr < nr && c < nc) {
+              const long long d = (long long)a.src[(size_t)(y4 + rr) * a.pitch + x4 + c] - o[rr][c];
+              es += (unsigned long long)(d * d);
+            }
+        if (es) atomicAdd(&acc[s * 25 + k], es);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.n_cand * 25; i += blockDim.x)
+      if (acc[i]) atomicAdd(a.sse + i, acc[i]);
+  } else if constexpr (MODE == ENC_TRIAL) {
     int S[4][4];
 #pragma unroll
     for (int rr = 0; rr < 4; rr++)
@@ -731,10 +765,10 @@ extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* re
   if (e == hipSuccess) e = dchoice.alloc((size_t)n_ctbs * 4, st);
   if (e == hipSuccess) e = dok.alloc(n_cand, st);
   if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "alf_decide_luma: device buffers: %s", hipGetErrorString(e));
-  e = hipMemcpy2DAsync(drec.p, (size_t)pitch * 2, R.samples, (size_t)R.stride * 2, (size_t)W * 2, H,
+  e = copy_rows(drec.p, (size_t)pitch * 2, R.samples, (size_t)R.stride * 2, (size_t)W * 2, H,
                        hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
-    e = hipMemcpy2DAsync(dsrc.p, (size_t)pitch * 2, S.samples, (size_t)S.stride * 2, (size_t)W * 2, H,
+    e = copy_rows(dsrc.p, (size_t)pitch * 2, S.samples, (size_t)S.stride * 2, (size_t)W * 2, H,
                          hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(drect.p, rects.data(), rects.size() * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
@@ -763,12 +797,12 @@ extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* re
   const unsigned threads = std::max(64u, (unsigned)((tile / 4) * (tile / 4)));
   const bool hbd = bd > 12;
   if (e == hipSuccess) {
-    if (hbd) hipLaunchKernelGGL((k_alf_enc<true, true>), grid, dim3(threads), 0, st, a, tl);
-    else hipLaunchKernelGGL((k_alf_enc<true, false>), grid, dim3(threads), 0, st, a, tl);
+    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_TRIAL, true>), grid, dim3(threads), 0, st, a, tl);
+    else hipLaunchKernelGGL((k_alf_enc<ENC_TRIAL, false>), grid, dim3(threads), 0, st, a, tl);
     hipLaunchKernelGGL(k_alf_pick, dim3((n_ctbs + 255) / 256), dim3(256), 0, st, (unsigned long long*)dsse.p,
                        (const uint8_t*)dok.p, (int)n_ctbs, (int)n_cand, (int32_t*)dchoice.p);
-    if (hbd) hipLaunchKernelGGL((k_alf_enc<false, true>), grid, dim3(threads), 0, st, a, tl);
-    else hipLaunchKernelGGL((k_alf_enc<false, false>), grid, dim3(threads), 0, st, a, tl);
+    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_COMMIT, true>), grid, dim3(threads), 0, st, a, tl);
+    else hipLaunchKernelGGL((k_alf_enc<ENC_COMMIT, false>), grid, dim3(threads), 0, st, a, tl);
     e = hipGetLastError();
   }
   std::vector<int32_t> choice(n_ctbs);
@@ -778,7 +812,7 @@ extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* re
   if (e == hipSuccess && sse_out)
     e = hipMemcpyAsync(sse.data(), dsse.p, sse.size() * 8, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess)
-    e = hipMemcpy2DAsync(R.samples, (size_t)R.stride * 2, dout.p, (size_t)pitch * 2, (size_t)W * 2, H,
+    e = copy_rows(R.samples, (size_t)R.stride * 2, dout.p, (size_t)pitch * 2, (size_t)W * 2, H,
                          hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "alf_decide_luma");
@@ -790,6 +824,165 @@ extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* re
     }
   }
   if (sse_out) memcpy(sse_out, sse.data(), sse.size() * 8);
+  return VVCLF_OK;
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Clip-index search of a luma APS (design_clip_rdo_for_luma_aps,
+// src/alf_aps_design.rs:697-768).  The reference walks filtIdx 0..24 and tap
+// j 0..11, replaying the whole picture for clip indices 1, 2, 3 of the tap
+// and keeping the first strict improvement of the picture's SSE.  The SSE is
+// a sum over classes and a trial of row k changes only class k's part, so
+// "trial < best" compares class k's SSE alone and the rows' searches are
+// independent: per tap j, candidate bank c (c = 1, 2, 3) holds every row's
+// current clip indices with tap j set to c, one k_alf_enc<ENC_CLASS> pass
+// gives every (c, class) SSE, and k_clip_pick makes the 25 choices.
+// ---------------------------------------------------------------------------
+struct clip_tables {
+  uint8_t perm[4][12];  // APS tap read at bank position k, per transposeIdx
+  uint32_t clipw[4];    // clip word per clip index
+};
+
+// banks[b] (b < n_banks): rows with the current clip indices; with j >= 0,
+// tap j of every row takes clip index b + 1
+__global__ void __launch_bounds__(256) k_clip_banks(const int32_t* coeff, const uint8_t* clip, int j, clip_tables t,
+                                                    uint32_t* banks, int n_banks) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_banks * ALF_BANK_WORDS) return;
+  const int b = i / ALF_BANK_WORDS, w = i % ALF_BANK_WORDS;
+  const int k = w / 48, tr = (w / 12) & 3, q = w % 12;
+  const int src = t.perm[tr][q];
+  const int idx = (j >= 0 && src == j) ? b + 1 : clip[k * 12 + src];
+  banks[i] = ((uint32_t)(uint16_t)(int16_t)coeff[k * 12 + src]) | (t.clipw[idx] << 16);
+}
+
+// per class k: with j < 0 the baseline (trial row 0 becomes the current
+// SSE); else the first of indices 1..3 whose SSE is strictly below the
+// current one, or 0.  Consumed trial rows are cleared for the next pass.
+__global__ void k_clip_pick(unsigned long long* trial, unsigned long long* cur, unsigned long long* base,
+                            uint8_t* clip, int j) {
+  const int k = threadIdx.x;
+  if (k >= 25) return;
+  if (j < 0) {
+    cur[k] = base[k] = trial[k];
+    trial[k] = 0;
+    return;
+  }
+  unsigned long long best = cur[k];
+  int bc = 0;
+  for (int c = 1; c <= 3; c++) {
+    const unsigned long long v = trial[(c - 1) * 25 + k];
+    if (v < best) {
+      best = v;
+      bc = c;
+    }
+    trial[(c - 1) * 25 + k] = 0;
+  }
+  clip[k * 12 + j] = (uint8_t)bc;
+  cur[k] = best;
+}
+
+}  // namespace
+
+extern "C" int vvclf_alf_clip_rdo_luma(const vvclf_picture* src, const vvclf_picture* rec,
+                                       const int32_t coeff[VVCLF_ALF_NUM_FILTERS][VVCLF_ALF_LUMA_NUM_COEFF],
+                                       uint32_t ctb_log2, uint32_t bd, uint32_t cfi, vvclf_alf_clip_rdo* out) {
+  clear_error();
+  (void)cfi;  // the search filters luma only
+  if (!src || !rec || !coeff || !out) return fail(VVCLF_ERR_INVALID, "alf_clip_rdo_luma: NULL argument");
+  const vvclf_plane &S = src->luma, &R = rec->luma;
+  const uint32_t W = R.width, H = R.height;
+  if (!S.samples || !R.samples || S.width != W || S.height != H || W == 0 || H == 0)
+    return fail(VVCLF_ERR_INVALID, "alf_clip_rdo_luma: source and reconstruction luma differ");
+  if (bd < 8 || bd > 16) return fail(VVCLF_ERR_INVALID, "alf_clip_rdo_luma: bit depth %u", bd);
+  if (ctb_log2 < 3 || ctb_log2 > 8)
+    return fail(VVCLF_ERR_UNSUPPORTED, "alf_clip_rdo_luma: ctb_log2_size_y %u", ctb_log2);
+  if (W > 65535 || H > 65535) return fail(VVCLF_ERR_UNSUPPORTED, "alf_clip_rdo_luma: picture above 65535 samples");
+  for (int c = 0; c < 25; c++)
+    for (int k = 0; k < 12; k++)
+      if (coeff[c][k] < -32768 || coeff[c][k] > 32767)
+        return fail(VVCLF_ERR_UNSUPPORTED, "alf_clip_rdo_luma: luma coefficient %d outside 16 bits", coeff[c][k]);
+  const uint32_t ctb = 1u << ctb_log2, wctb = (W + ctb - 1) / ctb, hctb = (H + ctb - 1) / ctb;
+  const uint32_t n_ctbs = wctb * hctb;
+  std::vector<uint64_t> rects(n_ctbs);
+  for (uint32_t ry = 0; ry < hctb; ry++)
+    for (uint32_t rx = 0; rx < wctb; rx++) rects[(size_t)ry * wctb + rx] = ctb_fetch_rect(nullptr, rx, ry, ctb, W, H);
+  clip_tables tabs;
+  alf_luma_bank_tables(bd, tabs.perm, tabs.clipw);
+  vvclf_ctx* ctx = default_ctx();
+  if (!ctx) return VVCLF_ERR_HIP;
+  hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  const uint32_t pitch = (W + 63) / 64 * 64;
+  const size_t plane = (size_t)pitch * H * 2;
+  const size_t trial_bytes = CLASS_MAX_CAND * 25 * 8;
+  dbuf drec, dsrc, drect, dcoeff, dclip, dbank, dtrial, dcur, dbase;
+  hipError_t e = drec.alloc(plane, st);
+  if (e == hipSuccess) e = dsrc.alloc(plane, st);
+  if (e == hipSuccess) e = drect.alloc(rects.size() * 8, st);
+  if (e == hipSuccess) e = dcoeff.alloc(25 * 12 * 4, st);
+  if (e == hipSuccess) e = dclip.alloc(25 * 12, st);
+  if (e == hipSuccess) e = dbank.alloc((size_t)3 * ALF_BANK_WORDS * 4, st);
+  if (e == hipSuccess) e = dtrial.alloc(trial_bytes, st);
+  if (e == hipSuccess) e = dcur.alloc(25 * 8, st);
+  if (e == hipSuccess) e = dbase.alloc(25 * 8, st);
+  if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "alf_clip_rdo_luma: device buffers: %s", hipGetErrorString(e));
+  e = copy_rows(drec.p, (size_t)pitch * 2, R.samples, (size_t)R.stride * 2, (size_t)W * 2, H,
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = copy_rows(dsrc.p, (size_t)pitch * 2, S.samples, (size_t)S.stride * 2, (size_t)W * 2, H,
+                         hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(drect.p, rects.data(), rects.size() * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dcoeff.p, coeff, 25 * 12 * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(dclip.p, 0, 25 * 12, st);
+  if (e == hipSuccess) e = hipMemsetAsync(dtrial.p, 0, trial_bytes, st);
+  enc_args a;
+  memset(&a, 0, sizeof(a));
+  a.rec = (const uint16_t*)drec.p;
+  a.src = (const uint16_t*)dsrc.p;
+  a.pitch = (int)pitch;
+  a.w = (int)W;
+  a.h = (int)H;
+  a.ctb_log2 = (int)ctb_log2;
+  a.wctb = (int)wctb;
+  a.hctb = (int)hctb;
+  a.bd = (int)bd;
+  a.rect = (const uint64_t*)drect.p;
+  a.banks = (const uint32_t*)dbank.p;
+  a.sse = (unsigned long long*)dtrial.p;
+  const int tl = ctb_log2 < 6 ? (int)ctb_log2 : 6, tile = 1 << tl;
+  const dim3 grid((W + tile - 1) / tile, (H + tile - 1) / tile);
+  const unsigned threads = std::max(64u, (unsigned)((tile / 4) * (tile / 4)));
+  const bool hbd = bd > 12;
+  for (int j = -1; j < 12 && e == hipSuccess; j++) {
+    const int nb = j < 0 ? 1 : 3;
+    a.n_cand = nb;
+    hipLaunchKernelGGL(k_clip_banks, dim3((nb * ALF_BANK_WORDS + 255) / 256), dim3(256), 0, st,
+                       (const int32_t*)dcoeff.p, (const uint8_t*)dclip.p, j, tabs, (uint32_t*)dbank.p, nb);
+    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_CLASS, true>), grid, dim3(threads), 0, st, a, tl);
+    else hipLaunchKernelGGL((k_alf_enc<ENC_CLASS, false>), grid, dim3(threads), 0, st, a, tl);
+    hipLaunchKernelGGL(k_clip_pick, dim3(1), dim3(64), 0, st, (unsigned long long*)dtrial.p,
+                       (unsigned long long*)dcur.p, (unsigned long long*)dbase.p, (uint8_t*)dclip.p, j);
+    e = hipGetLastError();
+  }
+  uint8_t clip[25 * 12];
+  uint64_t cur[25], base[25];
+  if (e == hipSuccess) e = hipMemcpyAsync(clip, dclip.p, sizeof(clip), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(cur, dcur.p, sizeof(cur), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(base, dbase.p, sizeof(base), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "alf_clip_rdo_luma");
+  memset(out, 0, sizeof(*out));
+  memcpy(out->coeff, coeff, sizeof(out->coeff));
+  memcpy(out->clip_idx, clip, sizeof(clip));
+  for (int k = 0; k < 25; k++) {
+    out->baseline_sse_y += base[k];
+    out->post_clip_sse_y += cur[k];
+  }
+  for (int i = 0; i < 25 * 12; i++)
+    if (clip[i]) out->alf_luma_clip_flag = 1;
   return VVCLF_OK;
 }
 
@@ -872,13 +1065,13 @@ int chroma_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plan
   if (e == hipSuccess) e = dchoice.alloc((size_t)n_ctbs * 4, st);
   if (e == hipSuccess) e = dok.alloc(n, st);
   if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "%s: device buffers: %s", name, hipGetErrorString(e));
-  e = hipMemcpy2DAsync(drec.p, (size_t)cpitch * 2, RP.samples, (size_t)RP.stride * 2, (size_t)Wc * 2, Hc,
+  e = copy_rows(drec.p, (size_t)cpitch * 2, RP.samples, (size_t)RP.stride * 2, (size_t)Wc * 2, Hc,
                        hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
-    e = hipMemcpy2DAsync(dsrc.p, (size_t)cpitch * 2, SP.samples, (size_t)SP.stride * 2, (size_t)Wc * 2, Hc,
+    e = copy_rows(dsrc.p, (size_t)cpitch * 2, SP.samples, (size_t)SP.stride * 2, (size_t)Wc * 2, Hc,
                          hipMemcpyHostToDevice, st);
   if (e == hipSuccess && cc)
-    e = hipMemcpy2DAsync(dl.p, (size_t)lpitch * 2, pre_luma->samples, (size_t)pre_luma->stride * 2, (size_t)W * 2, H,
+    e = copy_rows(dl.p, (size_t)lpitch * 2, pre_luma->samples, (size_t)pre_luma->stride * 2, (size_t)W * 2, H,
                          hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(drect.p, rects.data(), rects.size() * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
@@ -925,7 +1118,7 @@ int chroma_decide(const vvclf_picture* src, vvclf_picture* rec, const vvclf_plan
   if (e == hipSuccess) e = hipMemcpyAsync(choice.data(), dchoice.p, (size_t)n_ctbs * 4, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess && sse_out) e = hipMemcpyAsync(sse.data(), dsse.p, sse.size() * 8, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess)
-    e = hipMemcpy2DAsync(RP.samples, (size_t)RP.stride * 2, dout.p, (size_t)cpitch * 2, (size_t)Wc * 2, Hc,
+    e = copy_rows(RP.samples, (size_t)RP.stride * 2, dout.p, (size_t)cpitch * 2, (size_t)Wc * 2, Hc,
                          hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, name);
@@ -1005,10 +1198,10 @@ extern "C" int vvclf_sao_decide_picture(const vvclf_picture* src, const vvclf_pi
     e = dr[c].alloc((size_t)pitch * ph * 2, st);
     if (e == hipSuccess) e = ds[c].alloc((size_t)pitch * ph * 2, st);
     if (e == hipSuccess && pw && ph)
-      e = hipMemcpy2DAsync(dr[c].p, (size_t)pitch * 2, rp[c]->samples, (size_t)rp[c]->stride * 2, (size_t)pw * 2, ph,
+      e = copy_rows(dr[c].p, (size_t)pitch * 2, rp[c]->samples, (size_t)rp[c]->stride * 2, (size_t)pw * 2, ph,
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess && pw && ph)
-      e = hipMemcpy2DAsync(ds[c].p, (size_t)pitch * 2, sp[c]->samples, (size_t)sp[c]->stride * 2, (size_t)pw * 2, ph,
+      e = copy_rows(ds[c].p, (size_t)pitch * 2, sp[c]->samples, (size_t)sp[c]->stride * 2, (size_t)pw * 2, ph,
                            hipMemcpyHostToDevice, st);
     a.rec[c] = (const uint16_t*)dr[c].p;
     a.src[c] = (const uint16_t*)ds[c].p;
@@ -1082,10 +1275,10 @@ extern "C" int vvclf_sao_chroma_merge(const vvclf_picture* src, const vvclf_pict
     e = dr[c].alloc((size_t)pitch * ch * 2, st);
     if (e == hipSuccess) e = ds[c].alloc((size_t)pitch * ch * 2, st);
     if (e == hipSuccess && cw && ch)
-      e = hipMemcpy2DAsync(dr[c].p, (size_t)pitch * 2, rp[c]->samples, (size_t)rp[c]->stride * 2, (size_t)cw * 2, ch,
+      e = copy_rows(dr[c].p, (size_t)pitch * 2, rp[c]->samples, (size_t)rp[c]->stride * 2, (size_t)cw * 2, ch,
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess && cw && ch)
-      e = hipMemcpy2DAsync(ds[c].p, (size_t)pitch * 2, sp[c]->samples, (size_t)sp[c]->stride * 2, (size_t)cw * 2, ch,
+      e = copy_rows(ds[c].p, (size_t)pitch * 2, sp[c]->samples, (size_t)sp[c]->stride * 2, (size_t)cw * 2, ch,
                            hipMemcpyHostToDevice, st);
     a.rec[c] = (const uint16_t*)dr[c].p;
     a.src[c] = (const uint16_t*)ds[c].p;

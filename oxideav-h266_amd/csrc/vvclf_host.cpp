@@ -540,6 +540,11 @@ void build_luma_bank(const int32_t coeff[25][12], const uint8_t clip[25][12], ui
 }
 }  // namespace
 
+void alf_luma_bank_tables(uint32_t bd, uint8_t perm[4][12], uint32_t clipw[4]) {
+  memcpy(perm, kTranspose, sizeof(kTranspose));
+  for (uint8_t i = 0; i < 4; i++) clipw[i] = clip_word(bd, i);
+}
+
 uint64_t ctb_fetch_rect(const vvclf_bounds* tb, uint32_t rx, uint32_t ry, uint32_t ctb, uint32_t W, uint32_t H) {
   return ctb_rect(tb, rx, ry, ctb, W, H);
 }

@@ -318,7 +318,7 @@ int vvclf_lmcs_forward_luma(vvclf_plane* luma, const vvclf_lmcs_binding* b, cons
   if (e == hipSuccess) e = dr.alloc((size_t)n_rects * sizeof(vvclf_rect), st);
   if (e == hipSuccess) e = dbad.alloc(4, st);
   if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "lmcs_forward_luma: device buffers: %s", hipGetErrorString(e));
-  e = hipMemcpy2DAsync(dy.p, (size_t)W * 2, luma->samples, (size_t)luma->stride * 2, (size_t)W * 2, H,
+  e = copy_rows(dy.p, (size_t)W * 2, luma->samples, (size_t)luma->stride * 2, (size_t)W * 2, H,
                        hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(dl.p, lut.data(), 65536 * 2, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(dr.p, rects, (size_t)n_rects * sizeof(vvclf_rect), hipMemcpyHostToDevice, st);
@@ -335,7 +335,7 @@ int vvclf_lmcs_forward_luma(vvclf_plane* luma, const vvclf_lmcs_binding* b, cons
   if (e != hipSuccess) return hip_fail(e, "lmcs_forward_luma");
   if (bad)
     return fail(VVCLF_ERR_INVALID, "lmcs_forward_luma: a sample of 2^%u or more lies in a rectangle", d.bit_depth);
-  e = hipMemcpy2DAsync(luma->samples, (size_t)luma->stride * 2, dy.p, (size_t)W * 2, (size_t)W * 2, H,
+  e = copy_rows(luma->samples, (size_t)luma->stride * 2, dy.p, (size_t)W * 2, (size_t)W * 2, H,
                        hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   return e == hipSuccess ? VVCLF_OK : hip_fail(e, "lmcs_forward_luma: download");
@@ -386,7 +386,7 @@ int vvclf_lmcs_chroma_residual(vvclf_picture* pic, const vvclf_lmcs_binding* b, 
   if (e == hipSuccess) e = dres.alloc((size_t)n_res * 4, st);
   if (e != hipSuccess) return fail(VVCLF_ERR_NOMEM, "lmcs_chroma_residual: device buffers: %s", hipGetErrorString(e));
   auto up = [&](void* dst, const vvclf_plane& p, uint32_t w, uint32_t h) {
-    return hipMemcpy2DAsync(dst, (size_t)w * 2, p.samples, (size_t)p.stride * 2, (size_t)w * 2, h,
+    return copy_rows(dst, (size_t)w * 2, p.samples, (size_t)p.stride * 2, (size_t)w * 2, h,
                             hipMemcpyHostToDevice, st);
   };
   e = up(dl.p, L, W, H);
@@ -400,7 +400,7 @@ int vvclf_lmcs_chroma_residual(vvclf_picture* pic, const vvclf_lmcs_binding* b, 
     e = hipGetLastError();
   }
   auto down = [&](const vvclf_plane& p, const void* src, uint32_t w, uint32_t h) {
-    return hipMemcpy2DAsync(p.samples, (size_t)p.stride * 2, src, (size_t)w * 2, (size_t)w * 2, h,
+    return copy_rows(p.samples, (size_t)p.stride * 2, src, (size_t)w * 2, (size_t)w * 2, h,
                             hipMemcpyDeviceToHost, st);
   };
   if (e == hipSuccess && Wc && Hc) e = down(Pb, dcb.p, Wc, Hc);

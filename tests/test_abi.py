@@ -124,6 +124,24 @@ def test_alf_decide_rejects_before_any_device_work(vv):
     assert decide(vv, fn, src, src, 6, 17)[0] == vv.ERR_INVALID
 
 
+def test_alf_clip_rdo_rejects_before_any_device_work(vv):
+    """vvclf_alf_clip_rdo_luma checks its arguments on the host:
+    coefficients beyond 16 bits and CTB sizes outside 8..256 are refused,
+    pictures of different sizes and bit depths outside 8..16 are invalid."""
+    from alf_clip_rdo_cases import search
+    from helpers import Pic
+    src = Pic(vv, 32, 32, 100)
+    fn = vv.lib.vvclf_alf_clip_rdo_luma
+    wide = np.zeros((25, 12), np.int64)
+    wide[3][4] = -40000
+    assert search(vv, fn, src, src, wide)[0] == vv.ERR_UNSUPPORTED
+    assert b"16 bits" in vv.lib.vvclf_last_error()
+    zero = np.zeros((25, 12), np.int64)
+    assert search(vv, fn, src, src, zero, ctb_log2=9)[0] == vv.ERR_UNSUPPORTED
+    assert search(vv, fn, src, src, zero, bd=7)[0] == vv.ERR_INVALID
+    assert search(vv, fn, src, Pic(vv, 32, 16, 100), zero)[0] == vv.ERR_INVALID
+
+
 def test_derive_lmcs_matches_oracle(vv, orc):
     rng = np.random.default_rng(0)
     for trial in range(300):

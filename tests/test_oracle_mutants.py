@@ -15,6 +15,7 @@ import tempfile
 
 import pytest
 
+import alf_clip_rdo_cases
 import binding
 import sao_decide_cases
 import test_oracle_pins as pins
@@ -115,6 +116,13 @@ MUTANTS = [
      "bits += c[k]->sao_type_idx == VVCLF_SAO_BAND ? 10 :", 0, ["merge:merge_case_rate_weight"]),
     ("sao-merge-above-bits", "cctb, bd, &a->cb, &a->cr) + 8 * 2;", "cctb, bd, &a->cb, &a->cr) + 8 * 1;", 0,
      ["merge:merge_case_above_costs_two_bits"]),
+    ("clip-rdo-strict", "        if (t < best) {", "        if (t <= best) {", 0,
+     ["clip:case_no_change_keeps_zero", "clip:case_equal_trials_keep_first_index"]),
+    ("clip-rdo-order", "for (uint8_t c = 1; c <= 3; c++) {", "for (uint8_t c = 3; c >= 1; c--) {", 0,
+     ["clip:case_equal_trials_keep_first_index"]),
+    ("clip-rdo-commit", "aps->luma_clip_idx[k][j] = best_clip;", "aps->luma_clip_idx[k][j] = 0;", 0,
+     ["clip:case_result_replays"]),
+    ("clip-rdo-running", "      running = best;", "      running = running;", 0, ["clip:case_result_replays"]),
 ]
 
 
@@ -165,6 +173,8 @@ def test_pins_catch_mutant(mutant, mutant_libs, vv):
         try:
             if pin.startswith("sao:"):
                 sao_decide_cases.ALL_CASES[pin[4:]](vv, lib.oracle_sao_decide_picture)
+            elif pin.startswith("clip:"):
+                alf_clip_rdo_cases.ALL_CASES[pin[5:]](vv, lib, lib.oracle_alf_clip_rdo_luma)
             elif pin.startswith("merge:"):
                 sao_decide_cases.ALL_MERGE_CASES[pin[6:]](vv, lib.oracle_sao_chroma_merge,
                                                           lib.oracle_sao_decide_picture)
