@@ -69,17 +69,17 @@ __device__ __forceinline__ int class_w(const int (&W)[10][10], int vb, bool carv
 }
 
 template <int MODE, bool HBD>
-__global__ void __launch_bounds__(256) k_alf_enc(enc_args a, int tile_log2) {
-  const int tile = 1 << tile_log2, tb = tile >> 2;
+__global__ void __launch_bounds__(256) k_alf_enc(enc_args a, int tw_log2, int th_log2) {
+  const int tb = 1 << (tw_log2 - 2), nby = 1 << (th_log2 - 2);
   const int bx = threadIdx.x % tb, by = threadIdx.x / tb;
-  const int tx0 = blockIdx.x * tile, ty0 = blockIdx.y * tile;
+  const int tx0 = blockIdx.x << tw_log2, ty0 = blockIdx.y << th_log2;
   const int ctb = 1 << a.ctb_log2;
   const int rx = tx0 >> a.ctb_log2, ry = ty0 >> a.ctb_log2;
   const size_t ci = (size_t)ry * a.wctb + rx;
   const int x4 = tx0 + bx * 4, y4 = ty0 + by * 4;
   // lanes past the tile or the picture take part in the wave reductions with
   // zero error and touch no memory
-  const bool live = by < tb && x4 < a.w && y4 < a.h;
+  const bool live = by < nby && x4 < a.w && y4 < a.h;
   const uint64_t r = a.rect[ci];
   const int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff) - 1,
             cy1 = (int)(r >> 48) - 1;
@@ -260,6 +260,21 @@ __global__ void __launch_bounds__(256) k_alf_pick(unsigned long long* sse, const
   }
   choice[i] = t;
 }
+
+// Tiles of k_alf_enc, inside one CTB: 128 x 32 for CTBs of 128 and more (a
+// wave holds two block rows, so only one wave per CTB column runs both
+// virtual-boundary variants), else 64 x 64 or the CTB
+struct alf_tiling {
+  int twl, thl;
+  dim3 grid;
+  unsigned threads;
+  alf_tiling(uint32_t W, uint32_t H, uint32_t ctb_log2) {
+    twl = ctb_log2 < 6 ? (int)ctb_log2 : (ctb_log2 >= 7 ? 7 : 6);
+    thl = ctb_log2 < 6 ? (int)ctb_log2 : (ctb_log2 >= 7 ? 5 : 6);
+    grid = dim3((W + (1u << twl) - 1) >> twl, (H + (1u << thl) - 1) >> thl);
+    threads = std::max(64u, (1u << (twl - 2)) << (thl - 2));
+  }
+};
 
 struct dbuf {
   void* p = nullptr;
@@ -792,17 +807,15 @@ extern "C" int vvclf_alf_decide_luma(const vvclf_picture* src, vvclf_picture* re
   a.ok = (const uint8_t*)dok.p;
   a.sse = (unsigned long long*)dsse.p;
   a.choice = (const int32_t*)dchoice.p;
-  const int tl = ctb_log2 < 6 ? (int)ctb_log2 : 6, tile = 1 << tl;
-  const dim3 grid((W + tile - 1) / tile, (H + tile - 1) / tile);
-  const unsigned threads = std::max(64u, (unsigned)((tile / 4) * (tile / 4)));
+  const alf_tiling tg(W, H, ctb_log2);
   const bool hbd = bd > 12;
   if (e == hipSuccess) {
-    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_TRIAL, true>), grid, dim3(threads), 0, st, a, tl);
-    else hipLaunchKernelGGL((k_alf_enc<ENC_TRIAL, false>), grid, dim3(threads), 0, st, a, tl);
+    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_TRIAL, true>), tg.grid, dim3(tg.threads), 0, st, a, tg.twl, tg.thl);
+    else hipLaunchKernelGGL((k_alf_enc<ENC_TRIAL, false>), tg.grid, dim3(tg.threads), 0, st, a, tg.twl, tg.thl);
     hipLaunchKernelGGL(k_alf_pick, dim3((n_ctbs + 255) / 256), dim3(256), 0, st, (unsigned long long*)dsse.p,
                        (const uint8_t*)dok.p, (int)n_ctbs, (int)n_cand, (int32_t*)dchoice.p);
-    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_COMMIT, true>), grid, dim3(threads), 0, st, a, tl);
-    else hipLaunchKernelGGL((k_alf_enc<ENC_COMMIT, false>), grid, dim3(threads), 0, st, a, tl);
+    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_COMMIT, true>), tg.grid, dim3(tg.threads), 0, st, a, tg.twl, tg.thl);
+    else hipLaunchKernelGGL((k_alf_enc<ENC_COMMIT, false>), tg.grid, dim3(tg.threads), 0, st, a, tg.twl, tg.thl);
     e = hipGetLastError();
   }
   std::vector<int32_t> choice(n_ctbs);
@@ -952,17 +965,15 @@ extern "C" int vvclf_alf_clip_rdo_luma(const vvclf_picture* src, const vvclf_pic
   a.rect = (const uint64_t*)drect.p;
   a.banks = (const uint32_t*)dbank.p;
   a.sse = (unsigned long long*)dtrial.p;
-  const int tl = ctb_log2 < 6 ? (int)ctb_log2 : 6, tile = 1 << tl;
-  const dim3 grid((W + tile - 1) / tile, (H + tile - 1) / tile);
-  const unsigned threads = std::max(64u, (unsigned)((tile / 4) * (tile / 4)));
+  const alf_tiling tg(W, H, ctb_log2);
   const bool hbd = bd > 12;
   for (int j = -1; j < 12 && e == hipSuccess; j++) {
     const int nb = j < 0 ? 1 : 3;
     a.n_cand = nb;
     hipLaunchKernelGGL(k_clip_banks, dim3((nb * ALF_BANK_WORDS + 255) / 256), dim3(256), 0, st,
                        (const int32_t*)dcoeff.p, (const uint8_t*)dclip.p, j, tabs, (uint32_t*)dbank.p, nb);
-    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_CLASS, true>), grid, dim3(threads), 0, st, a, tl);
-    else hipLaunchKernelGGL((k_alf_enc<ENC_CLASS, false>), grid, dim3(threads), 0, st, a, tl);
+    if (hbd) hipLaunchKernelGGL((k_alf_enc<ENC_CLASS, true>), tg.grid, dim3(tg.threads), 0, st, a, tg.twl, tg.thl);
+    else hipLaunchKernelGGL((k_alf_enc<ENC_CLASS, false>), tg.grid, dim3(tg.threads), 0, st, a, tg.twl, tg.thl);
     hipLaunchKernelGGL(k_clip_pick, dim3(1), dim3(64), 0, st, (unsigned long long*)dtrial.p,
                        (unsigned long long*)dcur.p, (unsigned long long*)dbase.p, (uint8_t*)dclip.p, j);
     e = hipGetLastError();

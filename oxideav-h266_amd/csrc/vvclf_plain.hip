@@ -312,15 +312,20 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
 // ---------------------------------------------------------------------------
 // ALF luma: one thread per 4x4 block.  The block's 10x10 neighbourhood
 // (-3..+6) feeds both the classification window and the 7x7 diamond.
-// A workgroup covers one CTB-aligned tile; each block reads its class's 12
-// tap words of the CTB's permuted filter bank straight from memory (cached),
-// so that no wave waits on a workgroup barrier.
+// A workgroup covers one tile inside a CTB, 2^tw_log2 x 2^th_log2 samples;
+// each block reads its class's 12 tap words of the CTB's permuted filter bank
+// straight from memory (cached), so that no wave waits on a workgroup
+// barrier.  The two block rows next to the virtual boundary (CtbSizeY - 8 and
+// - 4) take their own classification and filter variants: with 128-wide
+// tiles a wave holds two block rows, so that only the wave holding both of
+// those rows runs two variants (with 64 x 64 tiles, four rows per wave, the
+// wave holding them ran three: 37.8 -> 30.7 us per 4K picture).
 // ---------------------------------------------------------------------------
 template <bool HBD>
-__global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
-  const int tile = 1 << tile_log2, tb = tile >> 2;
+__global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tw_log2, int th_log2) {
+  const int tb = 1 << (tw_log2 - 2), nby = 1 << (th_log2 - 2);
   const int bx = threadIdx.x % tb, by = threadIdx.x / tb;
-  const int tx0 = blockIdx.x * tile, ty0 = blockIdx.y * tile;
+  const int tx0 = blockIdx.x << tw_log2, ty0 = blockIdx.y << th_log2;
   const int ctb = 1 << j.ctb_log2;
   const int rx = tx0 >> j.ctb_log2, ry = ty0 >> j.ctb_log2;
   plane_view src = pick(j.b, j, 0), dst = pick(j.out, j, 0);
@@ -331,7 +336,7 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
   int bank_idx = (int)(cw & 0xff);
   const int x4 = tx0 + bx * 4, y4 = ty0 + by * 4;
   if (bank_idx == (int)ALF_OFF8) {
-    if (by >= tb) return;
+    if (by >= nby) return;
     for (int r = 0; r < 4; r++) {
       int y = y4 + r;
       if (y >= src.h) break;
@@ -343,7 +348,7 @@ __global__ void __launch_bounds__(256) k_alf_luma(dev_job j, int tile_log2) {
     return;
   }
   const uint32_t* bank = j.alf_bank + (size_t)bank_idx * ALF_BANK_WORDS;
-  if (by >= tb) return;
+  if (by >= nby) return;
   if (x4 >= src.w || y4 >= src.h) return;
   uint64_t r = j.alf_rect[ci];
   const int cx0 = (int)(r & 0xffff), cy0 = (int)((r >> 16) & 0xffff), cx1 = (int)((r >> 32) & 0xffff) - 1,
@@ -615,15 +620,14 @@ hipError_t launch_plain(const dev_job& j, hipStream_t s, hipEvent_t* ev) {
   }
   mark(4);
   if (j.do_alf) {
-    int tl = j.ctb_log2 < 6 ? j.ctb_log2 : 6;
-    int tile = 1 << tl;
-    unsigned threads = (unsigned)((tile / 4) * (tile / 4));
+    const int twl = j.ctb_log2 < 6 ? j.ctb_log2 : (j.ctb_log2 >= 7 ? 7 : 6);
+    const int thl = j.ctb_log2 < 6 ? j.ctb_log2 : (j.ctb_log2 >= 7 ? 5 : 6);
+    const unsigned threads = (1u << (twl - 2)) << (thl - 2);
+    const dim3 grid(cdiv(j.w, 1u << twl), cdiv(j.h, 1u << thl));
     if (j.alf_bd > 12)
-      hipLaunchKernelGGL(k_alf_luma<true>, dim3(cdiv(j.w, tile), cdiv(j.h, tile)), dim3(threads < 64 ? 64 : threads), 0,
-                         s, j, tl);
+      hipLaunchKernelGGL(k_alf_luma<true>, grid, dim3(threads < 64 ? 64 : threads), 0, s, j, twl, thl);
     else
-      hipLaunchKernelGGL(k_alf_luma<false>, dim3(cdiv(j.w, tile), cdiv(j.h, tile)), dim3(threads < 64 ? 64 : threads),
-                         0, s, j, tl);
+      hipLaunchKernelGGL(k_alf_luma<false>, grid, dim3(threads < 64 ? 64 : threads), 0, s, j, twl, thl);
   }
   mark(5);
   if (j.do_alf && ncomp == 3) {
