@@ -300,6 +300,15 @@ def served_run(vv, ctx, ref, seconds, rate, n_streams=32, seed=7, max_jobs=None,
         streams.append((sid.value, frames))
         specs.extend(zip(frames, sp))
     n_jobs = int(seconds * rate) if max_jobs is None else max_jobs
+    picks = [rnd.randrange(n_streams) for _ in range(n_jobs)]
+    # A frame's last submission starts from its generated input, so that its
+    # final output is checked against the oracle.  At an offered rate every
+    # submission does; at saturation the earlier ones filter what the frame
+    # holds (the same work on the same side information), since restoring a
+    # 4K picture's planes on the one submitting thread would bound the rate.
+    last = {}
+    for i, k in enumerate(picks):
+        last[id(streams[k][1][i % 2])] = i
     busy, used = {}, set()
     t_next = time.perf_counter()
     for i in range(n_jobs):
@@ -307,12 +316,13 @@ def served_run(vv, ctx, ref, seconds, rate, n_streams=32, seed=7, max_jobs=None,
             t_next += rnd.expovariate(rate)
             while time.perf_counter() < t_next:
                 time.sleep(max(0.0, min(0.001, t_next - time.perf_counter())))
-        sid, frames = streams[rnd.randrange(n_streams)]
+        sid, frames = streams[picks[i]]
         f = frames[i % 2]
         prev = busy.get(id(f))
         if prev is not None:  # the frame's previous job must be done before reuse
             vv.lib.vvclf_server_wait(s, prev, 60000)
-        f.reset()
+        if rate or last[id(f)] == i:
+            f.reset()
         t = C.c_uint64()
         vv.check(vv.lib.vvclf_server_submit(s, sid, f.desc, C.byref(t)))
         busy[id(f)] = t.value

@@ -329,29 +329,44 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
     }
     t.has_chroma_dbk = true;
   }
-  // Every thread walks its share of the records through all passes.
-  std::vector<std::array<uint64_t, 4>> hists((size_t)nt, std::array<uint64_t, 4>{0, 0, 0, 0});
+  // Every thread walks its share of the records through all passes: a
+  // record is read once for every pass over the same record array (luma and,
+  // without a separate chroma tree, chroma), as the passes write separate
+  // maps.  Per-thread histograms sit on cache lines of their own.
+  struct alignas(64) thread_hist {
+    uint64_t v[4] = {0, 0, 0, 0};
+  };
+  std::vector<thread_hist> hists((size_t)nt);
   std::atomic<bool> bad{false};
   int32_t bad_xy[2] = {0, 0};
   std::mutex bad_mu;
   auto work = [&](int ti) {
     int32_t xy[2];
-    for (const pass& ps : passes) {
-      const uint32_t b = (uint32_t)((uint64_t)ps.n * ti / nt), e = (uint32_t)((uint64_t)ps.n * (ti + 1) / nt);
-      const uint32_t* nf = ps.vert ? f.no_filter_cols : f.no_filter_rows;
-      const uint32_t n_nf = ps.vert ? f.n_no_filter_cols : f.n_no_filter_rows;
+    uint64_t* hist = hists[ti].v;
+    size_t g0 = 0;
+    while (g0 < passes.size()) {  // groups of consecutive passes over one record array
+      size_t g1 = g0 + 1;
+      while (g1 < passes.size() && passes[g1].cus == passes[g0].cus && passes[g1].n == passes[g0].n) g1++;
+      const uint32_t n = passes[g0].n;
+      const uint32_t b = (uint32_t)((uint64_t)n * ti / nt), e = (uint32_t)((uint64_t)n * (ti + 1) / nt);
       for (uint32_t i = b; i < e; i++) {
-        if (!walk::walk_cu(ps.pp, ps.cus, ps.grid->view(), ps.mv, i, ps.vert, nf, n_nf, W, H, ps.m->view(),
-                           hists[ti].data(), xy)) {
-          std::lock_guard<std::mutex> g(bad_mu);
-          if (!bad.exchange(true)) {
-            bad_xy[0] = xy[0];
-            bad_xy[1] = xy[1];
+        for (size_t k = g0; k < g1; k++) {
+          const pass& ps = passes[k];
+          const uint32_t* nf = ps.vert ? f.no_filter_cols : f.no_filter_rows;
+          const uint32_t n_nf = ps.vert ? f.n_no_filter_cols : f.n_no_filter_rows;
+          if (!walk::walk_cu(ps.pp, ps.cus, ps.grid->view(), ps.mv, i, ps.vert, nf, n_nf, W, H, ps.m->view(), hist,
+                             xy)) {
+            std::lock_guard<std::mutex> g(bad_mu);
+            if (!bad.exchange(true)) {
+              bad_xy[0] = xy[0];
+              bad_xy[1] = xy[1];
+            }
+            return;
           }
-          return;
         }
         if (bad.load(std::memory_order_relaxed)) return;
       }
+      g0 = g1;
     }
   };
   run_threads(nt, work);
@@ -360,7 +375,7 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
                 bad_xy[1]);
   uint64_t hist[4] = {0, 0, 0, 0};
   for (const auto& h : hists)
-    for (int i = 0; i < 4; i++) hist[i] += h[i];
+    for (int i = 0; i < 4; i++) hist[i] += h.v[i];
   // finish the maps in row bands (nt per map, all bands on nt threads):
   // claim marks off, work lists, and the check that lets a direction pass
   // run concurrently in place
@@ -735,21 +750,26 @@ int derive_sao_alf(const vvclf_frame& f, uint32_t W, uint32_t H, host_tables& t,
 }
 
 // LMCS inverse map as a 65536-entry table: idx_y_inv + inverse_map_luma_sample
-// (src/lmcs.rs:209-238) evaluated for every u16 input.
+// (src/lmcs.rs:209-238) evaluated for every u16 input.  idx_y_inv returns
+// the first bin in min..=max whose upper pivot exceeds the sample (else
+// max + 1, capped at 15); LmcsPivot is non-decreasing, so that index never
+// falls as the sample grows, and one walk over the samples advances it in
+// place of a scan per sample.
 void build_lmcs_lut(const vvclf_lmcs_binding& b, std::vector<uint16_t>& lut) {
   const vvclf_lmcs_derived& d = *b.derived;
   lut.resize(65536);
-  int64_t hi = ((int64_t)1 << d.bit_depth) - 1;
-  for (uint32_t s = 0; s < 65536; s++) {
-    uint32_t idx = b.min_bin_idx;
-    while (idx <= b.max_bin_idx) {
-      if (s < d.lmcs_pivot[idx + 1]) break;
-      idx++;
+  const int64_t hi = ((int64_t)1 << d.bit_depth) - 1;
+  uint32_t scan = b.min_bin_idx;  // the scan's stopping bin (max + 1 when none)
+  for (uint32_t s = 0; s < 65536;) {
+    while (scan <= b.max_bin_idx && s >= d.lmcs_pivot[scan + 1]) scan++;
+    // samples up to the next pivot the scan would stop at share the bin
+    const uint32_t end = scan <= b.max_bin_idx ? std::min<uint32_t>(d.lmcs_pivot[scan + 1], 65536) : 65536;
+    const uint32_t idx = std::min(scan, 15u);
+    const int64_t isc = d.inv_scale_coeff[idx], piv = d.lmcs_pivot[idx], inp = d.input_pivot[idx];
+    for (; s < end; s++) {
+      const int64_t v = inp + ((isc * ((int64_t)s - piv) + (1 << 10)) >> 11);
+      lut[s] = (uint16_t)(v < 0 ? 0 : (v > hi ? hi : v));
     }
-    idx = std::min(idx, 15u);
-    int64_t prod = (int64_t)d.inv_scale_coeff[idx] * ((int64_t)s - (int64_t)d.lmcs_pivot[idx]) + (1 << 10);
-    int64_t v = (int64_t)d.input_pivot[idx] + (prod >> 11);
-    lut[s] = (uint16_t)std::clamp<int64_t>(v, 0, hi);
   }
 }
 

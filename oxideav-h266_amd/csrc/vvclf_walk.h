@@ -193,6 +193,14 @@ VVCLF_HD bool claim(uint32_t* slot) {
 // tile the picture, the precondition of src/deblock.rs:463-465).  Records of
 // one pass may be walked concurrently: each writes only the slots it claims.
 // hist (may be NULL): luma bS histogram [0..2] and the chroma segment count [3].
+//
+// Along one edge the P-side record changes only at record borders, and what
+// the descriptor takes from it — the bS arms that read the two records only
+// (BDPCM, intra / CIIP), the transform length across the edge (its
+// coordinate along the edge normal is fixed), hence both maxFilterLengths,
+// and the averaged QP — is the same for every segment of a run with one P
+// record: it is worked out once per run.  bS from coded TBs and motion is
+// evaluated per segment, as the reference does.
 VVCLF_HD bool walk_cu(const plane_params& pp, const vvclf_deblock_cu* cus, const grid_view& grid,
                       const vvclf_mv_grid* mv, uint32_t idx, bool vert, const uint32_t* nf, uint32_t n_nf,
                       uint32_t pic_w, uint32_t pic_h, const map_view& m, uint64_t* hist, int32_t* err_xy) {
@@ -215,13 +223,46 @@ VVCLF_HD bool walk_cu(const plane_params& pp, const vvclf_deblock_cu* cus, const
   uint32_t n8 = n_cb / 8;
   uint32_t last_sb = umin(umax(n8, 1u) - 1, nsb - 1);
   uint32_t ctb_h_c = pp.ctb / umax(pp.sub_h, 1u);
+  const uint32_t sub_axis = vert ? pp.sub_w : pp.sub_h;
+  const int32_t qp_q = comp == 0 ? cu.qp_y : chroma_qp(pp, cu);
+  // records whose coded-TB flag does not depend on the position
+  const bool q_tu_fixed = !cu.has_tu64 && !(cu.has_tb_split && (comp == 0 || !cu.tb_luma_only));
+  const vvclf_mv_cell zero_cell{};
+  uint32_t seg_hist[4] = {0, 0, 0, 0};
   for (uint32_t off = 0; off < n_cb; off += grid_step) {
     bool is_tb = off == 0 ? filter_edge : tb_boundary(cu, off, vert, comp != 0);
     bool is_sb = comp == 0 && off % sb == 0 && off / sb <= last_sb && (off > 0 || filter_edge);
     if (!(is_tb || is_sb)) continue;
+    // per edge: the Q side's transform length across the edge and, for a
+    // sub-block edge inside the record, the maxFilterLength cap
+    const int32_t qx_e = vert ? (int32_t)(cu.x + off) : (int32_t)cu.x;
+    const int32_t qy_e = vert ? (int32_t)cu.y : (int32_t)(cu.y + off);
+    const uint32_t wq = tb_len(cu, qx_e, qy_e, vert, comp != 0);
+    const bool ctb_row = comp == 0 && !vert && off == 0 && pp.ctb > 0 && cu.y % pp.ctb == 0;
+    const bool chroma_row = comp != 0 && !vert && off == 0 && ctb_h_c > 0 && (cu.y / pp.sub_h) % ctb_h_c == 0;
+    uint32_t sb_cap = 0;  // 0: none; else the inner sub-block edge's mfl pair
+    if (is_sb && off != 0 && !is_tb) {
+      auto tb_at = [&](int64_t z) -> bool {
+        if (z < 0 || (uint64_t)z >= n_cb) return false;
+        if (z == 0) return filter_edge;
+        return tb_boundary(cu, (uint32_t)z, vert, false);
+      };
+      if (tb_at((int64_t)off - 4) || tb_at((int64_t)off + 4)) sb_cap = 1;
+      else if (off / sb == 1 || off / sb == umax(n8, 1u) - 1 || tb_at((int64_t)off - sb) || tb_at((int64_t)off + sb))
+        sb_cap = 2;
+      else
+        sb_cap = 3;
+    }
+    // the run's P-side values
+    int32_t run_pi = -1;
+    const vvclf_deblock_cu* p = nullptr;
+    int bs_fixed = -1;  // 0 or 2 when the records alone decide bS
+    bool p_tu_fixed = false, p_tu = false, q_tu = false;
+    uint32_t mflp = 0, mflq = 0, plt_bits = 0;
+    int32_t qpv = 0;
     for (uint32_t k = 0; k < n_other; k += seg_step) {
-      int32_t qx = vert ? (int32_t)(cu.x + off) : (int32_t)(cu.x + k);
-      int32_t qy = vert ? (int32_t)(cu.y + k) : (int32_t)(cu.y + off);
+      int32_t qx = vert ? qx_e : (int32_t)(cu.x + k);
+      int32_t qy = vert ? (int32_t)(cu.y + k) : qy_e;
       int32_t px = vert ? qx - 1 : qx, py = vert ? qy : qy - 1;
       int32_t pi = off == 0 ? grid.at(px, py) : (int32_t)idx;
       if (pi < 0) continue;
@@ -244,71 +285,87 @@ VVCLF_HD bool walk_cu(const plane_params& pp, const vvclf_deblock_cu* cus, const
         err_xy[1] = qy;
         return false;
       }
-      const vvclf_deblock_cu& p = cus[pi];
-      int bs = boundary_strength(comp, p, cu, mv, px, py, qx, qy, is_tb, is_sb);
-      if (hist && comp == 0) hist[bs]++;
-      if (bs == 0) continue;
-      uint32_t mflp, mflq;
-      bool ctb_row = false;
-      int32_t qpv;
-      if (comp == 0) {
-        mflp = mflq = 0;
-        if (is_tb) {
-          uint32_t wq = tb_len(cu, qx, qy, vert, false), wp = tb_len(p, px, py, vert, false);
-          if (wp <= 4 || wq <= 4) {
-            mflp = mflq = 1;
-          } else {
-            mflp = wp >= 32 ? 7 : 3;
-            mflq = wq >= 32 ? 7 : 3;
-          }
-        }
-        if (is_sb) {
-          if (off == 0) {
-            if (nsb > 1) mflq = umin(mflq, 5u);
-            if (p.has_num_sb) mflp = umin(mflp, 5u);
-          } else {
-            auto tb_at = [&](int64_t z) -> bool {
-              if (z < 0 || (uint64_t)z >= n_cb) return false;
-              if (z == 0) return filter_edge;
-              return tb_boundary(cu, (uint32_t)z, vert, false);
-            };
-            if (is_tb) {
-              mflp = umin(mflp, 5u);
-              mflq = umin(mflq, 5u);
-            } else if (tb_at((int64_t)off - 4) || tb_at((int64_t)off + 4)) {
+      if (pi != run_pi) {  // a new P record along this edge
+        run_pi = pi;
+        p = &cus[pi];
+        // derive_bs, src/deblock.rs:1055-1119: the arms that read the records only
+        if (comp == 0 && p->bdpcm_luma && cu.bdpcm_luma) bs_fixed = 0;
+        else if (comp != 0 && p->bdpcm_chroma && cu.bdpcm_chroma) bs_fixed = 0;
+        else if (p->intra || cu.intra || p->ciip || cu.ciip) bs_fixed = 2;
+        else bs_fixed = -1;
+        p_tu_fixed = !p->has_tu64 && !(p->has_tb_split && (comp == 0 || !p->tb_luma_only));
+        p_tu = comp == 0 ? p->tu_y_coded : (comp == 1 ? p->tu_cb_coded : p->tu_cr_coded);
+        q_tu = comp == 0 ? cu.tu_y_coded : (comp == 1 ? cu.tu_cb_coded : cu.tu_cr_coded);
+        const uint32_t wp = tb_len(*p, px, py, vert, comp != 0);
+        if (comp == 0) {
+          mflp = mflq = 0;
+          if (is_tb) {
+            if (wp <= 4 || wq <= 4) {
               mflp = mflq = 1;
-            } else if (off / sb == 1 || off / sb == umax(n8, 1u) - 1 || tb_at((int64_t)off - sb) ||
-                       tb_at((int64_t)off + sb)) {
-              mflp = mflq = 2;
             } else {
-              mflp = mflq = 3;
+              mflp = wp >= 32 ? 7 : 3;
+              mflq = wq >= 32 ? 7 : 3;
             }
           }
-        }
-        ctb_row = !vert && off == 0 && pp.ctb > 0 && cu.y % pp.ctb == 0;
-        qpv = (p.qp_y + cu.qp_y + 1) >> 1;
-      } else {
-        uint32_t sub_axis = vert ? pp.sub_w : pp.sub_h;
-        uint32_t wq = tb_len(cu, qx, qy, vert, true) / umax(sub_axis, 1u);
-        uint32_t wp = tb_len(p, px, py, vert, true) / umax(sub_axis, 1u);
-        if (wp >= 8 && wq >= 8) {
-          bool row = !vert && off == 0 && ctb_h_c > 0 && (cu.y / pp.sub_h) % ctb_h_c == 0;
-          mflp = row ? 1 : 3;
-          mflq = 3;
+          if (is_sb) {
+            if (off == 0) {
+              if (nsb > 1) mflq = umin(mflq, 5u);
+              if (p->has_num_sb) mflp = umin(mflp, 5u);
+            } else if (is_tb) {
+              mflp = umin(mflp, 5u);
+              mflq = umin(mflq, 5u);
+            } else {
+              mflp = mflq = sb_cap;
+            }
+          }
+          qpv = (p->qp_y + qp_q + 1) >> 1;
         } else {
-          mflp = mflq = 1;
+          const uint32_t wq_c = wq / umax(sub_axis, 1u), wp_c = wp / umax(sub_axis, 1u);
+          if (wp_c >= 8 && wq_c >= 8) {
+            mflp = chroma_row ? 1 : 3;
+            mflq = 3;
+          } else {
+            mflp = mflq = 1;
+          }
+          qpv = (chroma_qp(pp, *p) + qp_q + 1) >> 1;
         }
-        if (mflp == 1 && mflq == 1 && bs != 2) continue;  // src/deblock.rs:1823, :1888
-        qpv = (chroma_qp(pp, p) + chroma_qp(pp, cu) + 1) >> 1;
+        plt_bits = (p->plt != 0 ? 1u : 0u) | (cu.plt != 0 ? 2u : 0u);
       }
+      int bs;
+      if (bs_fixed >= 0) {
+        bs = bs_fixed;
+      } else if (is_tb && ((p_tu_fixed ? p_tu : tu_coded(*p, comp, px, py)) ||
+                           (q_tu_fixed ? q_tu : tu_coded(cu, comp, qx, qy)))) {
+        bs = 1;
+      } else if (comp != 0 || !is_sb) {
+        bs = 0;
+      } else if ((p->ibc != 0) != (cu.ibc != 0)) {
+        bs = 1;
+      } else {
+        // motion: the cells covering the two sides, compared in place
+        const vvclf_mv_cell *a = &zero_cell, *b = &zero_cell;
+        if (mv) {
+          const uint32_t cw = mv->cells_w, ch = mv->cells_h;
+          if (px >= 0 && py >= 0 && (uint32_t)px / 4 < cw && (uint32_t)py / 4 < ch)
+            a = &mv->cells[(size_t)((uint32_t)py / 4) * cw + (uint32_t)px / 4];
+          if ((uint32_t)qx / 4 < cw && (uint32_t)qy / 4 < ch) b = &mv->cells[(size_t)((uint32_t)qy / 4) * cw + (uint32_t)qx / 4];
+        }
+        if (p->ibc && cu.ibc) bs = mv && mv_far(a->mv_l0, b->mv_l0) ? 1 : 0;
+        else bs = mv && motion_differs(*a, *b) ? 1 : 0;
+      }
+      if (comp == 0) seg_hist[bs]++;
+      if (bs == 0) continue;
+      if (comp != 0 && mflp == 1 && mflq == 1 && bs != 2) continue;  // src/deblock.rs:1823, :1888
       int32_t qb = iclamp(qpv + pp.beta_off * 2, 0, 63);
       int32_t qt = iclamp(qpv + 2 * (bs - 1) + pp.tc_off * 2, 0, 65);
       if (tc_is_zero(qt)) continue;  // tC == 0: no-op (src/deblock.rs:1429, :1815)
-      if (hist && comp != 0) hist[3]++;
-      m.e[slot] = SLOT_CLAIMED |
-                  pack_edge((uint32_t)bs, mflp, mflq, p.plt != 0, cu.plt != 0, ctb_row, (uint32_t)qb, (uint32_t)qt);
+      if (comp != 0) seg_hist[3]++;
+      m.e[slot] = SLOT_CLAIMED | pack_edge((uint32_t)bs, mflp, mflq, (plt_bits & 1) != 0, (plt_bits & 2) != 0,
+                                           ctb_row, (uint32_t)qb, (uint32_t)qt);
     }
   }
+  if (hist)
+    for (int i = 0; i < 4; i++) hist[i] += seg_hist[i];
   return true;
 }
 

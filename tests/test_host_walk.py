@@ -43,3 +43,24 @@ def test_worker_pool_runs_every_index_once(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "missed or repeated indices: 0" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or not os.path.isdir(os.path.join(ROCM, "include", "hip")),
+                    reason="needs g++ and the HIP headers")
+def test_lmcs_table_matches_oracle_per_sample(tmp_path):
+    """The host's LMCS inverse table, built by one walk over the samples,
+    equals the oracle's per-sample bin scan and inverse map for every u16
+    input (tests/lmcs_lut_check.cpp)."""
+    exe = tmp_path / "lmcs_lut_check"
+    orc = os.path.join(ROOT, "oracle")
+    obj = tmp_path / "vvclf_oracle.o"
+    subprocess.run(["gcc", "-O2", "-c", "-o", str(obj), os.path.join(orc, "vvclf_oracle.c")], check=True,
+                   capture_output=True, timeout=300)
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{CSRC}", f"-I{orc}",
+           os.path.join(ROOT, "tests", "lmcs_lut_check.cpp"), os.path.join(CSRC, "vvclf_host.cpp"), str(obj),
+           f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-lpthread", "-o", str(exe)]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "lmcs tables identical" in r.stdout
