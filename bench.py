@@ -277,7 +277,7 @@ def fused_roofline(bt, workload, px):
 SERVE_SIZES = [(832, 480), (1920, 1080), (3840, 2160)]
 
 
-def served_run(vv, ctx, ref, seconds, rate, n_streams=32, seed=7, max_jobs=None, verify=True):
+def served_run(vv, ctx, ref, seconds, rate, n_streams=32, seed=7, max_jobs=None, verify=True, prep_threads=4):
     """Poisson arrivals at `rate` pictures/s over `n_streams` streams (two
     pictures per stream, reused); each picture's final output is compared with
     the oracle's at the end."""
@@ -286,7 +286,7 @@ def served_run(vv, ctx, ref, seconds, rate, n_streams=32, seed=7, max_jobs=None,
     cfg.hbm_budget_bytes = 64 << 30
     cfg.max_inflight = 64
     cfg.max_batch = 16
-    cfg.prep_threads = 6
+    cfg.prep_threads = prep_threads
     s = C.c_void_p()
     vv.check(vv.lib.vvclf_server_create(ctx.h, C.byref(cfg), C.byref(s)))
     streams, specs = [], []
@@ -299,6 +299,14 @@ def served_run(vv, ctx, ref, seconds, rate, n_streams=32, seed=7, max_jobs=None,
         frames = [vv.SynthFrame(c, ww, hh, seed=sd, **kw) for c, ww, hh, sd, kw in sp]
         streams.append((sid.value, frames))
         specs.extend(zip(frames, sp))
+    # warm-up: every frame once, so that the first-use costs (a stream's
+    # resident LMCS / ALF tables, the prep threads' pinned staging buffers,
+    # the device memory pool) fall outside the statistics
+    for sid, frames in streams:
+        for f in frames:
+            vv.check(vv.lib.vvclf_server_submit(s, sid, f.desc, C.byref(C.c_uint64())))
+    vv.check(vv.lib.vvclf_server_drain(s, 120000))
+    vv.check(vv.lib.vvclf_server_reset_stats(s))
     n_jobs = int(seconds * rate) if max_jobs is None else max_jobs
     picks = [rnd.randrange(n_streams) for _ in range(n_jobs)]
     # A frame's last submission starts from its generated input, so that its
@@ -336,7 +344,8 @@ def served_run(vv, ctx, ref, seconds, rate, n_streams=32, seed=7, max_jobs=None,
         refs = ref.outputs([sp for _, sp in todo])
         for (f, sp), r in zip(todo, refs):
             same(r, f.output_arrays(), f"served picture {sp}")
-    return {"streams": n_streams, "offered_pictures_per_s": rate if rate else "saturation", "pictures": int(st.completed),
+    return {"streams": n_streams, "offered_pictures_per_s": rate if rate else "saturation",
+            "warmup_pictures": sum(len(fr) for _, fr in streams), "pictures": int(st.completed),
             "failed": int(st.failed), "launches": int(st.launches),
             "pictures_per_launch": round(st.completed / st.launches, 2) if st.launches else None,
             "throughput_Mpixel_s": round(st.luma_pixels / st.wall_s / 1e6, 2) if st.wall_s else None,

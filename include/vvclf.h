@@ -737,6 +737,11 @@ int vvclf_server_poll(vvclf_server* srv, uint64_t ticket);
 int vvclf_server_wait(vvclf_server* srv, uint64_t ticket, uint32_t timeout_ms);
 int vvclf_server_drain(vvclf_server* srv, uint32_t timeout_ms);
 int vvclf_server_get_stats(vvclf_server* srv, vvclf_server_stats* out);
+/* Starts the statistics afresh (counters, latency quantiles, wall clock and
+ * admitted-bytes peak), e.g. after a warm-up whose first-use costs (resident
+ * tables, staging buffers, pool growth) the steady state does not pay.  No
+ * reference counterpart (the reference has no server). */
+int vvclf_server_reset_stats(vvclf_server* srv);
 /* Test hook: make the next `n` jobs fail at admission with VVCLF_ERR_HIP, to
  * check that one failed job does not disturb the others. */
 int vvclf_server_inject_failures(vvclf_server* srv, uint32_t n);

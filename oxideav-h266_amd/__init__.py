@@ -305,6 +305,7 @@ SIGNATURES = {
     "vvclf_server_wait": (C.c_int, [VP, u64, u32]),
     "vvclf_server_drain": (C.c_int, [VP, u32]),
     "vvclf_server_get_stats": (C.c_int, [VP, P(ServerStats)]),
+    "vvclf_server_reset_stats": (C.c_int, [VP]),
     "vvclf_server_inject_failures": (C.c_int, [VP, u32]),
 }
 

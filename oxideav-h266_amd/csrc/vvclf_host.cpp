@@ -2,6 +2,7 @@
 // host-side derivation of the compact per-edge / per-CTB tables the kernels
 // read, job lifetime, and the synchronous drop-in entry points.
 #include <hip/hip_runtime.h>
+#include <sched.h>
 #include <sys/mman.h>
 
 #include <algorithm>
@@ -105,8 +106,23 @@ struct host_pool {
   }
 };
 
+// CPUs this process may run on (its affinity mask, which a container or a
+// scheduler share narrows below what the machine has).
+int host_cpus() {
+  static const int n = [] {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0 && CPU_COUNT(&set) > 0) return CPU_COUNT(&set);
+    return (int)std::max(1u, std::thread::hardware_concurrency());
+  }();
+  return n;
+}
+
+// Up to 15 workers: enough for the callers' parallel steps to proceed side by
+// side (several serving threads each walking a large picture on up to
+// host_walk_threads() threads) on a 16-CPU share.
 host_pool& the_pool() {
-  static host_pool* p = new host_pool(std::max(0, host_walk_threads() - 1));  // workers live with the process
+  static host_pool* p = new host_pool(std::max(0, std::min(host_cpus(), 16) - 1));  // workers live with the process
   return *p;
 }
 
@@ -141,22 +157,26 @@ struct cu_grid {
   walk::grid_view view() const { return walk::grid_view{cells.get(), w, h}; }
 };
 
-// One direction's descriptors of one plane (claim marks stripped by
-// finish_rows).  A fresh map holds zeros without having written them.
+// One direction's descriptors of one plane.  A fresh map holds zeros
+// without having written them; the walk writes only descriptors that filter
+// and records its claims in a bitmap of one bit per slot (a few hundred KB
+// that stay in cache, where the map is megabytes).
 struct edge_map {
   map_vec e;
+  std::vector<uint64_t> claims;
   uint32_t w = 0, h = 0;
   void init(uint32_t w_, uint32_t h_) {
     w = w_;
     h = h_;
     e = map_vec();
     e.resize((size_t)w * h);
+    claims.assign(((size_t)w * h + 63) / 64, 0);
   }
   walk::map_view view() { return walk::map_view{e.data(), w, h}; }
 };
 
-// Rows [y0, y1) of a map: strips the claim marks, lists the nonzero
-// descriptors as tag | row << 16 | column (row-major), and for a luma map
+// Rows [y0, y1) of a map: lists the nonzero descriptors as
+// tag | row << 16 | column (row-major), and for a luma map
 // checks that consecutive same-direction edges of a line cannot reach each
 // other's used samples (luma_dir 1: along a row, vertical edges; 2: down a
 // column, horizontal edges, whose first and last entry per column are kept
@@ -178,12 +198,14 @@ void finish_rows(uint32_t* e, uint32_t w, uint32_t tag, int luma_dir, finish_ban
     b.le.assign(w, 0);
   }
   for (uint32_t y = b.y0; y < b.y1; y++) {
-    uint32_t* row = e + (size_t)y * w;
+    const uint32_t* row = e + (size_t)y * w;
     int32_t px = -1;
     uint32_t pe = 0;
     for (uint32_t x = 0; x < w; x++) {
-      const uint32_t v = row[x] & ~walk::SLOT_CLAIMED;
-      row[x] = v;
+      // most slots hold no descriptor: skip four zero slots at a time
+      while (x + 4 <= w && (row[x] | row[x + 1] | row[x + 2] | row[x + 3]) == 0) x += 4;
+      if (x >= w) break;
+      const uint32_t v = row[x];
       if (!v) continue;
       b.list.push_back(tag | (y << 16) | x);
       if (luma_dir == 1) {
@@ -235,7 +257,7 @@ bool join_bands(std::vector<finish_band>& bands, uint32_t w, int luma_dir, std::
 }  // namespace
 
 int host_walk_threads() {
-  static const int n = (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+  static const int n = std::min(8, host_cpus());
   return n;
 }
 
@@ -290,8 +312,14 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
   edge_map lv, lh, cv[2], ch[2];
   t.gw = (W + 3) / 4 + 2;
   t.gh = (H + 3) / 4 + 2;
-  lv.init(t.gw, t.gh);
-  lh.init(t.gw, t.gh);
+  // the maps are allocated (zero pages populated by the kernel) together,
+  // one per thread, before the walk
+  struct map_dims {
+    edge_map* m;
+    uint32_t w, h;
+  };
+  map_dims dims[6] = {{&lv, t.gw, t.gh}, {&lh, t.gw, t.gh}};
+  int n_maps = 2;
   const walk::plane_params pl = {0, 1, 1, prm.luma_beta_offset_div2, prm.luma_tc_offset_div2, 0, prm.bit_depth, ctb};
   passes.push_back({pl, f.cus, f.n_cus, &grid, f.mv_grid, true, &lv});
   passes.push_back({pl, f.cus, f.n_cus, &grid, f.mv_grid, false, &lh});
@@ -322,12 +350,20 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
                                comp == 1 ? prm.chroma_qp_offset_cb : prm.chroma_qp_offset_cr,
                                prm.bit_depth,
                                ctb};
-      cv[comp - 1].init(t.cvw, t.cvh);
-      ch[comp - 1].init(t.chw, t.chh);
+      dims[n_maps++] = {&cv[comp - 1], t.cvw, t.cvh};
+      dims[n_maps++] = {&ch[comp - 1], t.chw, t.chh};
       passes.push_back({pc, ccus, nc, cg, nullptr, true, &cv[comp - 1]});
       passes.push_back({pc, ccus, nc, cg, nullptr, false, &ch[comp - 1]});
     }
     t.has_chroma_dbk = true;
+  }
+  if (nt > 1) {
+    std::atomic<int> next_map{0};
+    run_threads(std::min(nt, n_maps), [&](int) {
+      for (int k; (k = next_map.fetch_add(1)) < n_maps;) dims[k].m->init(dims[k].w, dims[k].h);
+    });
+  } else {
+    for (int k = 0; k < n_maps; k++) dims[k].m->init(dims[k].w, dims[k].h);
   }
   // Every thread walks its share of the records through all passes: a
   // record is read once for every pass over the same record array (luma and,
@@ -355,7 +391,7 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
           const uint32_t* nf = ps.vert ? f.no_filter_cols : f.no_filter_rows;
           const uint32_t n_nf = ps.vert ? f.n_no_filter_cols : f.n_no_filter_rows;
           if (!walk::walk_cu(ps.pp, ps.cus, ps.grid->view(), ps.mv, i, ps.vert, nf, n_nf, W, H, ps.m->view(), hist,
-                             xy)) {
+                             xy, ps.m->claims.data())) {
             std::lock_guard<std::mutex> g(bad_mu);
             if (!bad.exchange(true)) {
               bad_xy[0] = xy[0];
@@ -376,9 +412,9 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
   uint64_t hist[4] = {0, 0, 0, 0};
   for (const auto& h : hists)
     for (int i = 0; i < 4; i++) hist[i] += h.v[i];
-  // finish the maps in row bands (nt per map, all bands on nt threads):
-  // claim marks off, work lists, and the check that lets a direction pass
-  // run concurrently in place
+  // finish the maps in row bands (nt per map, all bands on nt threads): work
+  // lists, and the check that lets a direction pass run concurrently in
+  // place
   std::vector<uint32_t> lists[6];
   bool ok[6] = {true, true, true, true, true, true};
   {

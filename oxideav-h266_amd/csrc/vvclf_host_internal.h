@@ -91,8 +91,8 @@ void build_lmcs_lut(const vvclf_lmcs_binding& b, std::vector<uint16_t>& lut);
 // use more than one).
 int host_walk_threads();
 // Runs fn(0) .. fn(n - 1) on the calling thread and a process-wide pool of
-// host_walk_threads() - 1 workers (started on first use), returning when all
-// have finished.  Concurrent callers share the workers.
+// min(16, CPUs) - 1 workers (started on first use), returning when all have
+// finished.  Concurrent callers share the workers.
 void parallel_for(int n, const std::function<void(int)>& fn);
 void build_alf_banks(const vvclf_alf_config& ac, const vvclf_alf_aps_binding& b, host_tables& t);
 // Plane copies between a caller's host plane and a pitched device plane: one

@@ -755,6 +755,17 @@ int vvclf_server_get_stats(vvclf_server* s, vvclf_server_stats* o) {
   return VVCLF_OK;
 }
 
+int vvclf_server_reset_stats(vvclf_server* s) {
+  if (!s) return fail(VVCLF_ERR_INVALID, "reset_stats: NULL");
+  std::lock_guard<std::mutex> g(s->mu);
+  s->submitted = s->completed = s->failed = s->rejected = s->pixels = s->launches = 0;
+  s->lat = lat_hist();
+  for (auto& h : s->stage) h = lat_hist();
+  s->bytes_peak = s->bytes_admitted;
+  s->any_submit = false;  // the wall clock restarts at the next submission
+  return VVCLF_OK;
+}
+
 int vvclf_server_inject_failures(vvclf_server* s, uint32_t n) {
   if (!s) return fail(VVCLF_ERR_INVALID, "inject: NULL");
   std::lock_guard<std::mutex> g(s->mu);

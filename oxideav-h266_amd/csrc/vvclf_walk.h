@@ -191,7 +191,8 @@ VVCLF_HD bool claim(uint32_t* slot) {
 // one plane, writing descriptors into m.  Returns false, with the segment's
 // position in err_xy, when two records claim one segment (the records do not
 // tile the picture, the precondition of src/deblock.rs:463-465).  Records of
-// one pass may be walked concurrently: each writes only the slots it claims.
+// one pass may be walked concurrently: each writes only the slots it claims
+// (in the map, or in a bitmap of claims).
 // hist (may be NULL): luma bS histogram [0..2] and the chroma segment count [3].
 //
 // Along one edge the P-side record changes only at record borders, and what
@@ -203,8 +204,13 @@ VVCLF_HD bool claim(uint32_t* slot) {
 // evaluated per segment, as the reference does.
 VVCLF_HD bool walk_cu(const plane_params& pp, const vvclf_deblock_cu* cus, const grid_view& grid,
                       const vvclf_mv_grid* mv, uint32_t idx, bool vert, const uint32_t* nf, uint32_t n_nf,
-                      uint32_t pic_w, uint32_t pic_h, const map_view& m, uint64_t* hist, int32_t* err_xy) {
+                      uint32_t pic_w, uint32_t pic_h, const map_view& m, uint64_t* hist, int32_t* err_xy,
+                      uint64_t* claims = nullptr) {
   const vvclf_deblock_cu& cu = cus[idx];
+  // claims: NULL marks a claimed slot in the map itself (bit 31, stripped
+  // afterwards); else one bit per slot in this bitmap, and the map receives
+  // only the descriptors that filter
+  const uint32_t mark = claims ? 0u : SLOT_CLAIMED;
   const uint32_t comp = pp.comp;
   if (comp != 0) {
     uint32_t lead_c = vert ? cu.x / pp.sub_w : cu.y / pp.sub_h;
@@ -253,6 +259,26 @@ VVCLF_HD bool walk_cu(const plane_params& pp, const vvclf_deblock_cu* cus, const
       else
         sb_cap = 3;
     }
+#ifndef __HIP_DEVICE_COMPILE__
+    // The edge's descriptor slots (one map row apart along a vertical edge),
+    // and the P-side cells of the record grid and the motion grid, requested
+    // together before the segments are visited, so that their cache misses
+    // overlap instead of following one another.
+    if (vert) {
+      for (uint32_t k = 0; k < n_other; k += seg_step) {
+        const uint32_t y = cu.y + k;
+        const uint32_t sy = comp == 0 ? y / 4 : y / pp.sub_h / 2;
+        const uint32_t sx = comp == 0 ? (uint32_t)qx_e / 4 : (uint32_t)qx_e / pp.sub_w / 8;
+        if (sy < m.h && sx < m.w) __builtin_prefetch(&m.e[(size_t)sy * m.w + sx], 1);
+        if (off == 0 && qx_e > 0 && y / 4 < grid.h) __builtin_prefetch(&grid.cells[(size_t)(y / 4) * grid.w + (qx_e - 1) / 4]);
+        if (comp == 0 && mv && y / 4 < mv->cells_h && (uint32_t)qx_e / 4 < mv->cells_w) {
+          const vvclf_mv_cell* c = &mv->cells[(size_t)(y / 4) * mv->cells_w + (uint32_t)qx_e / 4];
+          __builtin_prefetch(c);
+          if (qx_e >= 4) __builtin_prefetch(c - 1);
+        }
+      }
+    }
+#endif
     // the run's P-side values
     int32_t run_pi = -1;
     const vvclf_deblock_cu* p = nullptr;
@@ -280,7 +306,8 @@ VVCLF_HD bool walk_cu(const plane_params& pp, const vvclf_deblock_cu* cus, const
       }
       if (sx >= m.w || sy >= m.h) continue;  // entirely outside the picture
       const size_t slot = (size_t)sy * m.w + sx;
-      if (!claim(&m.e[slot])) {
+      if (claims ? (__atomic_fetch_or(&claims[slot >> 6], 1ull << (slot & 63), __ATOMIC_RELAXED) >> (slot & 63)) & 1
+                 : !claim(&m.e[slot])) {
         err_xy[0] = qx;
         err_xy[1] = qy;
         return false;
@@ -360,7 +387,7 @@ VVCLF_HD bool walk_cu(const plane_params& pp, const vvclf_deblock_cu* cus, const
       int32_t qt = iclamp(qpv + 2 * (bs - 1) + pp.tc_off * 2, 0, 65);
       if (tc_is_zero(qt)) continue;  // tC == 0: no-op (src/deblock.rs:1429, :1815)
       if (comp != 0) seg_hist[3]++;
-      m.e[slot] = SLOT_CLAIMED | pack_edge((uint32_t)bs, mflp, mflq, (plt_bits & 1) != 0, (plt_bits & 2) != 0,
+      m.e[slot] = mark | pack_edge((uint32_t)bs, mflp, mflq, (plt_bits & 1) != 0, (plt_bits & 2) != 0,
                                            ctb_row, (uint32_t)qb, (uint32_t)qt);
     }
   }

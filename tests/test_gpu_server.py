@@ -171,3 +171,34 @@ def test_server_rejects_unknown_tile_shape(vv):
     assert vv.lib.vvclf_server_create(ctx.h, C.byref(cfg), C.byref(s)) == -1  # VVCLF_ERR_INVALID
     assert b"48x48" in vv.lib.vvclf_last_error()
     ctx.close()
+
+
+def test_reset_stats_starts_afresh(vv, orc):
+    """Statistics after a reset count only the jobs submitted after it, and
+    the results of those jobs still match the oracle."""
+    say("server reset stats")
+    ctx = vv.Context()
+    s = server(vv, ctx)
+    sid = C.c_uint32()
+    vv.check(vv.lib.vvclf_server_open_stream(s, C.byref(sid)))
+    for seed in (1, 2, 3):
+        f = vv.SynthFrame(1, 416, 240, seed=seed)
+        vv.check(vv.lib.vvclf_server_submit(s, sid.value, f.desc, C.byref(C.c_uint64())))
+        vv.check(vv.lib.vvclf_server_drain(s, 60000))
+        f.close()
+    assert stats(vv, s).completed == 3
+    vv.check(vv.lib.vvclf_server_reset_stats(s))
+    st = stats(vv, s)
+    assert st.completed == 0 and st.submitted == 0 and st.launches == 0 and st.p99_ms == 0.0
+    fg = vv.SynthFrame(0, 832, 480, seed=9)
+    fo = vv.SynthFrame(0, 832, 480, seed=9)
+    assert orc.oracle_apply_in_loop_filters(fo.desc) == 0
+    t = C.c_uint64()
+    vv.check(vv.lib.vvclf_server_submit(s, sid.value, fg.desc, C.byref(t)))
+    assert vv.lib.vvclf_server_wait(s, t.value, 60000) == 0, vv.lib.vvclf_last_error()
+    st = stats(vv, s)
+    assert st.completed == 1 and st.submitted == 1 and st.luma_pixels == 832 * 480 and st.p50_ms > 0.0
+    for a, b in zip(fo.output_arrays(), fg.output_arrays()):
+        assert np.array_equal(a, b)
+    vv.lib.vvclf_server_destroy(s)
+    ctx.close()
