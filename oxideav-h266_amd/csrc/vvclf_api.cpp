@@ -125,34 +125,42 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
     Hc = f->pic.cb.height;
   }
   host_tables t;
-  bool do_lmcs = (stages & STAGE_LMCS) && f->lmcs.derived;
-  if (do_lmcs) {
-    if (f->lmcs.derived->bit_depth < 8 || f->lmcs.derived->bit_depth > 16 || f->lmcs.max_bin_idx > 15)
-      return fail(VVCLF_ERR_INVALID, "lmcs: invalid derived tables");
-    build_lmcs_lut(f->lmcs, t.lmcs_lut);
-  }
-  bool do_dbk = (stages & STAGE_DBK) && !f->deblock.disabled;
+  const bool do_lmcs = (stages & STAGE_LMCS) && f->lmcs.derived;
+  const bool do_dbk = (stages & STAGE_DBK) && !f->deblock.disabled;
   const bool gpu_dbk = do_dbk && !dry_stats && opts && opts->gpu_derive;
+  if (do_lmcs &&
+      (f->lmcs.derived->bit_depth < 8 || f->lmcs.derived->bit_depth > 16 || f->lmcs.max_bin_idx > 15))
+    return fail(VVCLF_ERR_INVALID, "lmcs: invalid derived tables");
   if (do_dbk) {
     if (f->deblock.bit_depth < 8 || f->deblock.bit_depth > 16)
       return fail(VVCLF_ERR_INVALID, "deblock: bit_depth %u", f->deblock.bit_depth);
     if (f->n_cus && !f->cus) return fail(VVCLF_ERR_INVALID, "deblock: NULL cus");
-    if (gpu_dbk) {  // sizes only; the descriptors are derived on the GPU after the upload
-      t.gw = (W + 3) / 4 + 2;
-      t.gh = (H + 3) / 4 + 2;
-      t.has_chroma_dbk = f->chroma_format_idc != 0 && Wc > 0;
-      if (t.has_chroma_dbk) {
-        t.cvw = (Wc + 7) / 8 + 1;
-        t.cvh = (Hc + 1) / 2 + 1;
-        t.chw = (Wc + 1) / 2 + 1;
-        t.chh = (Hc + 7) / 8 + 1;
-      }
-    } else if ((rc = derive_deblock_maps(*f, W, H, Wc, Hc, t, opts ? opts->host_threads : 1))) {
-      return rc;
-    }
   }
-  if ((rc = derive_sao_alf(*f, W, H, t, stages))) return rc;
+  // The host tables: LMCS inverse table, deblocking descriptors, SAO / ALF
+  // parameters.  Returns a status; the message stays with the thread that
+  // ran it.
+  auto derive_tables = [&]() -> int {
+    int trc;
+    if (do_lmcs) build_lmcs_lut(f->lmcs, t.lmcs_lut);
+    if (do_dbk) {
+      if (gpu_dbk) {  // sizes only; the descriptors are derived on the GPU after the upload
+        t.gw = (W + 3) / 4 + 2;
+        t.gh = (H + 3) / 4 + 2;
+        t.has_chroma_dbk = f->chroma_format_idc != 0 && Wc > 0;
+        if (t.has_chroma_dbk) {
+          t.cvw = (Wc + 7) / 8 + 1;
+          t.cvh = (Hc + 1) / 2 + 1;
+          t.chw = (Wc + 1) / 2 + 1;
+          t.chh = (Hc + 7) / 8 + 1;
+        }
+      } else if ((trc = derive_deblock_maps(*f, W, H, Wc, Hc, t, opts ? opts->host_threads : 1))) {
+        return trc;
+      }
+    }
+    return derive_sao_alf(*f, W, H, t, stages);
+  };
   if (dry_stats) {
+    if ((rc = derive_tables())) return rc;
     for (int i = 0; i < 3; i++) dry_stats[i] = t.bs_hist[i];
     dry_stats[3] = t.chroma_filtered;
     uint64_t sao_on = 0, alf_on = 0;
@@ -190,21 +198,6 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   d.pc = Wc ? (int32_t)round_up(Wc, 64) : 0;
   d.bd = (int32_t)f->pic.luma.bit_depth;
   d.bd_dbk = (int32_t)f->deblock.bit_depth;
-  d.do_lmcs = do_lmcs;
-  d.do_dbk = do_dbk;
-  d.do_dbk_chroma = do_dbk && t.has_chroma_dbk;
-  d.do_sao = t.do_sao;
-  d.sao_luma = t.sao_luma;
-  d.sao_chroma = t.sao_chroma && Wc > 0;
-  d.sao_ctb_log2 = (int32_t)t.sao_ctb_log2;
-  d.swctb = (int32_t)t.swctb;
-  d.shctb = (int32_t)t.shctb;
-  d.sao_bd = (int32_t)t.sao_bd;
-  d.do_alf = t.do_alf;
-  d.ctb_log2 = (int32_t)t.alf_ctb_log2;
-  d.wctb = (int32_t)t.wctb;
-  d.hctb = (int32_t)t.hctb;
-  d.alf_bd = (int32_t)t.alf_bd;
 
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) {
@@ -244,6 +237,47 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
     sets[s]->cr = Wc ? (uint16_t*)(base + (luma_elems + chroma_elems) * 2) : nullptr;
   }
   if (arena) d.out = d.in;
+  // With a pinned staging buffer, the samples are staged and uploaded while
+  // the tables are derived: both are host work on the worker pool, and the
+  // upload touches only the input sample set.
+  const bool early_upload = opts && opts->stage;
+  if (early_upload) {
+    hipError_t ue = hipSuccess;
+    int trc = VVCLF_OK;
+    std::string terr;
+    hipStream_t us = opts->stream ? opts->stream : j->pool_stream;
+    parallel_for(2, [&](int task) {
+      if (task == 0) {
+        trc = derive_tables();
+        if (trc) terr = last_error();
+      } else {
+        ue = hipSetDevice(ctx->device);
+        if (ue == hipSuccess) ue = upload_staged(j, f->pic, *opts->stage, us);
+      }
+    });
+    if (trc || ue != hipSuccess) {
+      vvclf_job_destroy(j);
+      return trc ? fail(trc, "%s", terr.c_str()) : hip_fail(ue, "upload (staged)");
+    }
+  } else if ((rc = derive_tables())) {
+    vvclf_job_destroy(j);
+    return rc;
+  }
+  d.do_lmcs = do_lmcs;
+  d.do_dbk = do_dbk;
+  d.do_dbk_chroma = do_dbk && t.has_chroma_dbk;
+  d.do_sao = t.do_sao;
+  d.sao_luma = t.sao_luma;
+  d.sao_chroma = t.sao_chroma && Wc > 0;
+  d.sao_ctb_log2 = (int32_t)t.sao_ctb_log2;
+  d.swctb = (int32_t)t.swctb;
+  d.shctb = (int32_t)t.shctb;
+  d.sao_bd = (int32_t)t.sao_bd;
+  d.do_alf = t.do_alf;
+  d.ctb_log2 = (int32_t)t.alf_ctb_log2;
+  d.wctb = (int32_t)t.wctb;
+  d.hctb = (int32_t)t.hctb;
+  d.alf_bd = (int32_t)t.alf_bd;
   if (!t.do_sao) d.b = d.a;
   // side tables
   side_builder sb;
@@ -378,13 +412,7 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   j->h_alf_ctb = t.alf_ctb;
   j->h_alf_rect = t.alf_rect;
   if (keep) *keep = std::move(t);
-  if (opts && opts->stage) {
-    hipError_t ue = upload_staged(j, f->pic, *opts->stage, opts->stream ? opts->stream : j->pool_stream);
-    if (ue != hipSuccess) {
-      vvclf_job_destroy(j);
-      return hip_fail(ue, "upload (staged)");
-    }
-  } else if ((rc = vvclf_job_set_input(j, &f->pic))) {
+  if (!early_upload && (rc = vvclf_job_set_input(j, &f->pic))) {
     vvclf_job_destroy(j);
     return rc;
   }
