@@ -197,8 +197,11 @@ __global__ void __launch_bounds__(256) k_dbk(dev_job j, int luma_blocks) {
 // SAO: A -> B, a strip of 8 samples by SAO_ROWS rows per thread, inside one
 // CTB (SAO CTBs are at least 16 luma / 8 chroma samples wide and high; strips
 // are aligned to 8 samples and SAO_ROWS rows).  blockIdx.z = component.  The
-// strip's SAO_ROWS + 2 rows are read once, all loads issued together, and
-// kept in registers; rows whose CTB has every offset in i8 use the packed
+// strip's own row is read with the CTB parameters, the rows above and below
+// only by waves holding an edge class that reads them (vertical and diagonal
+// classes: band offset, off and the horizontal class need the sample's row
+// alone); rows are kept in registers; rows whose CTB has every offset in i8
+// use the packed
 // table form, the other rows (and edge-offset rows whose neighbourhoods leave
 // the picture or the CTB's rectangle) go sample by sample.
 // ---------------------------------------------------------------------------
@@ -232,11 +235,10 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
   const int ex = seg == 0 ? max(x0 - 1, 0) : min(x0 + 8, src.w - 1);
   uint4 q[SAO_ROWS + 2];
   uint32_t e[SAO_ROWS + 2];
-#pragma unroll
-  for (int r = 0; r < SAO_ROWS + 2; r++) {
-    const uint16_t* row = src.p + (size_t)clampi(ys - 1 + r, 0, src.h - 1) * src.pitch;
-    q[r] = *reinterpret_cast<const uint4*>(row + x0);
-    e[r] = row[ex];
+  {
+    const uint16_t* row = src.p + (size_t)ys * src.pitch;
+    q[1] = *reinterpret_cast<const uint4*>(row + x0);
+    e[1] = row[ex];
   }
   const int rx = x0 >> log2c, ry = ys >> log2c;
   uint64_t prm = 0;
@@ -244,6 +246,19 @@ __global__ void __launch_bounds__(256) k_sao(dev_job j) {
   if (active && rx < j.swctb && ry < j.shctb) {
     ci = (size_t)ry * j.swctb + rx;
     prm = j.sao[ci * 3 + comp];
+  }
+  // the rows above and below only where an edge class reads them (band
+  // offset, off and the horizontal class read the sample's own row)
+  if (__any((prm & 3) == 2 && ((prm >> 2) & 3) != 0)) {
+#pragma unroll
+    for (int r = 0; r < SAO_ROWS + 2; r += 2) {
+      const uint16_t* row = src.p + (size_t)clampi(ys - 1 + r, 0, src.h - 1) * src.pitch;
+      q[r] = *reinterpret_cast<const uint4*>(row + x0);
+      e[r] = row[ex];
+    }
+  } else {
+    q[0] = q[2] = q[1];
+    e[0] = e[2] = e[1];
   }
   // samples x0-1 and x0+8 of each row: from the neighbouring lanes (which
   // hold the adjacent chunks), at the wave's ends from e[]
