@@ -88,47 +88,60 @@ struct mrect {
 
 // Global -> register -> LDS staging of a map rectangle (at most MH x MW
 // words, LDS row pitch MW); issue() for every rectangle first, then
-// commit(), so that all loads of a tile are in flight together.  Element i
-// covers LDS row i / MW, column i % MW (compile-time divisor).
+// commit(), so that all loads of a tile are in flight together.  Lanes
+// follow the rectangle's rows: a wave covers 64 / LPR rows per step, LPR
+// lanes (a power of two >= MW) per row, so that a lane's column is its lane
+// index within the row and a step's row is uniform across those lanes; no
+// index needs a division, and each wave walks its own rows.
 template <int MW, int MH, int NT>
 struct rect_stage {
-  static constexpr int PER = (MW * MH + NT - 1) / NT;
+  static constexpr int LPR = MW <= 16 ? 16 : (MW <= 32 ? 32 : 64);  // lanes per row
+  static constexpr int RPI = 64 / LPR;                              // rows per wave and step
+  static constexpr int NW = NT / 64;                                // waves
+  static constexpr int PER = (MH + NW * RPI - 1) / (NW * RPI);      // steps
+  static_assert(MW <= 64 && NT % 64 == 0, "one row per LPR lanes");
   uint32_t v[PER];
+  // row of step k for the calling lane; the column is col()
+  __device__ __forceinline__ static int row(int k) {
+    const int t = ltid();
+    return (k * NW + (t >> 6)) * RPI + ((t & 63) / LPR);
+  }
+  __device__ __forceinline__ static int col() { return ltid() & (LPR - 1); }
   // map == nullptr: nothing to load (every element 0)
   __device__ __forceinline__ void issue(const uint32_t* map, int mw, const mrect& r) {
     // 64-bit base once; element offsets in 32 bits (rows < MH, map widths
     // below 2^24)
     const uint32_t* base = map ? map + (size_t)r.y0 * mw + r.x0 : nullptr;
+    const int xx = col();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = ltid() + k * NT, yy = i / MW, xx = i - __mul24(yy, MW);
+      const int yy = row(k);
       v[k] = (map && yy < r.ny && xx < r.nx) ? gld(base + (__mul24(yy, mw) + xx)) : 0u;
     }
   }
-  // also lists the nonzero words as tag | (row << 6) | column, reserving the
-  // wave's list entries with one LDS atomic.  Called by all lanes of the
-  // wave.
+  // Stores the nonzero words (the only ones the deblocking reads) and lists
+  // them as tag | (row << 6) | column, reserving the wave's list entries with
+  // one LDS atomic.  Called by all lanes of the wave.
   __device__ __forceinline__ void commit(uint32_t* dst, const mrect& r, uint16_t* list, uint32_t* cnt,
                                          uint32_t tag) const {
     uint64_t m[PER];
     uint32_t tot = 0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      const int i = ltid() + k * NT, yy = i / MW, xx = i - __mul24(yy, MW);
-      const bool in = yy < r.ny && xx < r.nx;
-      if (in) dst[i] = v[k];
-      m[k] = __ballot(in && v[k] != 0);
+      m[k] = __ballot(v[k] != 0);  // elements outside the rectangle hold 0
       tot += (uint32_t)__popcll(m[k]);
     }
     if (tot == 0) return;
     uint32_t base = 0;
     if ((ltid() & 63) == 0) base = atomicAdd(cnt, tot);
     base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    const int xx = col();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[k] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[k], 0u));
-      if ((m[k] >> (ltid() & 63)) & 1) {
-        const int i = ltid() + k * NT, yy = i / MW, xx = i - __mul24(yy, MW);
+      if (v[k] != 0) {
+        const int yy = row(k);
+        dst[yy * MW + xx] = v[k];
         list[base + below] = (uint16_t)(tag | (uint32_t)(yy << 6) | (uint32_t)xx);
       }
       base += (uint32_t)__popcll(m[k]);
