@@ -64,6 +64,28 @@ def test_overlapping_cus_are_rejected(vv):
     assert b"overlap" in vv.lib.vvclf_last_error()
 
 
+def test_overlapping_cus_are_rejected_by_the_threaded_walk(vv):
+    """A picture of more than 4096 records walks them on several threads,
+    claiming segments in a shared bitmap: a record away from the picture's
+    top and left borders repeated at the end of the list (its left edge's
+    segments claimed by its earlier copy, on another thread) is still
+    rejected, and the same list without the repeat is accepted."""
+    f = vv.SynthFrame(0, 1920, 1080, seed=3, sao=0, alf=0)
+    d = f.desc.contents
+    n = d.n_cus
+    assert n > 4096
+    interior = [i for i in range(n) if d.cus[i].x > 0 and d.cus[i].y > 0]
+    for victim in (interior[0], interior[len(interior) // 2], interior[-1]):
+        recs = [d.cus[i] for i in range(n)]
+        arr = (vv.DeblockCu * (n + 1))(*recs, recs[victim])
+        d.cus = C.cast(arr, C.POINTER(vv.DeblockCu))
+        d.n_cus = n + 1
+        assert vv.lib.vvclf_frame_check(f.desc, None) == vv.ERR_INVALID, victim
+        assert b"overlap" in vv.lib.vvclf_last_error()
+        d.n_cus = n  # the same records without the repeat
+        assert vv.lib.vvclf_frame_check(f.desc, None) == 0
+
+
 def test_unsupported_chroma_format(vv):
     f = vv.SynthFrame(0, 64, 64, seed=1)
     f.desc.contents.chroma_format_idc = 3
