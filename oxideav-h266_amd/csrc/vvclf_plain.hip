@@ -109,10 +109,24 @@ __device__ __forceinline__ void dbk_luma_lane(const dev_job& j, int t) {
   const int x0 = VERT ? 4 * gx : 4 * gx + line;  // q0 position of this line
   const int y0 = VERT ? 4 * gy + line : 4 * gy;
   int p[8], q[8];
+  if (VERT && x0 - 8 >= 0 && x0 + 8 <= v.w) {
+    // the line's 16 samples in four 8-byte reads (x0 - 8 is a multiple of 4;
+    // a row below the picture reads its last row, as the clamped reads do)
+    const uint16_t* rp = v.p + (size_t)clampi(y0, 0, v.h - 1) * v.pitch + (x0 - 8);
+    const uint2* r2 = reinterpret_cast<const uint2*>(rp);
+    const uint2 a0 = r2[0], a1 = r2[1], b0 = r2[2], b1 = r2[3];
+    const uint32_t aw[4] = {a0.x, a0.y, a1.x, a1.y}, bw[4] = {b0.x, b0.y, b1.x, b1.y};
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    p[i] = VERT ? rdc(v, x0 - 1 - i, y0) : rdc(v, x0, y0 - 1 - i);
-    q[i] = VERT ? rdc(v, x0 + i, y0) : rdc(v, x0, y0 + i);
+    for (int i = 0; i < 8; i++) {
+      p[i] = (int)((aw[(7 - i) >> 1] >> (16 * ((7 - i) & 1))) & 0xffffu);
+      q[i] = (int)((bw[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      p[i] = VERT ? rdc(v, x0 - 1 - i, y0) : rdc(v, x0, y0 - 1 - i);
+      q[i] = VERT ? rdc(v, x0 + i, y0) : rdc(v, x0, y0 + i);
+    }
   }
   int beta, tc;
   edge_thresholds(e, j.bd_dbk, kTc, beta, tc);
@@ -160,8 +174,17 @@ __device__ __forceinline__ void dbk_chroma_lane(const dev_job& j, int t) {
   int x0, y0;  // q0 of this line
   if (VERT) { x0 = 8 * sx; y0 = 2 * sy + line; } else { x0 = 2 * sx + line; y0 = 8 * sy; }
   int c[8];
+  if (VERT && x0 - 4 >= 0 && x0 + 4 <= v.w) {
+    // the line's 8 samples in two 8-byte reads (x0 - 4 is a multiple of 4)
+    const uint2* r2 = reinterpret_cast<const uint2*>(v.p + (size_t)clampi(y0, 0, v.h - 1) * v.pitch + (x0 - 4));
+    const uint2 a0 = r2[0], a1 = r2[1];
+    const uint32_t aw[4] = {a0.x, a0.y, a1.x, a1.y};
 #pragma unroll
-  for (int i = 0; i < 8; i++) c[i] = VERT ? rdc(v, x0 - 4 + i, y0) : rdc(v, x0, y0 - 4 + i);
+    for (int i = 0; i < 8; i++) c[i] = (int)((aw[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; i++) c[i] = VERT ? rdc(v, x0 - 4 + i, y0) : rdc(v, x0, y0 - 4 + i);
+  }
   int beta, tc;
   edge_thresholds(e, j.bd_dbk, kTc, beta, tc);
   int mflp = (e >> E_MFLP_SHIFT) & 7, mflq = (e >> E_MFLQ_SHIFT) & 7;
