@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -28,6 +29,11 @@ int check_plane(const vvclf_plane& p, const char* name) {
   return VVCLF_OK;
 }
 
+// Bands of a staged copy of `bytes`: one per copy thread from 16 MB on (a
+// band's copy is its thread's share of the host work), else one DMA (whose
+// row copies still split over the copy threads from 4 MB on).
+int copy_bands(size_t bytes) { return bytes >= ((size_t)16 << 20) ? host_walk_threads() : 1; }
+
 // The side tables of a job, laid out at 256-byte offsets and assembled in
 // one copy pass: into a pinned buffer when one is given (no page faults on a
 // fresh heap block, and the upload is a direct DMA), else on the heap.  The
@@ -50,6 +56,23 @@ struct side_builder {
   size_t add(const std::vector<T, A>& v) {
     return add(v.data(), v.size() * sizeof(T));
   }
+  // bytes [lo, hi) of the blob: the tables, and zeros in the gaps and in the
+  // 4 bytes an empty table reserves
+  void fill(uint8_t* dst, size_t lo, size_t hi) const {
+    size_t at = 0;
+    auto zero = [&](size_t a, size_t b) {
+      a = std::max(a, lo);
+      b = std::min(b, hi);
+      if (a < b) memset(dst + a, 0, b - a);
+    };
+    for (const part& q : parts) {
+      zero(at, q.off);
+      const size_t a = std::max(q.off, lo), b = std::min(q.off + q.n, hi);
+      if (a < b) memcpy(dst + a, (const uint8_t*)q.p + (a - q.off), b - a);
+      at = q.off + q.n;
+    }
+    zero(at, size);
+  }
   const uint8_t* materialize(pinned_buf* st) {
     uint8_t* dst;
     if (st) {
@@ -59,15 +82,32 @@ struct side_builder {
       blob.resize(size);
       dst = blob.data();
     }
-    // tables, and zeros in the gaps and in the 4 bytes an empty table reserves
-    size_t at = 0;
-    for (const part& q : parts) {
-      if (q.off > at) memset(dst + at, 0, q.off - at);
-      if (q.n) memcpy(dst + q.off, q.p, q.n);
-      at = q.off + q.n;
-    }
-    if (size > at) memset(dst + at, 0, size - at);
+    fill(dst, 0, size);
     return dst;
+  }
+  // Assembly in the pinned buffer and upload to `dev` on `s`, in bands of
+  // 256-byte multiples on the worker pool for blobs of 4 MB and more: each
+  // band's DMA is queued as soon as its bytes are in place.  Returns once the
+  // stream has drained.
+  hipError_t upload(pinned_buf& st, uint8_t* dev, hipStream_t s) const {
+    if (!st.ensure(size)) return hipErrorOutOfMemory;
+    uint8_t* dst = (uint8_t*)st.p;
+    const int nb = size >= ((size_t)4 << 20) ? host_walk_threads() : 1;
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    std::atomic<int> err{(int)hipSuccess};
+    parallel_for(nb, [&](int b) {
+      const size_t lo = size * b / nb / 256 * 256, hi = b + 1 == nb ? size : size * (b + 1) / nb / 256 * 256;
+      if (lo >= hi) return;
+      fill(dst, lo, hi);
+      hipError_t be = hipSetDevice(device);
+      if (be == hipSuccess) be = hipMemcpyAsync(dev + lo, dst + lo, hi - lo, hipMemcpyHostToDevice, s);
+      int ok = (int)hipSuccess;
+      if (be != hipSuccess) err.compare_exchange_strong(ok, (int)be);
+    });
+    e = hipStreamSynchronize(s);
+    return err.load() != (int)hipSuccess ? (hipError_t)err.load() : e;
   }
 };
 
@@ -294,11 +334,6 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
   size_t o_actb = sb.add(t.alf_ctb), o_arect = sb.add(t.alf_rect), o_bank = sb.add(t.alf_bank);
   size_t o_cbank = sb.add(t.alf_cbank), o_ccbank = sb.add(t.alf_ccbank);
   const size_t host_side = sb.size;
-  const uint8_t* host_blob = sb.materialize(opts ? opts->side_stage : nullptr);
-  if (!host_blob) {
-    vvclf_job_destroy(j);
-    return fail(VVCLF_ERR_NOMEM, "pinned staging of the side tables (%zu bytes)", host_side);
-  }
   // GPU derivation: maps and lists live after the uploaded tables
   size_t tail = (host_side + 255) / 256 * 256;
   auto reserve = [&](size_t n) {
@@ -342,14 +377,24 @@ int job_create_stages(vvclf_ctx* ctx, const vvclf_frame* f, uint32_t stages, vvc
       return fail(VVCLF_ERR_NOMEM, "hipMalloc side tables: %s", hipGetErrorString(e));
     }
   }
-  e = ps ? hipMemcpyAsync(j->dev_side, host_blob, host_side, hipMemcpyHostToDevice, ps)
-         : hipMemcpy(j->dev_side, host_blob, host_side, hipMemcpyHostToDevice);
-  if (e == hipSuccess && gpu_dbk)
-    e = hipMemsetAsync((uint8_t*)j->dev_side + maps_begin, 0, maps_end - maps_begin, ps);
-  if (e == hipSuccess && ps) e = hipStreamSynchronize(ps);
+  if (ps && opts->side_stage && !gpu_dbk) {
+    e = sb.upload(*opts->side_stage, (uint8_t*)j->dev_side, ps);
+  } else {
+    const uint8_t* host_blob = sb.materialize(opts ? opts->side_stage : nullptr);
+    if (!host_blob) {
+      vvclf_job_destroy(j);
+      return fail(VVCLF_ERR_NOMEM, "pinned staging of the side tables (%zu bytes)", host_side);
+    }
+    e = ps ? hipMemcpyAsync(j->dev_side, host_blob, host_side, hipMemcpyHostToDevice, ps)
+           : hipMemcpy(j->dev_side, host_blob, host_side, hipMemcpyHostToDevice);
+    if (e == hipSuccess && gpu_dbk)
+      e = hipMemsetAsync((uint8_t*)j->dev_side + maps_begin, 0, maps_end - maps_begin, ps);
+    if (e == hipSuccess && ps) e = hipStreamSynchronize(ps);
+  }
   if (e != hipSuccess) {
     vvclf_job_destroy(j);
-    return hip_fail(e, "upload side tables");
+    return e == hipErrorOutOfMemory ? fail(VVCLF_ERR_NOMEM, "pinned staging of the side tables (%zu bytes)", host_side)
+                                    : hip_fail(e, "upload side tables");
   }
   uint8_t* sd = (uint8_t*)j->dev_side;
   d.lmcs_lut = do_lmcs ? (res ? res->lut : (const uint16_t*)(sd + o_lut)) : nullptr;
@@ -451,6 +496,16 @@ struct plane_copy {
   const uint8_t* src;
   size_t spitch, rows, bytes;
 };
+// rows b..e of the concatenated planes
+void copy_rows(const plane_copy* pc, int n, size_t b, size_t e) {
+  size_t base = 0;
+  for (int i = 0; i < n; i++) {
+    const size_t lo = std::max(b, base), hi = std::min(e, base + pc[i].rows);
+    for (size_t r = lo; r < hi; r++)
+      memcpy(pc[i].dst + (r - base) * pc[i].dpitch, pc[i].src + (r - base) * pc[i].spitch, pc[i].bytes);
+    base += pc[i].rows;
+  }
+}
 void copy_planes(const plane_copy* pc, int n) {
   size_t total = 0, rows = 0;
   for (int i = 0; i < n; i++) {
@@ -458,19 +513,23 @@ void copy_planes(const plane_copy* pc, int n) {
     rows += pc[i].rows;
   }
   const int nt = total >= ((size_t)4 << 20) ? host_walk_threads() : 1;
-  auto part = [&](int ti) {
-    // rows b..e of the concatenated planes
-    const size_t b = rows * ti / nt, e = rows * (ti + 1) / nt;
-    size_t base = 0;
-    for (int i = 0; i < n; i++) {
-      const size_t lo = std::max(b, base), hi = std::min(e, base + pc[i].rows);
-      for (size_t r = lo; r < hi; r++)
-        memcpy(pc[i].dst + (r - base) * pc[i].dpitch, pc[i].src + (r - base) * pc[i].spitch, pc[i].bytes);
-      base += pc[i].rows;
-    }
-  };
-  parallel_for(nt, part);
+  parallel_for(nt, [&](int ti) { copy_rows(pc, n, rows * ti / nt, rows * (ti + 1) / nt); });
 }
+
+// The staged sample set in bands of rows of the concatenated planes (luma,
+// then Cb and Cr at one pitch, contiguous in the pinned buffer and on the
+// device; copy_bands of them), so that the row copies of one band overlap
+// the DMA of another.
+struct bands {
+  size_t rows, py, pc, h;
+  int n;
+  explicit bands(const vvclf_job* j)
+      : rows((size_t)j->h + (j->wc ? 2 * (size_t)j->hc : 0)), py(j->d.py), pc(j->d.pc), h(j->h),
+        n(copy_bands(set_samples(j) * 2)) {}
+  size_t row_lo(int b) const { return rows * b / n; }
+  // offset in samples of row r of the concatenated planes
+  size_t off(size_t r) const { return r <= h ? r * py : py * h + (r - h) * pc; }
+};
 }  // namespace
 
 hipError_t upload_staged(vvclf_job* j, const vvclf_picture& pic, pinned_buf& st, hipStream_t s) {
@@ -489,16 +548,77 @@ hipError_t upload_staged(vvclf_job* j, const vvclf_picture& pic, pinned_buf& st,
     pc[np] = {(uint8_t*)(hc + (size_t)c * j->d.pc * j->hc), (size_t)j->d.pc * 2, (const uint8_t*)pl.samples,
               (size_t)pl.stride * 2, j->hc, (size_t)j->wc * 2};
   }
-  copy_planes(pc, np);
-  hipError_t e = hipMemcpyAsync(j->d.in.y, h, n * 2, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  return e;
+  // each band goes to the device as soon as its rows are in the pinned buffer
+  const bands bd(j);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::atomic<int> err{(int)hipSuccess};
+  parallel_for(bd.n, [&](int b) {
+    if (bd.n == 1) copy_planes(pc, np);
+    else copy_rows(pc, np, bd.row_lo(b), bd.row_lo(b + 1));
+    const size_t lo = bd.off(bd.row_lo(b)), hi = bd.off(bd.row_lo(b + 1));
+    hipError_t be = hipSetDevice(dev);
+    if (be == hipSuccess && hi > lo) be = hipMemcpyAsync(j->d.in.y + lo, h + lo, (hi - lo) * 2, hipMemcpyHostToDevice, s);
+    int ok = (int)hipSuccess;
+    if (be != hipSuccess) err.compare_exchange_strong(ok, (int)be);
+  });
+  // the stream drains what was queued also after a failed band
+  e = hipStreamSynchronize(s);
+  return err.load() != (int)hipSuccess ? (hipError_t)err.load() : e;
 }
 
 hipError_t download_staged_issue(vvclf_job* j, pinned_buf& st, hipStream_t s) {
   const size_t n = set_samples(j);
   if (!st.ensure(n * 2)) return hipErrorOutOfMemory;
   return hipMemcpyAsync(st.p, job_final(j, 0), n * 2, hipMemcpyDeviceToHost, s);
+}
+
+// Bands as in upload_staged: band b's DMA is followed by an event, and the
+// thread that takes band b copies its rows out once the event has completed.
+// The events belong to the calling thread (created on first use, held until
+// the process ends).
+hipError_t download_staged(vvclf_job* j, pinned_buf& st, hipStream_t s, vvclf_picture& pic) {
+  const size_t n = set_samples(j);
+  if (!st.ensure(n * 2)) return hipErrorOutOfMemory;
+  const bands bd(j);
+  static thread_local hipEvent_t ev[16] = {};
+  if (bd.n > 16) return hipErrorInvalidValue;
+  const uint16_t* d = job_final(j, 0);
+  uint16_t* h = (uint16_t*)st.p;
+  hipError_t e = hipSuccess;
+  int issued = 0;
+  for (; issued < bd.n && e == hipSuccess; issued++) {
+    if (!ev[issued]) e = hipEventCreateWithFlags(&ev[issued], hipEventDisableTiming);
+    const size_t lo = bd.off(bd.row_lo(issued)), hi = bd.off(bd.row_lo(issued + 1));
+    if (e == hipSuccess && hi > lo) e = hipMemcpyAsync(h + lo, d + lo, (hi - lo) * 2, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[issued], s);
+  }
+  if (e != hipSuccess) {
+    hipStreamSynchronize(s);
+    return e;
+  }
+  const uint16_t* hc = h + (size_t)j->d.py * j->h;
+  plane_copy pc[3] = {{(uint8_t*)pic.luma.samples, (size_t)pic.luma.stride * 2, (const uint8_t*)h, (size_t)j->d.py * 2,
+                       j->h, (size_t)j->w * 2}};
+  int np = 1;
+  for (int c = 0; c < 2 && j->wc; c++, np++) {
+    vvclf_plane& pl = c ? pic.cr : pic.cb;
+    pc[np] = {(uint8_t*)pl.samples, (size_t)pl.stride * 2, (const uint8_t*)(hc + (size_t)c * j->d.pc * j->hc),
+              (size_t)j->d.pc * 2, j->hc, (size_t)j->wc * 2};
+  }
+  std::atomic<int> err{(int)hipSuccess};
+  // the calling thread's events (a thread_local named inside the lambda would
+  // be each worker's own)
+  hipEvent_t* const evs = ev;
+  parallel_for(bd.n, [&](int b) {
+    const hipError_t be = hipEventSynchronize(evs[b]);
+    int ok = (int)hipSuccess;
+    if (be != hipSuccess) err.compare_exchange_strong(ok, (int)be);
+    else if (bd.n == 1) copy_planes(pc, np);
+    else copy_rows(pc, np, bd.row_lo(b), bd.row_lo(b + 1));
+  });
+  return (hipError_t)err.load();
 }
 
 void download_staged_finish(const vvclf_job* j, const pinned_buf& st, vvclf_picture& pic) {
@@ -899,10 +1019,8 @@ static int run_sync(vvclf_frame* f, uint32_t stages) {
   }
   rc = vvclf_job_run(j, ctx->stream);
   if (!rc) {
-    hipError_t e = download_staged_issue(j, stage, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess) download_staged_finish(j, stage, f->pic);
-    else rc = hip_fail(e, "download");
+    const hipError_t e = download_staged(j, stage, ctx->stream, f->pic);
+    if (e != hipSuccess) rc = hip_fail(e, "download");
   }
   vvclf_job_destroy(j);
   return rc;

@@ -55,16 +55,71 @@ namespace {
 constexpr size_t kMapThreshold = (size_t)1 << 20;
 }  // namespace
 
-void* zeroed_alloc_bytes(size_t bytes) {
+namespace {
+struct block_pool {
+  std::mutex mu;
+  std::vector<std::pair<void*, size_t>> free_blocks;  // (address, mapped size)
+  std::vector<std::pair<void*, size_t>> out;          // blocks handed out larger than asked
+  size_t held = 0;
+};
+constexpr size_t kPoolCap = (size_t)512 << 20;
+block_pool& blocks() {
+  static block_pool* p = new block_pool();  // lives with the process
+  return *p;
+}
+}  // namespace
+
+thread_local bool t_recycled = false;
+bool pooled_alloc_recycled() { return t_recycled; }
+
+void* pooled_alloc_bytes(size_t bytes) {
+  t_recycled = false;
   if (bytes < kMapThreshold) return std::calloc(bytes ? bytes : 1, 1);
+  block_pool& bp = blocks();
+  {
+    std::lock_guard<std::mutex> g(bp.mu);
+    size_t best = bp.free_blocks.size();
+    for (size_t i = 0; i < bp.free_blocks.size(); i++) {
+      const size_t sz = bp.free_blocks[i].second;
+      if (sz >= bytes && sz - bytes <= bytes / 4 && (best == bp.free_blocks.size() || sz < bp.free_blocks[best].second))
+        best = i;
+    }
+    if (best < bp.free_blocks.size()) {
+      const auto blk = bp.free_blocks[best];
+      bp.free_blocks.erase(bp.free_blocks.begin() + best);
+      bp.held -= blk.second;
+      if (blk.second != bytes) bp.out.push_back(blk);
+      t_recycled = true;
+      return blk.first;
+    }
+  }
   void* p = mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_POPULATE, -1, 0);
   return p == MAP_FAILED ? nullptr : p;
 }
 
-void zeroed_free_bytes(void* p, size_t bytes) {
+void pooled_free_bytes(void* p, size_t bytes) {
   if (!p) return;
-  if (bytes < kMapThreshold) std::free(p);
-  else munmap(p, bytes);
+  if (bytes < kMapThreshold) {
+    std::free(p);
+    return;
+  }
+  block_pool& bp = blocks();
+  size_t size = bytes;
+  {
+    std::lock_guard<std::mutex> g(bp.mu);
+    for (size_t i = 0; i < bp.out.size(); i++)
+      if (bp.out[i].first == p) {
+        size = bp.out[i].second;
+        bp.out.erase(bp.out.begin() + i);
+        break;
+      }
+    if (bp.held + size <= kPoolCap) {
+      bp.free_blocks.push_back({p, size});
+      bp.held += size;
+      return;
+    }
+  }
+  munmap(p, size);
 }
 
 namespace {
@@ -132,7 +187,7 @@ void run_threads(int nt, F&& fn) {
 }
 
 struct cu_grid {
-  std::unique_ptr<int32_t[]> cells;
+  std::vector<int32_t, pooled_alloc<int32_t>> cells;  // every cell written by build
   uint32_t w = 0, h = 0;
   // CuGrid::build, src/deblock.rs:637-660: later records overwrite earlier
   // ones.  Row bands on nt threads, each visiting the records in order, so
@@ -140,10 +195,11 @@ struct cu_grid {
   void build(uint32_t pic_w, uint32_t pic_h, const vvclf_deblock_cu* cus, uint32_t n, int nt) {
     w = (pic_w + 3) / 4;
     h = (pic_h + 3) / 4;
-    cells.reset(new int32_t[(size_t)w * h]);
+    cells = std::vector<int32_t, pooled_alloc<int32_t>>();
+    cells.resize((size_t)w * h);
     run_threads(nt, [&](int ti) {
       const uint32_t yb = (uint32_t)((uint64_t)h * ti / nt), ye = (uint32_t)((uint64_t)h * (ti + 1) / nt);
-      std::fill(cells.get() + (size_t)yb * w, cells.get() + (size_t)ye * w, -1);
+      std::fill(cells.data() + (size_t)yb * w, cells.data() + (size_t)ye * w, -1);
       for (uint32_t i = 0; i < n; i++) {
         const vvclf_deblock_cu& c = cus[i];
         const uint32_t y0 = std::max(c.y / 4, yb), y1 = std::min(std::min((c.y + c.h + 3) / 4, h), ye);
@@ -154,22 +210,23 @@ struct cu_grid {
       }
     });
   }
-  walk::grid_view view() const { return walk::grid_view{cells.get(), w, h}; }
+  walk::grid_view view() const { return walk::grid_view{cells.data(), w, h}; }
 };
 
-// One direction's descriptors of one plane.  A fresh map holds zeros
-// without having written them; the walk writes only descriptors that filter
-// and records its claims in a bitmap of one bit per slot (a few hundred KB
+// One direction's descriptors of one plane.  The map holds zeros before the
+// walk, which writes only descriptors that filter and records its claims in a bitmap of one bit per slot (a few hundred KB
 // that stay in cache, where the map is megabytes).
 struct edge_map {
   map_vec e;
   std::vector<uint64_t> claims;
   uint32_t w = 0, h = 0;
+  bool dirty = false;  // recycled storage: derive_deblock_maps clears it
   void init(uint32_t w_, uint32_t h_) {
     w = w_;
     h = h_;
     e = map_vec();
     e.resize((size_t)w * h);
+    dirty = pooled_alloc_recycled();
     claims.assign(((size_t)w * h + 63) / 64, 0);
   }
   walk::map_view view() { return walk::map_view{e.data(), w, h}; }
@@ -312,8 +369,6 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
   edge_map lv, lh, cv[2], ch[2];
   t.gw = (W + 3) / 4 + 2;
   t.gh = (H + 3) / 4 + 2;
-  // the maps are allocated (zero pages populated by the kernel) together,
-  // one per thread, before the walk
   struct map_dims {
     edge_map* m;
     uint32_t w, h;
@@ -357,6 +412,9 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
     }
     t.has_chroma_dbk = true;
   }
+  // the maps' storage, one map per thread: new storage holds zeros (pages
+  // populated by the kernel); recycled storage is then cleared in equal
+  // shares of the recycled maps' concatenation, one per thread
   if (nt > 1) {
     std::atomic<int> next_map{0};
     run_threads(std::min(nt, n_maps), [&](int) {
@@ -365,6 +423,20 @@ int derive_deblock_maps(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t W
   } else {
     for (int k = 0; k < n_maps; k++) dims[k].m->init(dims[k].w, dims[k].h);
   }
+  size_t dirty = 0;
+  for (int k = 0; k < n_maps; k++)
+    if (dims[k].m->dirty) dirty += dims[k].m->e.size();
+  if (dirty)
+    run_threads(nt, [&](int ti) {
+      const size_t b = dirty * ti / nt, e = dirty * (ti + 1) / nt;
+      size_t base = 0;
+      for (int k = 0; k < n_maps; k++) {
+        if (!dims[k].m->dirty) continue;
+        const size_t n = dims[k].m->e.size(), lo = std::max(b, base), hi = std::min(e, base + n);
+        if (lo < hi) memset(dims[k].m->e.data() + (lo - base), 0, (hi - lo) * 4);
+        base += n;
+      }
+    });
   // Every thread walks its share of the records through all passes: a
   // record is read once for every pass over the same record array (luma and,
   // without a separate chroma tree, chroma), as the passes write separate

@@ -18,24 +18,30 @@ namespace vvclf {
 enum : uint32_t { STAGE_LMCS = 1, STAGE_DBK = 2, STAGE_SAO = 4, STAGE_ALF = 8, STAGE_ALL = 15 };
 
 // Host copies of the derived tables, before upload.
-// Zero-filled storage that does not write its zeros: blocks of 1 MB and more
-// are anonymous mappings populated by the kernel in one call (zero pages
-// without a fault per page), smaller ones come from calloc; elements are
+// Storage of the large per-picture maps, recycled between pictures: blocks
+// of 1 MB and more return to a process-wide free list (up to 512 MB) when
+// released and are handed out again for a request of the same or up to 25 %
+// smaller size, so that a picture size that repeats finds its memory mapped
+// and cached.  New blocks hold zeros (anonymous mappings populated by the
+// kernel in one call; calloc below 1 MB); a recycled block holds what its
+// last user left, and pooled_alloc_recycled() tells the allocating thread
+// which of the two its last allocation was.  Elements are
 // default-initialised.
-void* zeroed_alloc_bytes(size_t bytes);
-void zeroed_free_bytes(void* p, size_t bytes);
+void* pooled_alloc_bytes(size_t bytes);
+void pooled_free_bytes(void* p, size_t bytes);
+bool pooled_alloc_recycled();
 template <class T>
-struct zeroed_alloc {
+struct pooled_alloc {
   using value_type = T;
-  zeroed_alloc() = default;
+  pooled_alloc() = default;
   template <class U>
-  zeroed_alloc(const zeroed_alloc<U>&) {}
+  pooled_alloc(const pooled_alloc<U>&) {}
   T* allocate(size_t n) {
-    void* p = zeroed_alloc_bytes(n * sizeof(T));
+    void* p = pooled_alloc_bytes(n * sizeof(T));
     if (!p) throw std::bad_alloc();
     return static_cast<T*>(p);
   }
-  void deallocate(T* p, size_t n) { zeroed_free_bytes(p, n * sizeof(T)); }
+  void deallocate(T* p, size_t n) { pooled_free_bytes(p, n * sizeof(T)); }
   template <class U>
   void construct(U* p) noexcept {
     ::new ((void*)p) U;
@@ -44,11 +50,11 @@ struct zeroed_alloc {
   void construct(U* p, A&&... a) {
     ::new ((void*)p) U(std::forward<A>(a)...);
   }
-  bool operator==(const zeroed_alloc&) const { return true; }
-  bool operator!=(const zeroed_alloc&) const { return false; }
+  bool operator==(const pooled_alloc&) const { return true; }
+  bool operator!=(const pooled_alloc&) const { return false; }
 };
-// A deblocking descriptor map; a fresh one resized to n holds n zeros.
-using map_vec = std::vector<uint32_t, zeroed_alloc<uint32_t>>;
+// A deblocking descriptor map.
+using map_vec = std::vector<uint32_t, pooled_alloc<uint32_t>>;
 
 struct host_tables {
   // deblock
@@ -171,12 +177,15 @@ int derive_deblock_gpu(const vvclf_frame& f, uint32_t W, uint32_t H, uint32_t Wc
                        hipStream_t s, uint32_t n_out[4]);
 
 // Host planes <-> a job's contiguous sample set (luma, Cb, Cr at the device
-// pitches) through a pinned buffer: CPU row copies plus one linear DMA.
-// upload_staged returns after the DMA completed (the buffer may be reused);
+// pitches) through a pinned buffer: CPU row copies plus linear DMAs.
+// upload_staged returns after the DMAs completed (the buffer may be reused);
 // download_staged_issue only queues the DMA into `st`, download_staged_finish
-// copies the rows out once `s` has reached it.
+// copies the rows out once `s` has reached it; download_staged does both in
+// bands of rows (the row copies of one band overlapping the DMA of the next)
+// and returns with the caller's planes written.
 const uint16_t* job_final(const vvclf_job* j, int comp);  // the final sample set's plane
 hipError_t upload_staged(vvclf_job* j, const vvclf_picture& pic, pinned_buf& st, hipStream_t s);
+hipError_t download_staged(vvclf_job* j, pinned_buf& st, hipStream_t s, vvclf_picture& pic);
 hipError_t download_staged_issue(vvclf_job* j, pinned_buf& st, hipStream_t s);
 void download_staged_finish(const vvclf_job* j, const pinned_buf& st, vvclf_picture& pic);
 
