@@ -109,6 +109,42 @@ def test_stages_standalone(vv, orc):
         assert_same(fo.output_arrays(), fg.output_arrays(), "alf")
 
 
+@pytest.mark.parametrize("w,h", [(3840, 2160), (3848, 2168)])
+def test_stages_standalone_banded(vv, orc, w, h):
+    """The drop-in stage calls at sizes whose sample sets (16 MB and more) and
+    side tables move in per-thread bands: the LMCS call's luma-only set, and
+    a width whose rows are padded to the pitch, so that band borders fall
+    inside and between the planes."""
+    say(f"standalone stages {w}x{h}, banded transfers")
+    fo = vv.SynthFrame(1, w, h, seed=21, rich=1, tiles=1)
+    fg = vv.SynthFrame(1, w, h, seed=21, rich=1, tiles=1)
+    do, dg = fo.desc.contents, fg.desc.contents
+    orc.oracle_lmcs_inverse_luma(C.byref(do.pic.luma), C.byref(do.lmcs))
+    assert vv.lib.vvclf_lmcs_inverse_luma(C.byref(dg.pic.luma), C.byref(dg.lmcs)) == 0, vv.lib.vvclf_last_error()
+    assert_same(fo.output_arrays(), fg.output_arrays(), "lmcs")
+    orc.oracle_apply_deblocking_clipped(C.byref(do.pic), do.cus, do.n_cus, do.chroma_cus, do.n_chroma_cus,
+                                        do.mv_grid, C.byref(do.deblock), 1, do.no_filter_cols,
+                                        do.n_no_filter_cols, do.no_filter_rows, do.n_no_filter_rows)
+    rc = vv.lib.vvclf_apply_deblocking_clipped(C.byref(dg.pic), dg.cus, dg.n_cus, dg.chroma_cus, dg.n_chroma_cus,
+                                               dg.mv_grid, C.byref(dg.deblock), 1, dg.no_filter_cols,
+                                               dg.n_no_filter_cols, dg.no_filter_rows, dg.n_no_filter_rows)
+    assert rc == 0, vv.lib.vvclf_last_error()
+    assert_same(fo.output_arrays(), fg.output_arrays(), "deblock")
+    orc.oracle_apply_sao_clipped(C.byref(do.pic), C.byref(do.sao), C.byref(do.sao_cfg), do.clip_bounds)
+    assert vv.lib.vvclf_apply_sao_clipped(C.byref(dg.pic), C.byref(dg.sao), C.byref(dg.sao_cfg),
+                                          dg.clip_bounds) == 0, vv.lib.vvclf_last_error()
+    assert_same(fo.output_arrays(), fg.output_arrays(), "sao")
+    assert orc.oracle_apply_alf_clipped(C.byref(do.pic), C.byref(do.alf), C.byref(do.alf_cfg),
+                                        C.byref(do.alf_binding), do.clip_bounds) == 0
+    assert vv.lib.vvclf_apply_alf_clipped(C.byref(dg.pic), C.byref(dg.alf), C.byref(dg.alf_cfg),
+                                          C.byref(dg.alf_binding), dg.clip_bounds) == 0, vv.lib.vvclf_last_error()
+    assert_same(fo.output_arrays(), fg.output_arrays(), "alf")
+    # and the whole chain in one call on a fresh copy
+    fo.close()
+    fg.close()
+    full_chain(vv, orc, 1, w, h, 22, rich=1)
+
+
 @pytest.mark.parametrize("bd,above", [(10, False), (10, True), (8, True), (12, True)])
 def test_alf_unclipped_taps(vv, orc, bd, above):
     """ALF luma where the filter sets clip nothing (every fixed set; an APS
