@@ -846,16 +846,37 @@ __device__ __forceinline__ void alf_pk_filter_rows(const uint32_t (&Q)[N][6], co
         }
       }
     } else {
+      // Without a virtual boundary, the pair dy rows and dx / 2 pairs
+      // further on (even dx) has this pair's centre as its b sample and this
+      // pair's a samples as its centre: its clipped b difference is the
+      // negation of this pair's clipped a difference (the clip is
+      // symmetric), taken from da[] instead of recomputed.
+      const s16x2 cl = as_s16x2(clip);
+      const s16x2 hi = {cl.x, cl.x}, lo = {cl.y, cl.y};
+      s16x2 da[NR][2];
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         const int m = MT[VB][R0 + r];
         const int dy = kTapDy[k] < m ? kTapDy[k] : m;
+        const int dx = kTapDx[k];
         const int R = r + 3;  // centre row in Q
 #pragma unroll
         for (int h = 0; h < 2; h++) {
           const int C = 3 + 2 * h;
-          alf_pk_tap(c0, c1, clip, as_s16x2(pw_pair(Q, R, C)), pw_pair(Q, R + dy, C + kTapDx[k]),
-                     pw_pair(Q, R - dy, C - kTapDx[k]), acc0[r][h], acc1[r][h]);
+          const s16x2 cur = as_s16x2(pw_pair(Q, R, C));
+          da[r][h] = __builtin_elementwise_min(
+              __builtin_elementwise_max(as_s16x2(pw_pair(Q, R + dy, C + dx)) - cur, lo), hi);
+          const int rs = r - dy, hs = h - dx / 2;
+          s16x2 sum;
+          if (VB == 0 && (dx & 1) == 0 && (dy | dx) != 0 && rs >= 0 && hs >= 0 && hs < 2) {
+            sum = da[r][h] - da[rs][hs];
+          } else {
+            const s16x2 db = __builtin_elementwise_min(
+                __builtin_elementwise_max(as_s16x2(pw_pair(Q, R - dy, C - dx)) - cur, lo), hi);
+            sum = da[r][h] + db;
+          }
+          acc0[r][h] = __builtin_amdgcn_sdot2(sum, as_s16x2(c0), acc0[r][h], false);
+          acc1[r][h] = __builtin_amdgcn_sdot2(sum, as_s16x2(c1), acc1[r][h], false);
         }
       }
     }
