@@ -850,7 +850,8 @@ __device__ __forceinline__ void alf_pk_filter_rows(const uint32_t (&Q)[N][6], co
       // further on (even dx) has this pair's centre as its b sample and this
       // pair's a samples as its centre: its clipped b difference is the
       // negation of this pair's clipped a difference (the clip is
-      // symmetric), taken from da[] instead of recomputed.
+      // symmetric), taken from da[] instead of recomputed; for dx = +-1 the
+      // pair at column 4 + dx takes it from the halves of two pairs.
       const s16x2 cl = as_s16x2(clip);
       const s16x2 hi = {cl.x, cl.x}, lo = {cl.y, cl.y};
       s16x2 da[NR][2];
@@ -870,6 +871,10 @@ __device__ __forceinline__ void alf_pk_filter_rows(const uint32_t (&Q)[N][6], co
           s16x2 sum;
           if (VB == 0 && (dx & 1) == 0 && (dy | dx) != 0 && rs >= 0 && hs >= 0 && hs < 2) {
             sum = da[r][h] - da[rs][hs];
+          } else if (VB == 0 && (dx & 1) != 0 && 2 * h == dx + 1 && rs >= 0) {
+            // odd dx: the mirrored samples are columns 4 and 5 of row rs,
+            // the high half of pair 0 and the low half of pair 1
+            sum = da[r][h] - as_s16x2(__builtin_amdgcn_alignbit(as_u32(da[rs][1]), as_u32(da[rs][0]), 16));
           } else {
             const s16x2 db = __builtin_elementwise_min(
                 __builtin_elementwise_max(as_s16x2(pw_pair(Q, R - dy, C - dx)) - cur, lo), hi);
